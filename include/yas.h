@@ -1,0 +1,264 @@
+/*
+ * yas.h - C ABI of the MI355X-native RGBDFusion dense path.
+ *
+ * Every entry point replaces one method of the reference's dense-fusion
+ * classes (RGBDFusion/include/VoxelMap.h, RayTraceEngine.h, MeshEngine.h,
+ * RGBDTracking.h, ImageProc.h); the citation above each declaration names it.
+ * Signatures use plain pointers and sizes only: image arguments are device
+ * pointers with a row pitch in bytes (the reference passes cv::cuda::GpuMat),
+ * poses are 4x4 row-major double matrices (the reference passes Sophus::SE3d),
+ * intrinsics are {fx, fy, cx, cy}.
+ *
+ * Conventions kept from the reference:
+ *   - A pose "T_map_cam" maps camera coordinates into the volume's frame
+ *     (the reference's mTcw / Tcm hold this transform despite their names).
+ *   - Tracker results map the reference (last) frame into the current one;
+ *     the caller forms T_cur = T_last * DT^-1 (FullSystem/src/Tracking.cpp:186).
+ *   - Vertex maps are float4 per pixel, w = 1 valid / -1 invalid.
+ *
+ * Work is issued on the context's HIP stream.  Calls that return host values
+ * (counts, poses, status) synchronise that stream; the others are
+ * asynchronous.  Errors are returned as yas_status codes, never by exiting
+ * (the reference prints and calls exit(0), RGBDFusion/utils/CudaUtils.h:15-25).
+ * Handles are externally synchronised: one thread at a time per handle.
+ */
+#ifndef YAS_H
+#define YAS_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+  YAS_OK = 0,
+  YAS_E_INVALID = 1,        /* bad argument or handle */
+  YAS_E_HIP = 2,            /* HIP runtime error; see yas_last_error() */
+  YAS_E_OUT_OF_BLOCKS = 3,  /* voxel-block pool exhausted; blocks that fit were allocated */
+  YAS_E_HASH_FULL = 4,      /* probe limit reached or block key out of range */
+  YAS_E_DIVERGED = 5,       /* tracker produced a NaN update (RGBDTracking.cu:187-191) */
+  YAS_E_CAPACITY = 6,       /* caller buffer or engine capacity too small; count reported */
+  YAS_E_REJECTED = 7        /* frame rejected by the tracking gate (Tracking.cpp:178-184) */
+} yas_status;
+
+/* Voxel storage variants of the one TSDF update rule (VoxelMap.cu:764-783). */
+typedef enum {
+  YAS_VOXEL_I16_U8 = 0,   /* reference: int16 sdf * 32767, uint8 weight (VoxelMap.h:26-30) */
+  YAS_VOXEL_F32_F32 = 1,  /* float sdf, float weight */
+  YAS_VOXEL_F16_U16 = 2   /* half sdf, uint16 weight */
+} yas_voxel_format;
+
+typedef enum { YAS_TRACK_RGB = 0, YAS_TRACK_DEPTH = 1, YAS_TRACK_RGBD = 2 } yas_track_mode;
+
+typedef struct yas_ctx yas_ctx;
+typedef struct yas_map yas_map;
+typedef struct yas_raycaster yas_raycaster;
+typedef struct yas_mesher yas_mesher;
+typedef struct yas_tracker yas_tracker;
+typedef struct yas_pipeline yas_pipeline;
+
+typedef struct {
+  int num_buckets;      /* hash slots (reference: bucketSize/hashTableSize) */
+  int num_blocks;       /* voxel-block pool capacity (reference: voxelBlockSize) */
+  float voxel_size;     /* metres */
+  float trunc_dist;     /* metres (mu) */
+  int voxel_format;     /* yas_voxel_format */
+  int max_weight;       /* weight cap (reference: 255) */
+  float depth_min;      /* metres (reference: 0.1) */
+  float depth_max;      /* metres (reference: 3.0) */
+} yas_map_params;
+
+typedef struct {
+  int num_allocated;    /* blocks in the pool */
+  int num_visible;      /* visible blocks of the last fuse / visibility check */
+  int flags;            /* bit0 pool overflow, bit1 probe limit, bit2 key out of range */
+  int reserved;
+} yas_map_stats;
+
+typedef struct {
+  int levels;           /* pyramid levels, 1..5 (reference NUM_PYR 5, RGBDTracking.h:7) */
+  int iters[5];         /* Gauss-Newton iterations per level (reference {10,5,3,3,3}) */
+  int mode;             /* yas_track_mode */
+  float w_rgb_h, w_rgb_b, w_icp_h, w_icp_b; /* H = w_rgb_h H_rgb + w_icp_h H_icp, b likewise
+                                               (reference RGBD: 1, 1, 100, 10; RGBDTracking.cu:521-545) */
+} yas_tracker_params;
+
+typedef struct {
+  int good;             /* 0 after a NaN update (IsTrackingGood) */
+  int icp_inliers;      /* last ICP match count at level 0 */
+  int rgb_inliers;      /* last photometric residual count at level 0 */
+  float last_error;
+} yas_track_stats;
+
+/* ------------------------------------------------------------------ core */
+const char* yas_version(void);
+/* Last error text of this thread (set whenever a call returns != YAS_OK). */
+const char* yas_last_error(void);
+
+/* Context: device + HIP stream.  stream == NULL creates a library-owned
+ * stream; otherwise the caller's hipStream_t is used (not destroyed). */
+int yas_ctx_create(int device, void* stream, yas_ctx** out);
+int yas_ctx_destroy(yas_ctx* ctx);
+int yas_ctx_synchronize(yas_ctx* ctx);
+void* yas_ctx_stream(yas_ctx* ctx);
+
+/* ------------------------------------------------------------------ map
+ * MapStruct (RGBDFusion/include/VoxelMap.h:32-144). */
+
+/* MapStruct(K) + create(...) + Reset() (VoxelMap.cu:46-81, VoxelMap.h:35). */
+int yas_map_create(yas_ctx* ctx, const yas_map_params* params, const float K[4], yas_map** out);
+/* Reset() (VoxelMap.cu:46-58): empty hash, pool and voxels. */
+int yas_map_reset(yas_map* map);
+/* Release() (VoxelMap.cu:103-136). */
+int yas_map_destroy(yas_map* map);
+/* SetPose / GetPose (VoxelMap.cu:12-22). */
+int yas_map_set_pose(yas_map* map, const double T_world_map[16]);
+int yas_map_get_pose(yas_map* map, double T_world_map[16]);
+/* Fuse(GpuMat depth, SE3d Tcm) (VoxelMap.cu:825-884): block allocation along
+ * each ray (K3), visibility (K4), TSDF integration (K5).  depth is float
+ * metres, device memory, `pitch` bytes per row.  Asynchronous; capacity
+ * problems are reported by yas_map_get_stats flags. */
+int yas_map_fuse(yas_map* map, const float* depth, size_t pitch, int width, int height,
+                 const double T_map_cam[16]);
+/* Same, with uint16 depth in units of 1/depth_scale metres converted on the
+ * device (System.cpp:52 depth.convertTo followed by Fuse). */
+int yas_map_fuse_u16(yas_map* map, const uint16_t* depth, size_t pitch, int width, int height,
+                     float depth_scale, const double T_map_cam[16]);
+/* CheckNumVisibleBlocks(cols, rows, Tcm) (VoxelMap.cu:788-823). Synchronous. */
+int yas_map_check_visible_blocks(yas_map* map, int width, int height, const double T_map_cam[16],
+                                 int* count);
+/* Counters and capacity flags (GetNumVisibleBlocks, VoxelMap.cu:201-206). Synchronous. */
+int yas_map_get_stats(yas_map* map, yas_map_stats* stats);
+/* Copy every allocated block to host memory in pool order: keys as int32
+ * triples (block coordinates), voxels in the map's storage format
+ * (512 per block, index z*64 + y*8 + x).  *count receives the number of
+ * blocks; at most `capacity` are copied.  Synchronous.  No reference
+ * counterpart beyond Hibernate's device-to-host copy (VoxelMap.cu:894-923). */
+int yas_map_export(yas_map* map, int32_t* keys, void* voxels, int capacity, int* count);
+/* Bytes per voxel of the map's storage format. */
+int yas_map_voxel_bytes(yas_map* map);
+
+/* ------------------------------------------------------------------ raycast
+ * RayTraceEngine (RGBDFusion/include/RayTraceEngine.h:11-48). */
+int yas_raycaster_create(yas_ctx* ctx, int width, int height, const float K[4], yas_raycaster** out);
+int yas_raycaster_destroy(yas_raycaster* rc);
+/* RayTrace(MapStruct*, SE3d) (RayTraceEngine.cu:412-450): the vertex map of
+ * the volume seen from T_map_cam, in the camera frame. Asynchronous. */
+int yas_raycast(yas_raycaster* rc, yas_map* map, const double T_map_cam[16]);
+/* GetVMap() (RayTraceEngine.cu:452-454): device alias of the last result,
+ * float4 per pixel, overwritten by the next yas_raycast. */
+int yas_raycaster_vmap(yas_raycaster* rc, float** vmap, size_t* pitch);
+/* The 1/8-resolution [zmin, zmax] maps of the last ray cast (device, float). */
+int yas_raycaster_zrange(yas_raycaster* rc, float** zmin, float** zmax, int* cols, int* rows);
+/* GetNumVisibleBlock() (RayTraceEngine.cu:38-43). Synchronous. */
+int yas_raycaster_num_visible(yas_raycaster* rc, int* count);
+
+/* ------------------------------------------------------------------ mesh
+ * MeshEngine (RGBDFusion/include/MeshEngine.h:8-24). */
+int yas_mesher_create(yas_ctx* ctx, int max_triangles, yas_mesher** out);
+int yas_mesher_destroy(yas_mesher* me);
+/* Meshify(MapStruct*) (MeshEngine.cu:293-339): marching cubes over every
+ * allocated block into a triangle soup on the device.  Triangles of a block
+ * are contiguous, in voxel-index then table order.  *num_triangles gets the
+ * full count even when it exceeds the capacity (then YAS_E_CAPACITY).
+ * Synchronous. */
+int yas_mesh_extract(yas_mesher* me, yas_map* map, int* num_triangles);
+/* Download the last extraction: 9 floats (3 vertices) per triangle for
+ * vertices and for face normals (either may be NULL). Synchronous. */
+int yas_mesher_download(yas_mesher* me, float* vertices, float* normals, int capacity);
+/* Per-block detail of the last extraction, in pool order: block keys
+ * (int32 triples), first triangle index, triangle count and, when `cases`
+ * is not NULL, the cube index of every voxel (-1 where a corner is
+ * unobserved; 512 per block).  Synchronous. */
+int yas_mesher_blocks(yas_mesher* me, int32_t* keys, int32_t* first, int32_t* count, int16_t* cases,
+                      int capacity, int* num_blocks);
+
+/* ------------------------------------------------------------------ tracker
+ * RGBDTracking (RGBDFusion/include/RGBDTracking.h:9-107). */
+/* RGBDTracking(w, h, K, bRGB, bIcp) (RGBDTracking.cu:5-64). */
+int yas_tracker_create(yas_ctx* ctx, int width, int height, const double K[4],
+                       const yas_tracker_params* params, yas_tracker** out);
+int yas_tracker_destroy(yas_tracker* trk);
+/* SetReferenceImage / SetTrackingImage (RGBDTracking.cu:66-78,102-116):
+ * float intensity, device memory. */
+int yas_tracker_set_reference_image(yas_tracker* trk, const float* gray, size_t pitch);
+int yas_tracker_set_tracking_image(yas_tracker* trk, const float* gray, size_t pitch);
+/* SetReferenceDepth / SetTrackingDepth (RGBDTracking.cu:80-100,118-138):
+ * float depth in metres, device memory. */
+int yas_tracker_set_reference_depth(yas_tracker* trk, const float* depth, size_t pitch);
+int yas_tracker_set_tracking_depth(yas_tracker* trk, const float* depth, size_t pitch);
+/* SetReferenceModel(GpuMat vmap) (RGBDTracking.cu:140-150). */
+int yas_tracker_set_reference_model(yas_tracker* trk, const float* vmap, size_t pitch);
+/* GetTransform(init, bSwapBuffer) (RGBDTracking.cu:152-214). Synchronous. */
+int yas_tracker_get_transform(yas_tracker* trk, const double init[16], int swap_buffers,
+                              double out[16], yas_track_stats* stats);
+/* SwapFrameBuffer() (RGBDTracking.cu:294-303). */
+int yas_tracker_swap_frame_buffer(yas_tracker* trk);
+/* IsTrackingGood() (RGBDTracking.cu:553-556). */
+int yas_tracker_is_tracking_good(yas_tracker* trk, int* good);
+/* One linearisation (ComputeSingleStepDepth / ComputeSingleStepRGB,
+ * RGBDTracking.cu:408-441 / 225-292) at `level` and pose T: the 28
+ * upper-triangle entries of [J|r]^T [J|r] and the residual count.
+ * kind 0 = ICP, 1 = photometric.  stats (may be NULL) receives the
+ * photometric {sum r, count, mean, Huber threshold}. Synchronous. */
+int yas_tracker_step_sums(yas_tracker* trk, int level, const double T[16], int kind, float out[29],
+                          float stats[4]);
+/* Device pointer and pitch of one pyramid image (test access):
+ * which = 0 current intensity, 1 reference intensity, 2 gx, 3 gy,
+ * 4 current inverse depth, 5 reference inverse depth (float);
+ * 6 current vmap, 7 current nmap, 8 reference vmap, 9 reference nmap (float4). */
+int yas_tracker_level(yas_tracker* trk, int which, int level, void** ptr, size_t* pitch, int* width,
+                      int* height);
+
+/* ------------------------------------------------------------------ image ops
+ * Free functions of RGBDFusion/include/ImageProc.h:8-24 plus the host-side
+ * conversions of FullSystem/src/System.cpp:43-52, on the context's stream. */
+int yas_depth_u16_to_f32(yas_ctx* ctx, const uint16_t* src, size_t src_pitch, float* dst, size_t dst_pitch,
+                         int width, int height, float depth_scale);
+/* mode 0: 0.299R + 0.587G + 0.114B in float; mode 1: 8-bit fixed-point luma (OpenCV RGB2GRAY). */
+int yas_rgb_to_intensity(yas_ctx* ctx, const uint8_t* rgb, size_t src_pitch, float* dst, size_t dst_pitch,
+                         int width, int height, int mode);
+int yas_depth_to_invdepth(yas_ctx* ctx, const float* depth, size_t src_pitch, float* inv, size_t dst_pitch,
+                          int width, int height);                                  /* ImageProc.cu:141-164 */
+int yas_pyrdown_depth(yas_ctx* ctx, const float* src, size_t src_pitch, float* dst, size_t dst_pitch,
+                      int dst_width, int dst_height);                             /* ImageProc.cu:166-187 */
+int yas_pyrdown_gauss(yas_ctx* ctx, const float* src, size_t src_pitch, int src_width, int src_height,
+                      float* dst, size_t dst_pitch);                              /* cv::cuda::pyrDown */
+int yas_compute_vertex_map(yas_ctx* ctx, const float* invdepth, size_t src_pitch, float* vmap,
+                           size_t dst_pitch, int width, int height, const float K[4],
+                           float cutoff);                                         /* ImageProc.cu:189-220 */
+int yas_compute_normal_map(yas_ctx* ctx, const float* vmap, size_t src_pitch, float* nmap, size_t dst_pitch,
+                           int width, int height);                                /* ImageProc.cu:222-259 */
+int yas_image_gradient(yas_ctx* ctx, const float* src, size_t src_pitch, float* gx, float* gy,
+                       size_t dst_pitch, int width, int height);                  /* ImageProc.cu:7-40 */
+int yas_vmap_to_depth(yas_ctx* ctx, const float* vmap, size_t src_pitch, float* depth, size_t dst_pitch,
+                      int width, int height);                                     /* ImageProc.cu:262-285 */
+
+/* ------------------------------------------------------------------ pipeline
+ * The dense half of Tracking::TrackRGBD / StereoInitialization
+ * (FullSystem/src/Tracking.cpp:139-197) on one volume, with every pose kept
+ * on the device so that a frame issues no host synchronisation. */
+int yas_pipeline_create(yas_ctx* ctx, int width, int height, const float K[4], const yas_map_params* map,
+                        const yas_tracker_params* tracker, float depth_scale, int gray_mode,
+                        yas_pipeline** out);
+int yas_pipeline_destroy(yas_pipeline* pl);
+/* One frame from device-resident uint16 depth and packed RGB8: frame 0 is
+ * fused at the identity; later frames ray-cast at the last pose, track, gate
+ * |t| > 0.1 m, and fuse at T_last * DT^-1.  Asynchronous. */
+int yas_pipeline_step(yas_pipeline* pl, const uint16_t* depth, size_t depth_pitch, const uint8_t* rgb,
+                      size_t rgb_pitch);
+/* Pose of frame `frame` (camera to map) and whether it was tracked
+ * (1) or rejected (0).  Synchronous. */
+int yas_pipeline_pose(yas_pipeline* pl, int frame, double T[16], int* tracked);
+int yas_pipeline_num_frames(yas_pipeline* pl, int* n);
+int yas_pipeline_map(yas_pipeline* pl, yas_map** map);
+int yas_pipeline_raycaster(yas_pipeline* pl, yas_raycaster** rc);
+int yas_pipeline_tracker(yas_pipeline* pl, yas_tracker** trk);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* YAS_H */

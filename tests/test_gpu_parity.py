@@ -1,0 +1,248 @@
+"""GPU parity: every HIP kernel through the C ABI against the scalar oracle.
+
+Bar (see each test): block sets, TSDF storage, z-range maps, ray-cast vertex
+maps, image pyramids and marching-cubes cases/triangles are bit-exact; the
+tracker's reduced sums are within FP32 reduction-order tolerance of the
+oracle's exact sums, and poses within 1e-4 m / 1e-4 rad.
+"""
+import numpy as np
+import pytest
+import torch
+
+from oracle import pyoracle as po
+from yetanotherslam_amd import rgbdfusion as rf
+from yetanotherslam_amd import synth
+
+from helpers import assert_bits_equal, frames, pose_err, sums_tolerance, to_dev
+
+pytestmark = pytest.mark.gpu
+
+FMTS = [(rf.VOXEL_I16_U8, po.VOX_REF), (rf.VOXEL_F32_F32, po.VOX_F32), (rf.VOXEL_F16_U16, po.VOX_F16)]
+
+
+def fuse_both(fmt_pair, n=3, width=320, height=240, voxel=0.008, trunc=0.024, buckets=1 << 16, blocks=1 << 15):
+    gfmt, ofmt = fmt_pair
+    p, K, fr = frames(n, width, height)
+    gt = synth.relative_poses(p, n)
+    om = po.OracleMap(voxel, trunc, precision=32, voxfmt=ofmt)
+    gm = rf.MapStruct(K.astype(np.float32), buckets, blocks, voxel, trunc, voxel_format=gfmt)
+    for t in range(n):
+        z = po.depth_u16_to_f32(fr[t][0])
+        om.fuse(z, K, gt[t])
+        gm.Fuse(to_dev(z), gt[t])
+    torch.cuda.synchronize()
+    return p, K, fr, gt, om, gm
+
+
+@pytest.mark.parametrize("fmt", FMTS, ids=["i16u8", "f32", "f16u16"])
+def test_fuse_block_set_and_tsdf_bit_exact(fmt):
+    _, _, _, _, om, gm = fuse_both(fmt)
+    ko, so, wo = om.export()
+    kg, sg, wg = gm.export_sorted()
+    assert gm.stats().flags == 0
+    assert len(ko) > 100
+    np.testing.assert_array_equal(ko, kg)
+    assert_bits_equal(so, sg, "sdf")
+    assert_bits_equal(wo, wg, "weight")
+
+
+def test_fuse_c1_full_resolution_bit_exact():
+    """640x480, 8 mm voxels, truncation 3 voxels (the plain-method configuration)."""
+    _, _, _, _, om, gm = fuse_both(FMTS[1], n=2, width=640, height=480, buckets=1 << 20, blocks=1 << 17)
+    ko, so, wo = om.export()
+    kg, sg, wg = gm.export_sorted()
+    np.testing.assert_array_equal(ko, kg)
+    assert_bits_equal(so, sg, "sdf")
+    assert_bits_equal(wo, wg, "weight")
+
+
+def test_visible_block_count_matches():
+    p, K, fr, gt, om, gm = fuse_both(FMTS[1])
+    for T in (gt[0], gt[2]):
+        n_or = len(om.visible(p.width, p.height, K, T))
+        n_gpu = gm.CheckNumVisibleBlocks(p.width, p.height, T)
+        assert n_or == n_gpu
+
+
+def test_raycast_bit_exact():
+    p, K, fr, gt, om, gm = fuse_both(FMTS[1])
+    rc = rf.RayTraceEngine(p.width, p.height, K.astype(np.float32))
+    T = gt[2]
+    rc.RayTrace(gm, T)
+    vg = rc.GetVMap()
+    zmin_g, zmax_g = rc.GetZRange()
+    vo, zmin_o, zmax_o = om.raycast(p.width, p.height, K, T)
+    assert_bits_equal(zmin_o, zmin_g, "zmin")
+    assert_bits_equal(zmax_o, zmax_g, "zmax")
+    np.testing.assert_array_equal(vo[..., 3], vg[..., 3])
+    valid = vo[..., 3] > 0
+    assert valid.mean() > 0.3
+    assert_bits_equal(vo[valid][:, :3], vg[valid][:, :3], "vertex")
+
+
+def test_image_ops_bit_exact():
+    p, K, fr = frames(1)
+    d, rgb, _ = fr[0]
+    z = po.depth_u16_to_f32(d)
+    g = po.rgb_to_intensity(rgb)
+    ctx = rf.default_context()
+    lib = rf.native
+    W, H = p.width, p.height
+    # device conversions from raw frames
+    dz = torch.empty((H, W), dtype=torch.float32, device="cuda")
+    dg = torch.empty((H, W), dtype=torch.float32, device="cuda")
+    du = to_dev(d)
+    dc = to_dev(rgb)
+    lib.call("yas_depth_u16_to_f32", ctx.h, rf._dev(du), W * 2, rf._dev(dz), W * 4, W, H, 1000.0)
+    lib.call("yas_rgb_to_intensity", ctx.h, rf._dev(dc), W * 3, rf._dev(dg), W * 4, W, H, 0)
+    torch.cuda.synchronize()
+    assert_bits_equal(z, dz.cpu().numpy(), "depth u16->f32")
+    assert_bits_equal(g, dg.cpu().numpy(), "intensity")
+    # Gaussian pyramid level
+    out = torch.empty((H // 2, W // 2), dtype=torch.float32, device="cuda")
+    lib.call("yas_pyrdown_gauss", ctx.h, rf._dev(dg), W * 4, W, H, rf._dev(out), (W // 2) * 4)
+    torch.cuda.synchronize()
+    assert_bits_equal(po.pyrdown_gauss(g), out.cpu().numpy(), "pyrdown")
+    # gradients
+    gx = torch.empty((H, W), dtype=torch.float32, device="cuda")
+    gy = torch.empty_like(gx)
+    lib.call("yas_image_gradient", ctx.h, rf._dev(dg), W * 4, rf._dev(gx), rf._dev(gy), W * 4, W, H)
+    torch.cuda.synchronize()
+    ogx, ogy = po.gradient(g)
+    assert_bits_equal(ogx, gx.cpu().numpy(), "gx")
+    assert_bits_equal(ogy, gy.cpu().numpy(), "gy")
+    # inverse depth, vertex map, normal map
+    inv = torch.empty((H, W), dtype=torch.float32, device="cuda")
+    lib.call("yas_depth_to_invdepth", ctx.h, rf._dev(dz), W * 4, rf._dev(inv), W * 4, W, H)
+    vm = torch.empty((H, W, 4), dtype=torch.float32, device="cuda")
+    nm = torch.empty((H, W, 4), dtype=torch.float32, device="cuda")
+    Kf = K.astype(np.float32)
+    lib.call("yas_compute_vertex_map", ctx.h, rf._dev(inv), W * 4, rf._dev(vm), W * 16, W, H, rf._ptr(Kf), 3.0)
+    lib.call("yas_compute_normal_map", ctx.h, rf._dev(vm), W * 16, rf._dev(nm), W * 16, W, H)
+    torch.cuda.synchronize()
+    oinv = po.depth_to_invdepth(z)
+    assert_bits_equal(oinv, inv.cpu().numpy(), "invdepth")
+    invfx = float(np.float32(1.0 / np.float64(Kf[0])))
+    invfy = float(np.float32(1.0 / np.float64(Kf[1])))
+    ovm = po.vertex_map(oinv, invfx, invfy, Kf[2], Kf[3])
+    assert_bits_equal(ovm, vm.cpu().numpy(), "vertex map")
+    onm = po.normal_map(ovm.astype(np.float32))
+    assert_bits_equal(onm, nm.cpu().numpy(), "normal map")
+
+
+def tracker_pair(mode, levels=3, iters=(10, 5, 3), weights=(1.0, 1.0, 100.0, 10.0), width=320, height=240):
+    """Oracle and GPU trackers loaded with frames 0 (reference) and 1 (current);
+    the reference model is frame 0's own vertex map."""
+    p, K, fr = frames(2, width, height)
+    z0, z1 = (po.depth_u16_to_f32(fr[i][0]) for i in range(2))
+    g0, g1 = (po.rgb_to_intensity(fr[i][1]) for i in range(2))
+    Kf = K.astype(np.float32)
+    invfx = float(np.float32(1.0 / np.float64(Kf[0])))
+    invfy = float(np.float32(1.0 / np.float64(Kf[1])))
+    vm = po.vertex_map(po.depth_to_invdepth(z0), invfx, invfy, Kf[2], Kf[3]).astype(np.float32)
+    ot = po.OracleTracker(p.width, p.height, K, levels, iters, mode, weights)
+    gt_ = rf.RGBDTracking(p.width, p.height, K, levels, iters, mode, weights)
+    ot.set_reference_image(g0); ot.set_reference_depth(z0); ot.set_reference_model(vm)
+    ot.set_tracking_image(g1); ot.set_tracking_depth(z1)
+    gt_.SetReferenceImage(to_dev(g0)); gt_.SetReferenceDepth(to_dev(z0)); gt_.SetReferenceModel(to_dev(vm))
+    gt_.SetTrackingImage(to_dev(g1)); gt_.SetTrackingDepth(to_dev(z1))
+    torch.cuda.synchronize()
+    DT = np.linalg.inv(synth.pose(p, 1)) @ synth.pose(p, 0)
+    return p, K, ot, gt_, DT
+
+
+def test_tracker_pyramids_bit_exact():
+    p, K, ot, gt_, _ = tracker_pair(po.MODE_RGBD)
+    for lvl in range(3):
+        for which in range(10):
+            assert_bits_equal(ot.level(which, lvl), gt_.level(which, lvl), "buffer %d level %d" % (which, lvl))
+
+
+@pytest.mark.parametrize("kind", [0, 1], ids=["icp", "photometric"])
+def test_step_sums_within_reduction_tolerance(kind):
+    p, K, ot, gt_, DT = tracker_pair(po.MODE_RGBD)
+    for lvl in range(3):
+        for T in (np.eye(4), DT):
+            so, st_o = ot.step_sums(lvl, T, kind)
+            sg, st_g = gt_.step_sums(lvl, T, kind)
+            assert so[28] == sg[28], (lvl, so[28], sg[28])  # counts are exact
+            ok = sums_tolerance(sg, so)
+            assert ok.all(), (kind, lvl, np.argwhere(~ok).ravel(), sg[~ok], so[~ok])
+
+
+@pytest.mark.parametrize("mode", [po.MODE_DEPTH, po.MODE_RGB, po.MODE_RGBD], ids=["icp", "rgb", "rgbd"])
+def test_get_transform_matches_oracle(mode):
+    p, K, ot, gt_, DT = tracker_pair(mode)
+    To, good_o = ot.get_transform(swap=False)
+    Tg, st = gt_.GetTransform(swap=False)
+    assert good_o and st.good
+    et, er = pose_err(Tg, To)
+    assert et < 1e-4 and er < 1e-4, (et, er)
+
+
+@pytest.mark.parametrize("fmt", FMTS, ids=["i16u8", "f32", "f16u16"])
+def test_mesh_cases_and_triangles_bit_exact(fmt):
+    _, _, _, _, om, gm = fuse_both(fmt)
+    mo = om.mesh()
+    me = rf.MeshEngine(2_000_000)
+    mg = me.Meshify(gm, with_cases=True)
+    keys, counts, cases, tris, normals = rf.canonical_mesh(mg)
+    np.testing.assert_array_equal(mo["keys"], keys)
+    np.testing.assert_array_equal(mo["cases"], cases)
+    np.testing.assert_array_equal(mo["counts"], counts)
+    assert mg["num_triangles"] == len(mo["tris"]) > 1000
+    assert_bits_equal(mo["tris"], tris, "vertices")
+    assert_bits_equal(mo["normals"], normals, "normals")
+
+
+def test_pipeline_matches_oracle_pipeline():
+    """The dense half of Tracking::TrackRGBD over 6 frames: GPU poses within
+    1e-4 m / 1e-4 rad of the oracle's, same accept/reject decisions."""
+    n = 6
+    p, K, fr = frames(n)
+    vox, mu = 0.008, 0.024
+    ctx = rf.default_context()
+    mp = rf.map_params(1 << 16, 1 << 15, vox, mu, rf.VOXEL_F32_F32)
+    tp = rf.tracker_params(3, (10, 5, 3), rf.TRACK_RGBD, (1.0, 1.0, 100.0, 10.0))
+    pl = rf.DensePipeline(p.width, p.height, K, mp, tp, ctx=ctx)
+    op = po.OraclePipeline(p.width, p.height, K, vox, mu, 3, (10, 5, 3), po.MODE_RGBD)
+    for t in range(n):
+        d, rgb, _ = fr[t]
+        pl.step(to_dev(d), to_dev(rgb))
+        To, ok_o, _ = op.step(d, rgb)
+        Tg, ok_g = pl.pose(t)
+        assert ok_o == ok_g, t
+        et, er = pose_err(Tg, To)
+        assert et < 1e-4 and er < 1e-4, (t, et, er)
+
+
+def test_pool_exhaustion_reports_and_keeps_a_subset():
+    p, K, fr = frames(1)
+    z = po.depth_u16_to_f32(fr[0][0])
+    om = po.OracleMap(0.008, 0.024)
+    om.fuse(z, K, np.eye(4))
+    cap = 200
+    gm = rf.MapStruct(K.astype(np.float32), 1 << 14, cap, 0.008, 0.024)
+    gm.Fuse(to_dev(z), np.eye(4))
+    st = gm.stats()
+    assert st.num_allocated == cap
+    assert st.flags & 1
+    kg, _ = gm.export()
+    ko = {tuple(k) for k in om.export(with_voxels=False)}
+    assert {tuple(k) for k in kg} <= ko
+
+
+def test_empty_depth_frame_allocates_nothing_and_tracks_identity():
+    p, K, fr = frames(1)
+    W, H = p.width, p.height
+    gm = rf.MapStruct(K.astype(np.float32), 1 << 14, 1 << 12, 0.008, 0.024)
+    zeros = torch.zeros((H, W), dtype=torch.float32, device="cuda")
+    gm.Fuse(zeros, np.eye(4))
+    assert gm.stats().num_allocated == 0
+    trk = rf.RGBDTracking(W, H, K, 3, (10, 5, 3), rf.TRACK_DEPTH)
+    trk.SetReferenceDepth(zeros)
+    trk.SetReferenceModel(torch.zeros((H, W, 4), dtype=torch.float32, device="cuda"))
+    trk.SetTrackingDepth(zeros)
+    T, st = trk.GetTransform()
+    assert st.good
+    np.testing.assert_array_equal(T, np.eye(4))

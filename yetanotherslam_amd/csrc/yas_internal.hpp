@@ -1,0 +1,302 @@
+// Internal definitions shared by the HIP translation units of libyas.
+//
+// Arithmetic conventions (kept identical to the reference's kernels and
+// evaluated in the order written; the library is compiled with
+// -ffp-contract=off and IEEE division/sqrt, so each expression is one
+// rounding per operation):
+//   * rigid transform p' = R p + t, each row summed left to right;
+//   * projection u = fx * x / z + cx ((fx*x)/z then + cx);
+//   * float->int "round down" conversions saturate and map NaN to 0.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <string>
+
+#include "../../include/yas.h"
+
+// ------------------------------------------------------------------ errors
+void yas_set_error(const char* fmt, ...);
+
+#define YAS_HIP(expr)                                                                              \
+  do {                                                                                             \
+    hipError_t e_ = (expr);                                                                        \
+    if (e_ != hipSuccess) {                                                                        \
+      yas_set_error("%s failed: %s (%s:%d)", #expr, hipGetErrorString(e_), __FILE__, __LINE__);     \
+      return YAS_E_HIP;                                                                            \
+    }                                                                                              \
+  } while (0)
+
+#define YAS_LAUNCH_CHECK()                                                                         \
+  do {                                                                                             \
+    hipError_t e_ = hipGetLastError();                                                             \
+    if (e_ != hipSuccess) {                                                                        \
+      yas_set_error("kernel launch failed: %s (%s:%d)", hipGetErrorString(e_), __FILE__, __LINE__); \
+      return YAS_E_HIP;                                                                            \
+    }                                                                                              \
+  } while (0)
+
+#define YAS_REQUIRE(cond, msg)                                                                     \
+  do {                                                                                             \
+    if (!(cond)) {                                                                                 \
+      yas_set_error("%s", msg);                                                                    \
+      return YAS_E_INVALID;                                                                        \
+    }                                                                                              \
+  } while (0)
+
+struct yas_ctx {
+  int device;
+  hipStream_t stream;
+  bool own_stream;
+};
+
+namespace yas {
+
+constexpr uint64_t kEmptyKey = ~0ull;
+constexpr int kMaxProbe = 65536;
+constexpr int kBlockThreads = 256;
+
+enum : int { kFlagPoolOverflow = 1, kFlagProbeLimit = 2, kFlagKeyRange = 4 };
+// Map counters in device memory.
+enum : int { kCtrAlloc = 0, kCtrVisible = 1, kCtrFlags = 2, kCtrWalk = 3, kCtrCount = 8 };
+
+// ------------------------------------------------------------------ keys
+// Block coordinates are packed as three 21-bit two's-complement fields.
+__host__ __device__ inline bool key_in_range(int x, int y, int z) {
+  return x >= -(1 << 20) && x < (1 << 20) && y >= -(1 << 20) && y < (1 << 20) && z >= -(1 << 20) &&
+         z < (1 << 20);
+}
+__host__ __device__ inline uint64_t pack_key(int x, int y, int z) {
+  return ((uint64_t)(uint32_t)(x & 0x1FFFFF)) | ((uint64_t)(uint32_t)(y & 0x1FFFFF) << 21) |
+         ((uint64_t)(uint32_t)(z & 0x1FFFFF) << 42);
+}
+__host__ __device__ inline int sext21(uint64_t v) {
+  int x = (int)(v & 0x1FFFFF);
+  return (x & 0x100000) ? x - 0x200000 : x;
+}
+__host__ __device__ inline void unpack_key(uint64_t k, int& x, int& y, int& z) {
+  x = sext21(k);
+  y = sext21(k >> 21);
+  z = sext21(k >> 42);
+}
+// Reference spatial hash (VoxelStructUtils.h:16-21): int32 wrapping
+// multiplies, xor, remainder fixed into [0, n).  It picks the home slot.
+__host__ __device__ inline int ref_hash(int x, int y, int z, int n) {
+  uint32_t h = ((uint32_t)x * 73856093u) ^ ((uint32_t)y * 19349669u) ^ ((uint32_t)z * 83492791u);
+  int32_t r = (int32_t)h;
+  r %= n;
+  return r < 0 ? r + n : r;
+}
+// VoxelStructUtils.h:39-49 / :56-70.
+__host__ __device__ inline int vox_to_block(int v) {
+  if (v < 0) v -= 7;
+  return v / 8;
+}
+__host__ __device__ inline int vox_to_local(int v) {
+  int l = v % 8;
+  return l < 0 ? l + 8 : l;
+}
+
+// ------------------------------------------------------------------ conversions
+__device__ inline int floor_sat(float f) {
+  if (f != f) return 0;
+  f = floorf(f);
+  if (f <= -2147483648.0f) return INT32_MIN;
+  if (f >= 2147483648.0f) return INT32_MAX;
+  return (int)f;
+}
+__device__ inline int ceil_sat(float f) {
+  if (f != f) return 0;
+  f = ceilf(f);
+  if (f <= -2147483648.0f) return INT32_MIN;
+  if (f >= 2147483648.0f) return INT32_MAX;
+  return (int)f;
+}
+__device__ inline int trunc_sat(float f) {
+  if (f != f) return 0;
+  f = truncf(f);
+  if (f <= -2147483648.0f) return INT32_MIN;
+  if (f >= 2147483648.0f) return INT32_MAX;
+  return (int)f;
+}
+
+// ------------------------------------------------------------------ poses
+struct PoseF {
+  float r[9];
+  float t[3];
+};
+struct PosePair {
+  PoseF T;     // camera -> map
+  PoseF Tinv;  // map -> camera
+};
+struct Mat44 {
+  double m[16];
+};
+
+__host__ __device__ inline void xform(const PoseF& T, float x, float y, float z, float& ox, float& oy,
+                                      float& oz) {
+  ox = T.r[0] * x + T.r[1] * y + T.r[2] * z + T.t[0];
+  oy = T.r[3] * x + T.r[4] * y + T.r[5] * z + T.t[1];
+  oz = T.r[6] * x + T.r[7] * y + T.r[8] * z + T.t[2];
+}
+__host__ __device__ inline void rot3(const PoseF& T, float x, float y, float z, float& ox, float& oy,
+                                     float& oz) {
+  ox = T.r[0] * x + T.r[1] * y + T.r[2] * z;
+  oy = T.r[3] * x + T.r[4] * y + T.r[5] * z;
+  oz = T.r[6] * x + T.r[7] * y + T.r[8] * z;
+}
+// Double pose -> float pose, and its inverse formed in double
+// (R^T, -(R^T t) with each sum left to right) before rounding.
+__host__ __device__ inline void pose_pair_from(const double* T, PosePair& p) {
+  for (int i = 0; i < 3; ++i) {
+    for (int j = 0; j < 3; ++j) {
+      p.T.r[3 * i + j] = (float)T[4 * i + j];
+      p.Tinv.r[3 * i + j] = (float)T[4 * j + i];
+    }
+    p.T.t[i] = (float)T[4 * i + 3];
+    p.Tinv.t[i] = (float)(-(T[i] * T[3] + T[4 + i] * T[7] + T[8 + i] * T[11]));
+  }
+}
+__host__ __device__ inline void mul44(const double* A, const double* B, double* C) {
+  double t[16];
+  for (int i = 0; i < 4; ++i)
+    for (int j = 0; j < 4; ++j)
+      t[4 * i + j] = A[4 * i] * B[j] + A[4 * i + 1] * B[4 + j] + A[4 * i + 2] * B[8 + j] + A[4 * i + 3] * B[12 + j];
+  for (int i = 0; i < 16; ++i) C[i] = t[i];
+}
+__host__ __device__ inline void inv44(const double* T, double* o) {
+  double t[16];
+  for (int i = 0; i < 3; ++i) {
+    for (int j = 0; j < 3; ++j) t[4 * i + j] = T[4 * j + i];
+    t[4 * i + 3] = -(T[i] * T[3] + T[4 + i] * T[7] + T[8 + i] * T[11]);
+  }
+  t[12] = t[13] = t[14] = 0.0;
+  t[15] = 1.0;
+  for (int i = 0; i < 16; ++i) o[i] = t[i];
+}
+
+struct Intr {
+  float fx, fy, cx, cy, invfx, invfy;
+};
+// float copies of K and float(1/(double)f), as the reference derives them
+// (VoxelMap.cu:827-832, RayTraceEngine.cu:14-19).
+inline Intr make_intr(const float K[4]) {
+  Intr k;
+  k.fx = K[0];
+  k.fy = K[1];
+  k.cx = K[2];
+  k.cy = K[3];
+  k.invfx = (float)(1.0 / (double)k.fx);
+  k.invfy = (float)(1.0 / (double)k.fy);
+  return k;
+}
+
+// ------------------------------------------------------------------ voxels
+// Storage variants of the one update rule of VoxelMap.cu:764-783:
+// first observation -> (sdf, 1); else ((old*w + sdf)/(w+1), min(cap, w+1)).
+struct VoxI16 {  // int16 sdf (x32767, truncation) | uint8 weight << 16
+  typedef uint32_t T;
+  static constexpr int kFormat = YAS_VOXEL_I16_U8;
+  __device__ static inline float sdf(T v) { return (float)(int16_t)(v & 0xFFFFu) / 32767.0f; }
+  __device__ static inline bool observed(T v) { return ((v >> 16) & 0xFFu) != 0; }
+  __device__ static inline uint32_t pack(float f) {
+    int i = trunc_sat(f * 32767.0f);
+    i = i < -32768 ? -32768 : (i > 32767 ? 32767 : i);
+    return (uint32_t)(uint16_t)(int16_t)i;
+  }
+  __device__ static inline T update(T v, float s, int) {
+    int w = (int)((v >> 16) & 0xFFu);
+    if (w == 0) return pack(s) | (1u << 16);
+    float old = (float)(int16_t)(v & 0xFFFFu) / 32767.0f;
+    uint32_t ns = pack((old * (float)w + s) / (float)(w + 1));
+    int nw = w + 1 < 255 ? w + 1 : 255;
+    return ns | ((uint32_t)nw << 16);
+  }
+};
+struct VoxF32 {  // float sdf, float weight
+  typedef float2 T;
+  static constexpr int kFormat = YAS_VOXEL_F32_F32;
+  __device__ static inline float sdf(T v) { return v.x; }
+  __device__ static inline bool observed(T v) { return v.y != 0.0f; }
+  __device__ static inline T update(T v, float s, int maxw) {
+    float w = v.y;
+    if (w == 0.0f) return make_float2(s, 1.0f);
+    float ns = (v.x * w + s) / (w + 1.0f);
+    return make_float2(ns, fminf((float)maxw, w + 1.0f));
+  }
+};
+struct VoxF16 {  // half sdf | uint16 weight << 16
+  typedef uint32_t T;
+  static constexpr int kFormat = YAS_VOXEL_F16_U16;
+  __device__ static inline float h2f(uint32_t h) {
+    _Float16 x = __builtin_bit_cast(_Float16, (uint16_t)(h & 0xFFFFu));
+    return (float)x;
+  }
+  __device__ static inline uint32_t f2h(float f) { return (uint32_t)__builtin_bit_cast(uint16_t, (_Float16)f); }
+  __device__ static inline float sdf(T v) { return h2f(v); }
+  __device__ static inline bool observed(T v) { return (v >> 16) != 0; }
+  __device__ static inline T update(T v, float s, int maxw) {
+    int w = (int)(v >> 16);
+    if (w == 0) return f2h(s) | (1u << 16);
+    float old = h2f(v);
+    uint32_t ns = f2h((old * (float)w + s) / (float)(w + 1));
+    int nw = w + 1 < maxw ? w + 1 : maxw;
+    return ns | ((uint32_t)nw << 16);
+  }
+};
+
+inline int voxel_bytes(int fmt) { return fmt == YAS_VOXEL_F32_F32 ? 8 : 4; }
+
+// ------------------------------------------------------------------ map view
+// Device view of one volume: open-addressed hash (keys/vals), the block
+// pool (block_keys/voxels) and counters.
+struct MapView {
+  unsigned long long* keys;        // [num_buckets], kEmptyKey when free
+  int* vals;                       // [num_buckets], pool index or -1
+  unsigned long long* block_keys;  // [num_blocks]
+  void* voxels;                    // [num_blocks * 512] of the storage type
+  int* visible;                    // [num_blocks] pool indices of visible blocks
+  int* ctr;                        // counters (kCtr*)
+  int num_buckets;
+  int num_blocks;
+  float voxel_size;
+  float trunc;
+  float depth_min, depth_max;
+  int max_weight;
+  int format;
+};
+
+// Linear-probe lookup from the reference hash's home slot.
+__device__ inline int find_block(const MapView& m, uint64_t key, int home) {
+  int slot = home;
+  const int lim = m.num_buckets < kMaxProbe ? m.num_buckets : kMaxProbe;
+  for (int p = 0; p < lim; ++p) {
+    uint64_t k = m.keys[slot];
+    if (k == key) return m.vals[slot];
+    if (k == kEmptyKey) return -1;
+    slot = slot + 1 == m.num_buckets ? 0 : slot + 1;
+  }
+  return -1;
+}
+__device__ inline int find_block_xyz(const MapView& m, int bx, int by, int bz) {
+  if (!key_in_range(bx, by, bz)) return -1;
+  return find_block(m, pack_key(bx, by, bz), ref_hash(bx, by, bz, m.num_buckets));
+}
+
+// ------------------------------------------------------------------ wave64
+__device__ inline int lane_id() { return __lane_id(); }
+__device__ inline unsigned long long lane_mask_lt() {
+  int l = lane_id();
+  return l == 0 ? 0ull : (~0ull >> (64 - l));
+}
+
+// Wave-wide sum of one float (butterfly over the 64 lanes).
+__device__ inline float wave_sum(float v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+}  // namespace yas

@@ -1,0 +1,633 @@
+// Voxel-hashed TSDF volume on the device: MapStruct's allocation, visibility
+// and integration (RGBDFusion/src/VoxelMap.cu:24-58, 517-884) re-designed for
+// gfx950.
+//
+// Layout in HBM:
+//   keys[num_buckets]        u64 packed block coordinate, ~0 = free slot
+//   vals[num_buckets]        i32 pool index of the slot's block
+//   block_keys[num_blocks]   u64 key of each pool block (dense list of the
+//                            allocated blocks; visibility scans this list
+//                            instead of the whole table)
+//   voxels[num_blocks*512]   4 or 8 bytes per voxel, index z*64 + y*8 + x, so
+//                            one z-slice of a block is 64 consecutive voxels:
+//                            one coalesced 256/512-byte access per wavefront.
+//
+// The hash is open addressing with linear probing from the reference hash's
+// home slot.  Insertion is lock-free: a 64-bit compare-and-swap claims a free
+// slot for a key; within a wavefront, lanes carrying the same block key are
+// reduced to one leader with ballots before any memory traffic, and new
+// blocks take pool indices from one atomic per wavefront.  The resulting set
+// of allocated blocks does not depend on the order of insertion.
+#include <cstdarg>
+#include <cstring>
+#include <mutex>
+#include <vector>
+
+#include "yas_internal.hpp"
+
+using namespace yas;
+
+// ------------------------------------------------------------------ errors
+static thread_local std::string g_last_error;
+
+void yas_set_error(const char* fmt, ...) {
+  char buf[1024];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  g_last_error = buf;
+}
+
+struct yas_map {
+  yas_ctx* ctx;
+  yas_map_params P;
+  Intr K;
+  double pose[16];
+  std::mutex pose_mutex;
+  MapView v;
+  PosePair* pose_dev;  // pose of the current call
+  float* depth_f32;    // scratch for uint16 depth (lazily sized)
+  size_t depth_f32_bytes;
+  int* host_ctr;       // pinned mirror of the counters
+};
+
+extern "C" {
+
+const char* yas_version(void) { return "yas-mi355x 0.1.0 (gfx950)"; }
+const char* yas_last_error(void) { return g_last_error.c_str(); }
+
+int yas_ctx_create(int device, void* stream, yas_ctx** out) {
+  YAS_REQUIRE(out != nullptr, "yas_ctx_create: out is NULL");
+  int n = 0;
+  YAS_HIP(hipGetDeviceCount(&n));
+  YAS_REQUIRE(device >= 0 && device < n, "yas_ctx_create: no such device");
+  YAS_HIP(hipSetDevice(device));
+  yas_ctx* c = new yas_ctx();
+  c->device = device;
+  if (stream) {
+    c->stream = (hipStream_t)stream;
+    c->own_stream = false;
+  } else {
+    hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+      delete c;
+      yas_set_error("hipStreamCreate failed: %s", hipGetErrorString(e));
+      return YAS_E_HIP;
+    }
+    c->own_stream = true;
+  }
+  *out = c;
+  return YAS_OK;
+}
+
+int yas_ctx_destroy(yas_ctx* ctx) {
+  if (!ctx) return YAS_OK;
+  hipSetDevice(ctx->device);
+  hipStreamSynchronize(ctx->stream);
+  if (ctx->own_stream) hipStreamDestroy(ctx->stream);
+  delete ctx;
+  return YAS_OK;
+}
+
+int yas_ctx_synchronize(yas_ctx* ctx) {
+  YAS_REQUIRE(ctx, "null context");
+  YAS_HIP(hipStreamSynchronize(ctx->stream));
+  return YAS_OK;
+}
+
+void* yas_ctx_stream(yas_ctx* ctx) { return ctx ? (void*)ctx->stream : nullptr; }
+
+}  // extern "C"
+
+// ------------------------------------------------------------------ kernels
+
+__global__ void __launch_bounds__(256) k_reset_hash(unsigned long long* keys, int* vals, int n) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    keys[i] = kEmptyKey;
+    vals[i] = -1;
+  }
+}
+
+__global__ void __launch_bounds__(256) k_zero_u32(uint32_t* p, size_t n) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+    p[i] = 0u;
+}
+
+// Writes the float pose pair of a by-value double pose and clears the
+// visible-block counter for the call that follows.
+__global__ void k_pose_prep(Mat44 T, PosePair* out, int* ctr) {
+  if (threadIdx.x == 0) {
+    PosePair p;
+    pose_pair_from(T.m, p);
+    *out = p;
+    if (ctr) ctr[kCtrVisible] = 0;
+  }
+}
+
+// Same from a device-resident double pose (used by the pipeline).
+__global__ void k_pose_prep_dev(const double* T, PosePair* out, int* ctr) {
+  if (threadIdx.x == 0) {
+    PosePair p;
+    pose_pair_from(T, p);
+    *out = p;
+    if (ctr) ctr[kCtrVisible] = 0;
+  }
+}
+
+__global__ void __launch_bounds__(256) k_depth_u16(const uint16_t* src, size_t sp, float* dst, size_t dp, int W,
+                                                     int H, float scale) {
+  int x = blockIdx.x * 64 + (threadIdx.x & 63);
+  int y = blockIdx.y * 4 + (threadIdx.x >> 6);
+  if (x >= W || y >= H) return;
+  uint16_t d = ((const uint16_t*)((const char*)src + y * sp))[x];
+  ((float*)((char*)dst + y * dp))[x] = (float)d * scale;
+}
+
+// Insert the keys held by the lanes with `has`, one CAS per distinct key of
+// the wavefront; new blocks get consecutive pool indices from one atomic.
+__device__ inline void wave_insert(const MapView& m, uint64_t key, bool has, int home) {
+  bool pending = has;
+  bool leader = false;
+  for (;;) {
+    unsigned long long mask = __ballot(pending);
+    if (mask == 0ull) break;
+    int l = __ffsll((unsigned long long)mask) - 1;
+    unsigned long long lk = __shfl((unsigned long long)key, l);
+    if (lane_id() == l) leader = true;
+    if (pending && key == lk) pending = false;
+  }
+  bool won = false;
+  int slot = home;
+  if (leader) {
+    const int lim = m.num_buckets < kMaxProbe ? m.num_buckets : kMaxProbe;
+    int p = 0;
+    for (; p < lim; ++p) {
+      unsigned long long k = __hip_atomic_load(&m.keys[slot], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      if (k == key) break;
+      if (k == kEmptyKey) {
+        unsigned long long old = atomicCAS(&m.keys[slot], kEmptyKey, (unsigned long long)key);
+        if (old == kEmptyKey) {
+          won = true;
+          break;
+        }
+        if (old == key) break;
+      }
+      slot = slot + 1 == m.num_buckets ? 0 : slot + 1;
+    }
+    if (p == lim) atomicOr(&m.ctr[kCtrFlags], kFlagProbeLimit);
+  }
+  unsigned long long wm = __ballot(won);
+  if (wm) {
+    int first = __ffsll(wm) - 1;
+    int base = 0;
+    if (lane_id() == first) base = atomicAdd(&m.ctr[kCtrAlloc], __popcll(wm));
+    base = __shfl(base, first);
+    if (won) {
+      int idx = base + __popcll(wm & lane_mask_lt());
+      if (idx < m.num_blocks) {
+        m.block_keys[idx] = key;
+        m.vals[slot] = idx;
+      } else {
+        m.vals[slot] = -1;
+        atomicOr(&m.ctr[kCtrFlags], kFlagPoolOverflow);
+      }
+    }
+  }
+}
+
+// Pixel ray -> block (VoxelStructUtils.h:97-110, 28-49).
+__device__ inline void pixel_block(int x, int y, float z, const Intr& K, const PoseF& T, float vs, int& bx, int& by,
+                                   int& bz) {
+  float px = K.invfx * ((float)x - K.cx) * z;
+  float py = K.invfy * ((float)y - K.cy) * z;
+  float wx, wy, wz;
+  xform(T, px, py, z, wx, wy, wz);
+  bx = vox_to_block(floor_sat(wx / vs));
+  by = vox_to_block(floor_sat(wy / vs));
+  bz = vox_to_block(floor_sat(wz / vs));
+}
+
+// K3 (VoxelMap.cu:517-652): every block on the 3D Bresenham walk between the
+// blocks of depth-mu/2 and depth+mu/2 along each pixel ray.  16x16 pixels per
+// workgroup, so a wavefront covers a 16x4 patch whose rays mostly share
+// blocks.
+__global__ void __launch_bounds__(256) k_alloc(MapView m, const float* depth, size_t pitch, int W, int H, Intr K,
+                                               const PosePair* pose, const int* gate) {
+  if (gate && *gate == 0) return;
+  const int x = blockIdx.x * 16 + (threadIdx.x & 15);
+  const int y = blockIdx.y * 16 + (threadIdx.x >> 4);
+  const PoseF T = pose->T;
+  const float vs = m.voxel_size;
+  int px = 0, py = 0, pz = 0;          // current block
+  int dax = 0, day = 0, daz = 0;       // unit step along the dominant axis
+  int d1x = 0, d1y = 0, d1z = 0;       // unit step along the first other axis
+  int d2x = 0, d2y = 0, d2z = 0;       // unit step along the second other axis
+  int Ea = 0, E1 = 0, E2 = 0, err1 = 0, err2 = 0;
+  int remaining = 0;
+  if (x < W && y < H) {
+    float d = ((const float*)((const char*)depth + (size_t)y * pitch))[x];
+    const float half = m.trunc * 0.5f;
+    if (!(d != d || d < m.depth_min || d > m.depth_max)) {
+      float dn = fmaxf(m.depth_min, d - half);
+      float df = fminf(m.depth_max, d + half);
+      if (dn < df) {
+        int ex, ey, ez;
+        pixel_block(x, y, dn, K, T, vs, px, py, pz);
+        pixel_block(x, y, df, K, T, vs, ex, ey, ez);
+        int dx = ex - px, dy = ey - py, dz = ez - pz;
+        int ix = dx < 0 ? -1 : 1, iy = dy < 0 ? -1 : 1, iz = dz < 0 ? -1 : 1;
+        int ax = abs(dx), ay = abs(dy), az = abs(dz);
+        if (ax >= ay && ax >= az) {
+          dax = ix; d1y = iy; d2z = iz;
+          Ea = ax << 1; E1 = ay << 1; E2 = az << 1;
+          remaining = ax + 1;
+        } else if (ay >= ax && ay >= az) {
+          day = iy; d1x = ix; d2z = iz;
+          Ea = ay << 1; E1 = ax << 1; E2 = az << 1;
+          remaining = ay + 1;
+        } else {
+          // the reference's z branch walks y first, then x (VoxelMap.cu:628-647)
+          daz = iz; d1y = iy; d2x = ix;
+          Ea = az << 1; E1 = ay << 1; E2 = ax << 1;
+          remaining = az + 1;
+        }
+        err1 = E1 - 1;
+        err2 = E2 - 1;
+      }
+    }
+  }
+  for (;;) {
+    bool has = remaining > 0;
+    unsigned long long act = __ballot(has);
+    if (act == 0ull) break;
+    bool ok = has && key_in_range(px, py, pz);
+    if (has && !ok) atomicOr(&m.ctr[kCtrFlags], kFlagKeyRange);
+    uint64_t key = ok ? pack_key(px, py, pz) : 0ull;
+    int home = ok ? ref_hash(px, py, pz, m.num_buckets) : 0;
+    if (lane_id() == __ffsll(act) - 1) atomicAdd(&m.ctr[kCtrWalk], __popcll(act));
+    wave_insert(m, key, ok, home);
+    if (has) {
+      --remaining;
+      if (remaining > 0) {
+        if (err1 > 0) { px += d1x; py += d1y; pz += d1z; err1 -= Ea; }
+        if (err2 > 0) { px += d2x; py += d2y; pz += d2z; err2 -= Ea; }
+        err1 += E1;
+        err2 += E2;
+        px += dax; py += day; pz += daz;
+      }
+    }
+  }
+}
+
+// VoxelStructUtils.h:112-147.
+__device__ inline bool block_visible(int bx, int by, int bz, const PoseF& Ti, const Intr& K, int W, int H,
+                                     float vs, float dmin, float dmax) {
+  const float scale = vs * 8.0f;
+#pragma unroll
+  for (int c = 0; c < 8; ++c) {
+    float x = (float)(bx + (c & 1)) * scale;
+    float y = (float)(by + ((c >> 1) & 1)) * scale;
+    float z = (float)(bz + ((c >> 2) & 1)) * scale;
+    float qx, qy, qz;
+    xform(Ti, x, y, z, qx, qy, qz);
+    float u = K.fx * qx / qz + K.cx;
+    float v = K.fy * qy / qz + K.cy;
+    if (!(u < 0.0f || v < 0.0f || u > (float)(W - 1) || v > (float)(H - 1) || qz < dmin || qz > dmax)) return true;
+  }
+  return false;
+}
+
+// K4 (VoxelMap.cu:654-717): compact the visible blocks of the pool into
+// `visible` with one atomic per wavefront.
+__global__ void __launch_bounds__(256) k_visible(MapView m, const PosePair* pose, Intr K, int W, int H,
+                                                 const int* gate) {
+  if (gate && *gate == 0) return;
+  const int n0 = m.ctr[kCtrAlloc];
+  const int n = n0 < m.num_blocks ? n0 : m.num_blocks;
+  const PoseF Ti = pose->Tinv;
+  for (int base = blockIdx.x * blockDim.x; base < n; base += gridDim.x * blockDim.x) {
+    int i = base + threadIdx.x;
+    bool vis = false;
+    if (i < n) {
+      int bx, by, bz;
+      unpack_key(m.block_keys[i], bx, by, bz);
+      vis = block_visible(bx, by, bz, Ti, K, W, H, m.voxel_size, m.depth_min, m.depth_max);
+    }
+    unsigned long long mask = __ballot(vis);
+    if (mask) {
+      int first = __ffsll(mask) - 1;
+      int b = 0;
+      if (lane_id() == first) b = atomicAdd(&m.ctr[kCtrVisible], __popcll(mask));
+      b = __shfl(b, first);
+      const int slot = b + __popcll(mask & lane_mask_lt());
+      if (vis && slot < m.num_blocks) m.visible[slot] = i;
+    }
+  }
+}
+
+// K5 (VoxelMap.cu:719-786): one wavefront per visible block, lane = (x, y),
+// looping over the 8 z-slices; each slice is 64 consecutive voxels.
+template <class VT>
+__global__ void __launch_bounds__(256) k_integrate(MapView m, const PosePair* pose, Intr K, const float* depth,
+                                                   size_t pitch, int W, int H) {
+  const int nv = m.ctr[kCtrVisible];
+  const int nvis = nv < m.num_blocks ? nv : m.num_blocks;
+  const int lane = threadIdx.x & 63;
+  const int lx = lane & 7, ly = lane >> 3;
+  const int nwaves = gridDim.x * (blockDim.x >> 6);
+  const PoseF Ti = pose->Tinv;
+  const float vs = m.voxel_size, mu = m.trunc;
+  typename VT::T* vox = (typename VT::T*)m.voxels;
+  for (int w = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); w < nvis; w += nwaves) {
+    const int blk = m.visible[w];
+    int bx, by, bz;
+    unpack_key(m.block_keys[blk], bx, by, bz);
+    typename VT::T* bv = vox + (size_t)blk * 512;
+    const float wx = (float)(bx * 8 + lx) * vs;
+    const float wy = (float)(by * 8 + ly) * vs;
+#pragma unroll 2
+    for (int lz = 0; lz < 8; ++lz) {
+      const float wz = (float)(bz * 8 + lz) * vs;
+      float px, py, pz;
+      xform(Ti, wx, wy, wz, px, py, pz);
+      int u = floor_sat(K.fx * px / pz + K.cx + 0.5f);
+      int v = floor_sat(K.fy * py / pz + K.cy + 0.5f);
+      if (u < 0 || v < 0 || u > W - 1 || v > H - 1) continue;
+      float d = ((const float*)((const char*)depth + (size_t)v * pitch))[u];
+      if (d != d || d > m.depth_max || d < m.depth_min) continue;
+      float sdf = d - pz;
+      if (sdf < -mu) continue;
+      sdf = fminf(1.0f, sdf / mu);
+      const int li = lz * 64 + lane;
+      bv[li] = VT::update(bv[li], sdf, m.max_weight);
+    }
+  }
+}
+
+// ------------------------------------------------------------------ host side
+
+static int grid_for(long n, int per, int cap) {
+  long g = (n + per - 1) / per;
+  if (g < 1) g = 1;
+  if (g > cap) g = cap;
+  return (int)g;
+}
+
+static int launch_integrate(yas_map* map, const float* depth, size_t pitch, int W, int H) {
+  hipStream_t s = map->ctx->stream;
+  // enough waves to cover a large visible set; excess waves exit at once
+  int grid = grid_for(map->P.num_blocks, 4, 2048);
+  switch (map->P.voxel_format) {
+    case YAS_VOXEL_I16_U8:
+      hipLaunchKernelGGL(k_integrate<VoxI16>, dim3(grid), dim3(256), 0, s, map->v, map->pose_dev, map->K, depth, pitch, W, H);
+      break;
+    case YAS_VOXEL_F16_U16:
+      hipLaunchKernelGGL(k_integrate<VoxF16>, dim3(grid), dim3(256), 0, s, map->v, map->pose_dev, map->K, depth, pitch, W, H);
+      break;
+    default:
+      hipLaunchKernelGGL(k_integrate<VoxF32>, dim3(grid), dim3(256), 0, s, map->v, map->pose_dev, map->K, depth, pitch, W, H);
+  }
+  YAS_LAUNCH_CHECK();
+  return YAS_OK;
+}
+
+// Allocation + visibility + integration with the pose already in pose_dev.
+// A non-null `gate` pointing at 0 turns the call into a no-op on the device.
+int yas_map_fuse_prepared(yas_map* map, const float* depth, size_t pitch, int W, int H, const int* gate) {
+  hipStream_t s = map->ctx->stream;
+  dim3 g((W + 15) / 16, (H + 15) / 16);
+  hipLaunchKernelGGL(k_alloc, g, dim3(256), 0, s, map->v, depth, pitch, W, H, map->K, (const PosePair*)map->pose_dev,
+                     gate);
+  YAS_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_visible, dim3(grid_for(map->P.num_blocks, 256, 4096)), dim3(256), 0, s, map->v,
+                     (const PosePair*)map->pose_dev, map->K, W, H, gate);
+  YAS_LAUNCH_CHECK();
+  return launch_integrate(map, depth, pitch, W, H);
+}
+
+int yas_map_prepare_pose_dev(yas_map* map, const double* T_dev) {
+  hipLaunchKernelGGL(k_pose_prep_dev, dim3(1), dim3(64), 0, map->ctx->stream, T_dev, map->pose_dev, map->v.ctr);
+  YAS_LAUNCH_CHECK();
+  return YAS_OK;
+}
+
+static int prepare_pose(yas_map* map, const double T[16], bool reset_visible) {
+  Mat44 M;
+  std::memcpy(M.m, T, sizeof(M.m));
+  hipLaunchKernelGGL(k_pose_prep, dim3(1), dim3(64), 0, map->ctx->stream, M, map->pose_dev,
+                     reset_visible ? map->v.ctr : (int*)nullptr);
+  YAS_LAUNCH_CHECK();
+  return YAS_OK;
+}
+
+MapView yas_map_view(yas_map* map) { return map->v; }
+const PosePair* yas_map_pose_dev(yas_map* map) { return map->pose_dev; }
+Intr yas_map_intr(yas_map* map) { return map->K; }
+yas_ctx* yas_map_ctx(yas_map* map) { return map->ctx; }
+
+// Visibility only (used by the ray caster): clears and refills the visible list.
+int yas_map_visible_async(yas_map* map, const double T[16], int W, int H) {
+  int rc = prepare_pose(map, T, true);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_visible, dim3(grid_for(map->P.num_blocks, 256, 4096)), dim3(256), 0, map->ctx->stream,
+                     map->v, (const PosePair*)map->pose_dev, map->K, W, H, (const int*)nullptr);
+  YAS_LAUNCH_CHECK();
+  return YAS_OK;
+}
+int yas_map_visible_async_dev(yas_map* map, const double* T_dev, int W, int H) {
+  int rc = yas_map_prepare_pose_dev(map, T_dev);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_visible, dim3(grid_for(map->P.num_blocks, 256, 4096)), dim3(256), 0, map->ctx->stream,
+                     map->v, (const PosePair*)map->pose_dev, map->K, W, H, (const int*)nullptr);
+  YAS_LAUNCH_CHECK();
+  return YAS_OK;
+}
+
+int yas_convert_depth_u16(yas_ctx* ctx, const uint16_t* src, size_t sp, float* dst, size_t dp, int W, int H,
+                          float depth_scale) {
+  float s = (float)(1.0 / (double)depth_scale);
+  hipLaunchKernelGGL(k_depth_u16, dim3((W + 63) / 64, (H + 3) / 4), dim3(256), 0, ctx->stream, src, sp, dst, dp, W, H, s);
+  YAS_LAUNCH_CHECK();
+  return YAS_OK;
+}
+
+extern "C" {
+
+int yas_map_create(yas_ctx* ctx, const yas_map_params* P, const float K[4], yas_map** out) {
+  YAS_REQUIRE(ctx && P && K && out, "yas_map_create: null argument");
+  YAS_REQUIRE(P->num_buckets > 0 && P->num_blocks > 0, "yas_map_create: sizes must be positive");
+  YAS_REQUIRE(P->voxel_size > 0.f && P->trunc_dist > 0.f, "yas_map_create: voxel_size and trunc_dist must be > 0");
+  YAS_REQUIRE(P->voxel_format >= 0 && P->voxel_format <= 2, "yas_map_create: unknown voxel format");
+  YAS_REQUIRE(P->max_weight >= 1 && P->max_weight <= (P->voxel_format == YAS_VOXEL_I16_U8 ? 255 : 65535),
+              "yas_map_create: max_weight out of range for the voxel format");
+  YAS_HIP(hipSetDevice(ctx->device));
+  yas_map* m = new yas_map();
+  m->ctx = ctx;
+  m->P = *P;
+  m->K = make_intr(K);
+  for (int i = 0; i < 16; ++i) m->pose[i] = (i % 5 == 0) ? 1.0 : 0.0;
+  MapView& v = m->v;
+  std::memset(&v, 0, sizeof(v));
+  v.num_buckets = P->num_buckets;
+  v.num_blocks = P->num_blocks;
+  v.voxel_size = P->voxel_size;
+  v.trunc = P->trunc_dist;
+  v.depth_min = P->depth_min;
+  v.depth_max = P->depth_max;
+  v.max_weight = P->max_weight;
+  v.format = P->voxel_format;
+  size_t vb = (size_t)voxel_bytes(P->voxel_format);
+  hipError_t e = hipSuccess;
+  auto alloc = [&](void** p, size_t bytes) {
+    if (e == hipSuccess) e = hipMalloc(p, bytes);
+  };
+  alloc((void**)&v.keys, sizeof(unsigned long long) * P->num_buckets);
+  alloc((void**)&v.vals, sizeof(int) * P->num_buckets);
+  alloc((void**)&v.block_keys, sizeof(unsigned long long) * P->num_blocks);
+  alloc((void**)&v.voxels, vb * 512 * (size_t)P->num_blocks);
+  alloc((void**)&v.visible, sizeof(int) * P->num_blocks);
+  alloc((void**)&v.ctr, sizeof(int) * kCtrCount);
+  alloc((void**)&m->pose_dev, sizeof(PosePair));
+  if (e == hipSuccess) e = hipHostMalloc((void**)&m->host_ctr, sizeof(int) * kCtrCount, hipHostMallocDefault);
+  if (e != hipSuccess) {
+    yas_set_error("yas_map_create: allocation failed: %s", hipGetErrorString(e));
+    yas_map_destroy(m);
+    return YAS_E_HIP;
+  }
+  int rc = yas_map_reset(m);
+  if (rc) {
+    yas_map_destroy(m);
+    return rc;
+  }
+  *out = m;
+  return YAS_OK;
+}
+
+int yas_map_reset(yas_map* m) {
+  YAS_REQUIRE(m, "null map");
+  hipStream_t s = m->ctx->stream;
+  hipLaunchKernelGGL(k_reset_hash, dim3(grid_for(m->P.num_buckets, 256, 8192)), dim3(256), 0, s, m->v.keys, m->v.vals,
+                     m->P.num_buckets);
+  YAS_LAUNCH_CHECK();
+  size_t words = (size_t)voxel_bytes(m->P.voxel_format) * 512 * (size_t)m->P.num_blocks / 4;
+  hipLaunchKernelGGL(k_zero_u32, dim3(grid_for((long)(words / 4), 256, 16384)), dim3(256), 0, s, (uint32_t*)m->v.voxels,
+                     words);
+  hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(256), 0, s, (uint32_t*)m->v.ctr, (size_t)kCtrCount);
+  YAS_LAUNCH_CHECK();
+  return YAS_OK;
+}
+
+int yas_map_destroy(yas_map* m) {
+  if (!m) return YAS_OK;
+  hipSetDevice(m->ctx->device);
+  hipStreamSynchronize(m->ctx->stream);
+  hipFree(m->v.keys);
+  hipFree(m->v.vals);
+  hipFree(m->v.block_keys);
+  hipFree(m->v.voxels);
+  hipFree(m->v.visible);
+  hipFree(m->v.ctr);
+  hipFree(m->pose_dev);
+  hipFree(m->depth_f32);
+  if (m->host_ctr) hipHostFree(m->host_ctr);
+  delete m;
+  return YAS_OK;
+}
+
+int yas_map_set_pose(yas_map* m, const double T[16]) {
+  YAS_REQUIRE(m && T, "null argument");
+  std::lock_guard<std::mutex> lk(m->pose_mutex);
+  std::memcpy(m->pose, T, sizeof(m->pose));
+  return YAS_OK;
+}
+
+int yas_map_get_pose(yas_map* m, double T[16]) {
+  YAS_REQUIRE(m && T, "null argument");
+  std::lock_guard<std::mutex> lk(m->pose_mutex);
+  std::memcpy(T, m->pose, sizeof(m->pose));
+  return YAS_OK;
+}
+
+int yas_map_fuse(yas_map* m, const float* depth, size_t pitch, int W, int H, const double T[16]) {
+  YAS_REQUIRE(m && depth && T, "yas_map_fuse: null argument");
+  YAS_REQUIRE(W > 0 && H > 0 && pitch >= (size_t)W * sizeof(float), "yas_map_fuse: bad image size");
+  YAS_HIP(hipSetDevice(m->ctx->device));
+  int rc = prepare_pose(m, T, true);
+  if (rc) return rc;
+  return yas_map_fuse_prepared(m, depth, pitch, W, H, nullptr);
+}
+
+int yas_map_fuse_u16(yas_map* m, const uint16_t* depth, size_t pitch, int W, int H, float depth_scale,
+                     const double T[16]) {
+  YAS_REQUIRE(m && depth && T, "yas_map_fuse_u16: null argument");
+  YAS_REQUIRE(W > 0 && H > 0 && pitch >= (size_t)W * sizeof(uint16_t), "yas_map_fuse_u16: bad image size");
+  YAS_REQUIRE(depth_scale > 0.f, "yas_map_fuse_u16: depth_scale must be > 0");
+  YAS_HIP(hipSetDevice(m->ctx->device));
+  size_t need = (size_t)W * H * sizeof(float);
+  if (m->depth_f32_bytes < need) {
+    YAS_HIP(hipStreamSynchronize(m->ctx->stream));
+    hipFree(m->depth_f32);
+    m->depth_f32 = nullptr;
+    m->depth_f32_bytes = 0;
+    YAS_HIP(hipMalloc((void**)&m->depth_f32, need));
+    m->depth_f32_bytes = need;
+  }
+  int rc = yas_convert_depth_u16(m->ctx, depth, pitch, m->depth_f32, (size_t)W * sizeof(float), W, H, depth_scale);
+  if (rc) return rc;
+  return yas_map_fuse(m, m->depth_f32, (size_t)W * sizeof(float), W, H, T);
+}
+
+int yas_map_check_visible_blocks(yas_map* m, int W, int H, const double T[16], int* count) {
+  YAS_REQUIRE(m && T && count, "null argument");
+  YAS_HIP(hipSetDevice(m->ctx->device));
+  int rc = yas_map_visible_async(m, T, W, H);
+  if (rc) return rc;
+  YAS_HIP(hipMemcpyAsync(m->host_ctr, m->v.ctr, sizeof(int) * kCtrCount, hipMemcpyDeviceToHost, m->ctx->stream));
+  YAS_HIP(hipStreamSynchronize(m->ctx->stream));
+  *count = m->host_ctr[kCtrVisible];
+  return YAS_OK;
+}
+
+int yas_map_get_stats(yas_map* m, yas_map_stats* st) {
+  YAS_REQUIRE(m && st, "null argument");
+  YAS_HIP(hipSetDevice(m->ctx->device));
+  YAS_HIP(hipMemcpyAsync(m->host_ctr, m->v.ctr, sizeof(int) * kCtrCount, hipMemcpyDeviceToHost, m->ctx->stream));
+  YAS_HIP(hipStreamSynchronize(m->ctx->stream));
+  int a = m->host_ctr[kCtrAlloc];
+  st->num_allocated = a < m->P.num_blocks ? a : m->P.num_blocks;
+  st->num_visible = m->host_ctr[kCtrVisible];
+  st->flags = m->host_ctr[kCtrFlags];
+  st->reserved = m->host_ctr[kCtrWalk];
+  return YAS_OK;
+}
+
+int yas_map_voxel_bytes(yas_map* m) { return m ? voxel_bytes(m->P.voxel_format) : 0; }
+
+int yas_map_export(yas_map* m, int32_t* keys, void* voxels, int capacity, int* count) {
+  YAS_REQUIRE(m && count, "null argument");
+  yas_map_stats st;
+  int rc = yas_map_get_stats(m, &st);
+  if (rc) return rc;
+  *count = st.num_allocated;
+  int n = st.num_allocated < capacity ? st.num_allocated : capacity;
+  if (n <= 0) return YAS_OK;
+  if (keys) {
+    std::vector<unsigned long long> k(n);
+    YAS_HIP(hipMemcpy(k.data(), m->v.block_keys, sizeof(unsigned long long) * n, hipMemcpyDeviceToHost));
+    for (int i = 0; i < n; ++i) unpack_key(k[i], keys[3 * i], keys[3 * i + 1], keys[3 * i + 2]);
+  }
+  if (voxels) {
+    size_t bytes = (size_t)voxel_bytes(m->P.voxel_format) * 512 * (size_t)n;
+    YAS_HIP(hipMemcpy(voxels, m->v.voxels, bytes, hipMemcpyDeviceToHost));
+  }
+  return st.num_allocated > capacity ? YAS_E_CAPACITY : YAS_OK;
+}
+
+int yas_depth_u16_to_f32(yas_ctx* ctx, const uint16_t* src, size_t sp, float* dst, size_t dp, int W, int H,
+                         float depth_scale) {
+  YAS_REQUIRE(ctx && src && dst && W > 0 && H > 0 && depth_scale > 0.f, "yas_depth_u16_to_f32: bad argument");
+  return yas_convert_depth_u16(ctx, src, sp, dst, dp, W, H, depth_scale);
+}
+
+}  // extern "C"

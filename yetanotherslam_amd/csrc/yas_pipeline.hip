@@ -1,0 +1,226 @@
+// The dense half of the reference's per-frame tracking step
+// (FullSystem/src/Tracking.cpp:139-197, StereoInitialization + TrackRGBD) on
+// one volume, with every pose kept in device memory: ray cast at the last
+// pose, track against the ray-cast model, gate |t| > 0.1 m, compose
+// T_cur = T_last * DT^-1 and fuse at T_cur.  A frame enqueues kernels only;
+// the host never waits on the device between frames.
+#include <cstring>
+#include <vector>
+
+#include "yas_internal.hpp"
+
+using namespace yas;
+
+int yas_map_fuse_prepared(yas_map* map, const float* depth, size_t pitch, int W, int H, const int* gate);
+int yas_map_prepare_pose_dev(yas_map* map, const double* T_dev);
+int yas_convert_depth_u16(yas_ctx* ctx, const uint16_t* src, size_t sp, float* dst, size_t dp, int W, int H,
+                          float depth_scale);
+int yas_raycast_dev(yas_raycaster* rc, yas_map* map, const double* T_dev);
+float4* yas_raycaster_vmap_dev(yas_raycaster* rc);
+int yas_tracker_solve_async(yas_tracker* t);
+int yas_tracker_init_identity_async(yas_tracker* t);
+const double* yas_tracker_result_dev(yas_tracker* t);
+const int* yas_tracker_good_dev(yas_tracker* t);
+int yas_tracker_set_tracking_image_dev(yas_tracker* t, const float* g, size_t pitch);
+int yas_tracker_set_tracking_depth_dev(yas_tracker* t, const float* d, size_t pitch);
+int yas_rgb_to_gray_async(yas_ctx* ctx, const uint8_t* rgb, size_t sp, float* dst, size_t dp, int W, int H, int mode);
+
+namespace {
+
+// Tracking.cpp:178-187: reject |t| > 0.1 m (or a NaN update), otherwise
+// T_last <- T_last * DT^-1.  Records the frame's pose and outcome and sets
+// the gate that lets the fusion kernels run.
+__global__ void k_post_track(const double* DT, const int* good, double* T_last, double* pose_out, int* tracked,
+                             int* gate) {
+  if (threadIdx.x != 0) return;
+  double tn = sqrt(DT[3] * DT[3] + DT[7] * DT[7] + DT[11] * DT[11]);
+  int ok = (*good != 0) && !(tn > 0.1);
+  if (ok) {
+    double Di[16];
+    inv44(DT, Di);
+    mul44(T_last, Di, T_last);
+  }
+  for (int i = 0; i < 16; ++i) pose_out[i] = T_last[i];
+  *tracked = ok;
+  *gate = ok;
+}
+
+__global__ void k_record_first(const double* T_last, double* pose_out, int* tracked) {
+  if (threadIdx.x != 0) return;
+  for (int i = 0; i < 16; ++i) pose_out[i] = T_last[i];
+  *tracked = 1;
+}
+
+__global__ void k_set_identity(double* T) {
+  if (threadIdx.x < 16) T[threadIdx.x] = (threadIdx.x % 5 == 0) ? 1.0 : 0.0;
+}
+
+}  // namespace
+
+struct yas_pipeline {
+  yas_ctx* ctx;
+  int W, H;
+  float depth_scale;
+  int gray_mode;
+  yas_map* map;
+  yas_raycaster* rc;
+  yas_tracker* trk;
+  float* depth;
+  float* gray;
+  double* T_last;
+  double* poses;
+  int* tracked;
+  int* gate;
+  int frames;
+  int cap;
+};
+
+static int grow(yas_pipeline* p) {
+  int ncap = p->cap ? p->cap * 2 : 1024;
+  double* np = nullptr;
+  int* nt = nullptr;
+  YAS_HIP(hipMalloc((void**)&np, sizeof(double) * 16 * (size_t)ncap));
+  YAS_HIP(hipMalloc((void**)&nt, sizeof(int) * (size_t)ncap));
+  if (p->cap) {
+    YAS_HIP(hipMemcpyAsync(np, p->poses, sizeof(double) * 16 * p->cap, hipMemcpyDeviceToDevice, p->ctx->stream));
+    YAS_HIP(hipMemcpyAsync(nt, p->tracked, sizeof(int) * p->cap, hipMemcpyDeviceToDevice, p->ctx->stream));
+    YAS_HIP(hipStreamSynchronize(p->ctx->stream));
+    hipFree(p->poses);
+    hipFree(p->tracked);
+  }
+  p->poses = np;
+  p->tracked = nt;
+  p->cap = ncap;
+  return YAS_OK;
+}
+
+extern "C" {
+
+int yas_pipeline_create(yas_ctx* ctx, int W, int H, const float K[4], const yas_map_params* mp,
+                        const yas_tracker_params* tp, float depth_scale, int gray_mode, yas_pipeline** out) {
+  YAS_REQUIRE(ctx && K && mp && tp && out, "yas_pipeline_create: null argument");
+  YAS_REQUIRE(depth_scale > 0.f, "yas_pipeline_create: depth_scale must be > 0");
+  YAS_HIP(hipSetDevice(ctx->device));
+  yas_pipeline* p = new yas_pipeline();
+  std::memset(p, 0, sizeof(*p));
+  p->ctx = ctx;
+  p->W = W;
+  p->H = H;
+  p->depth_scale = depth_scale;
+  p->gray_mode = gray_mode;
+  double Kd[4] = {K[0], K[1], K[2], K[3]};
+  int rc = yas_map_create(ctx, mp, K, &p->map);
+  if (!rc) rc = yas_raycaster_create(ctx, W, H, K, &p->rc);
+  if (!rc) rc = yas_tracker_create(ctx, W, H, Kd, tp, &p->trk);
+  if (rc) {
+    yas_pipeline_destroy(p);
+    return rc;
+  }
+  hipError_t e = hipMalloc((void**)&p->depth, sizeof(float) * W * H);
+  if (e == hipSuccess) e = hipMalloc((void**)&p->gray, sizeof(float) * W * H);
+  if (e == hipSuccess) e = hipMalloc((void**)&p->T_last, sizeof(double) * 16);
+  if (e == hipSuccess) e = hipMalloc((void**)&p->gate, sizeof(int));
+  if (e != hipSuccess) {
+    yas_set_error("yas_pipeline_create: %s", hipGetErrorString(e));
+    yas_pipeline_destroy(p);
+    return YAS_E_HIP;
+  }
+  if ((rc = grow(p))) {
+    yas_pipeline_destroy(p);
+    return rc;
+  }
+  hipLaunchKernelGGL(k_set_identity, dim3(1), dim3(64), 0, ctx->stream, p->T_last);
+  YAS_LAUNCH_CHECK();
+  *out = p;
+  return YAS_OK;
+}
+
+int yas_pipeline_destroy(yas_pipeline* p) {
+  if (!p) return YAS_OK;
+  hipSetDevice(p->ctx->device);
+  hipStreamSynchronize(p->ctx->stream);
+  yas_map_destroy(p->map);
+  yas_raycaster_destroy(p->rc);
+  yas_tracker_destroy(p->trk);
+  hipFree(p->depth);
+  hipFree(p->gray);
+  hipFree(p->T_last);
+  hipFree(p->poses);
+  hipFree(p->tracked);
+  hipFree(p->gate);
+  delete p;
+  return YAS_OK;
+}
+
+int yas_pipeline_step(yas_pipeline* p, const uint16_t* depth, size_t dpitch, const uint8_t* rgb, size_t cpitch) {
+  YAS_REQUIRE(p && depth && rgb, "yas_pipeline_step: null argument");
+  YAS_HIP(hipSetDevice(p->ctx->device));
+  int rc;
+  if (p->frames >= p->cap && (rc = grow(p))) return rc;
+  const int W = p->W, H = p->H;
+  const size_t fp = (size_t)W * sizeof(float);
+  if ((rc = yas_convert_depth_u16(p->ctx, depth, dpitch, p->depth, fp, W, H, p->depth_scale))) return rc;
+  if ((rc = yas_rgb_to_gray_async(p->ctx, rgb, cpitch, p->gray, fp, W, H, p->gray_mode))) return rc;
+  hipStream_t s = p->ctx->stream;
+  double* pose_out = p->poses + 16 * (size_t)p->frames;
+  int* tracked_out = p->tracked + p->frames;
+  if (p->frames == 0) {
+    // StereoInitialization (Tracking.cpp:139-155)
+    if ((rc = yas_tracker_set_reference_image(p->trk, p->gray, fp))) return rc;
+    if ((rc = yas_tracker_set_reference_depth(p->trk, p->depth, fp))) return rc;
+    if ((rc = yas_map_prepare_pose_dev(p->map, p->T_last))) return rc;
+    if ((rc = yas_map_fuse_prepared(p->map, p->depth, fp, W, H, nullptr))) return rc;
+    hipLaunchKernelGGL(k_record_first, dim3(1), dim3(64), 0, s, (const double*)p->T_last, pose_out, tracked_out);
+    YAS_LAUNCH_CHECK();
+  } else {
+    // TrackRGBD (Tracking.cpp:162-197)
+    if ((rc = yas_raycast_dev(p->rc, p->map, p->T_last))) return rc;
+    if ((rc = yas_tracker_set_reference_model(p->trk, (const float*)yas_raycaster_vmap_dev(p->rc), (size_t)W * 16)))
+      return rc;
+    if ((rc = yas_tracker_set_tracking_image_dev(p->trk, p->gray, fp))) return rc;
+    if ((rc = yas_tracker_set_tracking_depth_dev(p->trk, p->depth, fp))) return rc;
+    if ((rc = yas_tracker_init_identity_async(p->trk))) return rc;
+    if ((rc = yas_tracker_solve_async(p->trk))) return rc;
+    yas_tracker_swap_frame_buffer(p->trk);
+    hipLaunchKernelGGL(k_post_track, dim3(1), dim3(64), 0, s, yas_tracker_result_dev(p->trk), yas_tracker_good_dev(p->trk),
+                       p->T_last, pose_out, tracked_out, p->gate);
+    YAS_LAUNCH_CHECK();
+    if ((rc = yas_map_prepare_pose_dev(p->map, p->T_last))) return rc;
+    if ((rc = yas_map_fuse_prepared(p->map, p->depth, fp, W, H, p->gate))) return rc;
+  }
+  p->frames++;
+  return YAS_OK;
+}
+
+int yas_pipeline_pose(yas_pipeline* p, int frame, double T[16], int* tracked) {
+  YAS_REQUIRE(p && T, "null argument");
+  YAS_REQUIRE(frame >= 0 && frame < p->frames, "yas_pipeline_pose: no such frame");
+  YAS_HIP(hipSetDevice(p->ctx->device));
+  YAS_HIP(hipStreamSynchronize(p->ctx->stream));
+  YAS_HIP(hipMemcpy(T, p->poses + 16 * (size_t)frame, sizeof(double) * 16, hipMemcpyDeviceToHost));
+  if (tracked) YAS_HIP(hipMemcpy(tracked, p->tracked + frame, sizeof(int), hipMemcpyDeviceToHost));
+  return YAS_OK;
+}
+
+int yas_pipeline_num_frames(yas_pipeline* p, int* n) {
+  YAS_REQUIRE(p && n, "null argument");
+  *n = p->frames;
+  return YAS_OK;
+}
+int yas_pipeline_map(yas_pipeline* p, yas_map** m) {
+  YAS_REQUIRE(p && m, "null argument");
+  *m = p->map;
+  return YAS_OK;
+}
+int yas_pipeline_raycaster(yas_pipeline* p, yas_raycaster** r) {
+  YAS_REQUIRE(p && r, "null argument");
+  *r = p->rc;
+  return YAS_OK;
+}
+int yas_pipeline_tracker(yas_pipeline* p, yas_tracker** t) {
+  YAS_REQUIRE(p && t, "null argument");
+  *t = p->trk;
+  return YAS_OK;
+}
+
+}  // extern "C"

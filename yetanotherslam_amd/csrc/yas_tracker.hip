@@ -1,0 +1,962 @@
+// Dense RGB-D tracker: the reference's RGBDTracking
+// (RGBDFusion/src/RGBDTracking.cu, RGBDFusion/include/TrackingUtils.h) and the
+// image operations it uses (RGBDFusion/src/ImageProc.cu).
+//
+// MI355X design: the whole Gauss-Newton loop stays on the device.  Each
+// linearisation is one grid-stride kernel whose workgroups reduce their
+// per-pixel products (28 upper-triangle entries of [J|r]^T[J|r] plus a count)
+// through wavefront butterflies and LDS; the last workgroup to finish (atomic
+// ticket) folds the per-workgroup partials in a fixed order, so results are
+// deterministic.  A one-workgroup kernel then solves the 6x6 system in FP64
+// (the reference's Eigen LDLT, restated), applies T <- exp(xi) T in FP64 and
+// keeps the best pose, all in device memory.  The host only enqueues kernels:
+// no per-iteration download (the reference blocks on ~4 downloads per
+// iteration, RGBDTracking.cu:252-288,434-437).
+#include <cfloat>
+#include <cmath>
+#include <cstring>
+
+#include "yas_internal.hpp"
+
+using namespace yas;
+
+namespace {
+
+constexpr int kRedGroups = 256;  // workgroups of a reduction pass
+constexpr int kMaxLevels = 5;
+
+struct Level {
+  int w, h;
+  float fx, fy, cx, cy, invfx, invfy;
+  double Kd[4];
+};
+
+struct TrackState {
+  double est[16];
+  double last_ok[16];
+  float sums_rgb[32];
+  float sums_icp[32];
+  float rgb_stats[4];  // sum r, count, mean, Huber threshold
+  float last_err;
+  int good;
+  int icp_inliers;
+  int rgb_inliers;
+  unsigned ticket[4];
+};
+
+// ------------------------------------------------------------------ reductions
+// Sum N floats over the workgroup; the result is valid in thread 0.
+template <int N>
+__device__ inline void block_sum(float (&v)[N], float* lds /* [4 * N] */) {
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+  for (int i = 0; i < N; ++i) v[i] = wave_sum(v[i]);
+  if (lane == 0)
+#pragma unroll
+    for (int i = 0; i < N; ++i) lds[wid * N + i] = v[i];
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const int nw = blockDim.x >> 6;
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+      float s = lds[i];
+      for (int w = 1; w < nw; ++w) s += lds[w * N + i];
+      v[i] = s;
+    }
+  }
+  __syncthreads();
+}
+
+// Store this workgroup's N sums; the last workgroup to arrive folds all
+// partials (row r = workgroup r) and returns true with the totals in `out`
+// (thread 0).  The ticket is re-armed for the next pass.
+template <int N>
+__device__ inline bool grid_sum(float (&v)[N], float* partials, unsigned* ticket, float* lds, float (&out)[N]) {
+  __shared__ bool last;
+  block_sum<N>(v, lds);
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int i = 0; i < N; ++i) partials[blockIdx.x * 32 + i] = v[i];
+    __threadfence();
+    unsigned t = atomicAdd(ticket, 1u);
+    last = (t == gridDim.x - 1);
+  }
+  __syncthreads();
+  if (!last) return false;
+  __threadfence();
+  float r[N];
+#pragma unroll
+  for (int i = 0; i < N; ++i) r[i] = 0.0f;
+  for (int g = threadIdx.x; g < (int)gridDim.x; g += blockDim.x)
+#pragma unroll
+    for (int i = 0; i < N; ++i)
+      r[i] += __hip_atomic_load(&partials[g * 32 + i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  block_sum<N>(r, lds);
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int i = 0; i < N; ++i) out[i] = r[i];
+    *ticket = 0u;
+  }
+  return true;
+}
+
+__device__ inline void load_pose_f(const double* T, float* lds_pose /* 12 */) {
+  if (threadIdx.x < 12) {
+    int i = threadIdx.x / 4, j = threadIdx.x % 4;
+    int k = j < 3 ? 3 * i + j : 9 + i;
+    lds_pose[k] = (float)T[4 * i + j];
+  }
+  __syncthreads();
+}
+
+// ------------------------------------------------------------------ image ops
+__device__ inline float ld(const float* p, size_t pitch, int x, int y) {
+  return ((const float*)((const char*)p + (size_t)y * pitch))[x];
+}
+
+__global__ void __launch_bounds__(256) k_copy_f32(const float* src, size_t sp, float* dst, int W, int H) {
+  int x = blockIdx.x * 64 + (threadIdx.x & 63), y = blockIdx.y * 4 + (threadIdx.x >> 6);
+  if (x < W && y < H) dst[(size_t)y * W + x] = ld(src, sp, x, y);
+}
+
+// mode 0: 0.299R + 0.587G + 0.114B (float); mode 1: 8-bit fixed point.
+__global__ void __launch_bounds__(256) k_rgb_to_gray(const uint8_t* rgb, size_t sp, float* dst, size_t dp, int W,
+                                                     int H, int mode) {
+  int x = blockIdx.x * 64 + (threadIdx.x & 63), y = blockIdx.y * 4 + (threadIdx.x >> 6);
+  if (x >= W || y >= H) return;
+  const uint8_t* p = rgb + (size_t)y * sp + 3 * x;
+  int r = p[0], g = p[1], b = p[2];
+  float v = mode == 0 ? 0.299f * (float)r + 0.587f * (float)g + 0.114f * (float)b
+                      : (float)((r * 4899 + g * 9617 + b * 1868 + (1 << 13)) >> 14);
+  ((float*)((char*)dst + (size_t)y * dp))[x] = v;
+}
+
+// OpenCV pyrDown restated: binomial [1 4 6 4 1]/16 per axis (vertical sums of
+// the 5 source columns first, then horizontal), BORDER_REFLECT_101.
+__device__ inline int reflect101(int i, int n) {
+  if (i < 0) i = -i;
+  if (i >= n) i = 2 * n - 2 - i;
+  return i;
+}
+__global__ void __launch_bounds__(256) k_pyrdown_gauss(const float* src, size_t sp, int sw, int sh, float* dst,
+                                                       size_t dp, int dw, int dh) {
+  int x = blockIdx.x * 16 + (threadIdx.x & 15), y = blockIdx.y * 16 + (threadIdx.x >> 4);
+  if (x >= dw || y >= dh) return;
+  const float k[5] = {0.0625f, 0.25f, 0.375f, 0.25f, 0.0625f};
+  int rows[5];
+#pragma unroll
+  for (int i = 0; i < 5; ++i) rows[i] = reflect101(2 * y + i - 2, sh);
+  float col[5];
+#pragma unroll
+  for (int j = 0; j < 5; ++j) {
+    int sx = reflect101(2 * x + j - 2, sw);
+    float acc = k[0] * ld(src, sp, sx, rows[0]);
+#pragma unroll
+    for (int i = 1; i < 5; ++i) acc = acc + k[i] * ld(src, sp, sx, rows[i]);
+    col[j] = acc;
+  }
+  float acc = k[0] * col[0];
+#pragma unroll
+  for (int j = 1; j < 5; ++j) acc = acc + k[j] * col[j];
+  ((float*)((char*)dst + (size_t)y * dp))[x] = acc;
+}
+
+// ImageProc.cu:7-25.
+__global__ void __launch_bounds__(256) k_gradient(const float* s, size_t sp, float* gx, float* gy, size_t dp, int W,
+                                                  int H) {
+  int x = blockIdx.x * 64 + (threadIdx.x & 63), y = blockIdx.y * 4 + (threadIdx.x >> 6);
+  if (x >= W || y >= H) return;
+  float a = 0.0f, b = 0.0f;
+  if (!(x < 1 || y < 1 || x > W - 2 || y > H - 2)) {
+    a = (ld(s, sp, x + 1, y) - ld(s, sp, x - 1, y)) * 0.5f;
+    b = (ld(s, sp, x, y + 1) - ld(s, sp, x, y - 1)) * 0.5f;
+  }
+  ((float*)((char*)gx + (size_t)y * dp))[x] = a;
+  ((float*)((char*)gy + (size_t)y * dp))[x] = b;
+}
+
+// ImageProc.cu:141-153.
+__global__ void __launch_bounds__(256) k_invdepth(const float* z, size_t sp, float* o, size_t dp, int W, int H) {
+  int x = blockIdx.x * 64 + (threadIdx.x & 63), y = blockIdx.y * 4 + (threadIdx.x >> 6);
+  if (x >= W || y >= H) return;
+  float v = ld(z, sp, x, y);
+  ((float*)((char*)o + (size_t)y * dp))[x] = (v > 0.2f && v < 8.0f) ? 1.0f / v : 0.0f;
+}
+
+// ImageProc.cu:166-175, last row/column (left unwritten there) set to 0.
+__global__ void __launch_bounds__(256) k_pyrdown_depth(const float* s, size_t sp, float* o, size_t dp, int W, int H) {
+  int x = blockIdx.x * 64 + (threadIdx.x & 63), y = blockIdx.y * 4 + (threadIdx.x >> 6);
+  if (x >= W || y >= H) return;
+  float v = (x < W - 1 && y < H - 1) ? ld(s, sp, 2 * x, 2 * y) : 0.0f;
+  ((float*)((char*)o + (size_t)y * dp))[x] = v;
+}
+
+// Whole-float4 subsampling of the reference model (departure from the
+// reference's float kernel run over float4 data, RGBDTracking.cu:146).
+__global__ void __launch_bounds__(256) k_pyrdown_vmap(const float4* s, size_t sp, float4* o, size_t dp, int W, int H) {
+  int x = blockIdx.x * 64 + (threadIdx.x & 63), y = blockIdx.y * 4 + (threadIdx.x >> 6);
+  if (x >= W || y >= H) return;
+  float4 v = make_float4(0.f, 0.f, 0.f, -1.f);
+  if (x < W - 1 && y < H - 1) v = ((const float4*)((const char*)s + (size_t)(2 * y) * sp))[2 * x];
+  ((float4*)((char*)o + (size_t)y * dp))[x] = v;
+}
+
+__global__ void __launch_bounds__(256) k_copy_f4(const float4* s, size_t sp, float4* o, int W, int H) {
+  int x = blockIdx.x * 64 + (threadIdx.x & 63), y = blockIdx.y * 4 + (threadIdx.x >> 6);
+  if (x < W && y < H) o[(size_t)y * W + x] = ((const float4*)((const char*)s + (size_t)y * sp))[x];
+}
+
+// ImageProc.cu:189-209.
+__global__ void __launch_bounds__(256) k_vertex(const float* inv, size_t sp, float4* v, size_t dp, int W, int H,
+                                                float invfx, float invfy, float cx, float cy, float cut) {
+  int x = blockIdx.x * 64 + (threadIdx.x & 63), y = blockIdx.y * 4 + (threadIdx.x >> 6);
+  if (x >= W || y >= H) return;
+  float iz = ld(inv, sp, x, y);
+  float z = 1.0f / iz;
+  float4 o = make_float4(0.f, 0.f, 0.f, -1.f);
+  if (iz > 0.0f && z < cut) o = make_float4(z * ((float)x - cx) * invfx, z * ((float)y - cy) * invfy, z, 1.0f);
+  ((float4*)((char*)v + (size_t)y * dp))[x] = o;
+}
+
+// ImageProc.cu:222-249.
+__global__ void __launch_bounds__(256) k_normal(const float4* v, size_t sp, float4* n, size_t dp, int W, int H) {
+  int x = blockIdx.x * 64 + (threadIdx.x & 63), y = blockIdx.y * 4 + (threadIdx.x >> 6);
+  if (x >= W || y >= H) return;
+  float4 o = make_float4(0.f, 0.f, 0.f, -1.f);
+  if (x != W - 1 && y != H - 1) {
+    const float4 a = ((const float4*)((const char*)v + (size_t)y * sp))[x];
+    const float4 b = ((const float4*)((const char*)v + (size_t)y * sp))[x + 1];
+    const float4 c = ((const float4*)((const char*)v + (size_t)(y + 1) * sp))[x];
+    if (a.w > 0.f && b.w > 0.f && c.w > 0.f) {
+      float p0 = c.x - a.x, p1 = c.y - a.y, p2 = c.z - a.z;
+      float q0 = b.x - a.x, q1 = b.y - a.y, q2 = b.z - a.z;
+      float r0 = p1 * q2 - p2 * q1, r1 = p2 * q0 - p0 * q2, r2 = p0 * q1 - p1 * q0;
+      float n2 = r0 * r0 + r1 * r1 + r2 * r2;
+      if (n2 > 0.0f) {
+        float nn = sqrtf(n2);
+        r0 = r0 / nn; r1 = r1 / nn; r2 = r2 / nn;
+      }
+      o = make_float4(r0, r1, r2, 1.0f);
+    }
+  }
+  ((float4*)((char*)n + (size_t)y * dp))[x] = o;
+}
+
+// ImageProc.cu:262-275.
+__global__ void __launch_bounds__(256) k_vmap_to_depth(const float4* v, size_t sp, float* d, size_t dp, int W, int H) {
+  int x = blockIdx.x * 64 + (threadIdx.x & 63), y = blockIdx.y * 4 + (threadIdx.x >> 6);
+  if (x >= W || y >= H) return;
+  float4 a = ((const float4*)((const char*)v + (size_t)y * sp))[x];
+  ((float*)((char*)d + (size_t)y * dp))[x] = a.w > 0.f ? a.z : 0.0f;
+}
+
+// ------------------------------------------------------------------ ICP
+// RGBDTracking.cu:305-406: projective association of each reference vertex
+// into the current frame, gated at sin(20 deg) normal angle and 1 cm.
+__global__ void __launch_bounds__(256) k_icp(Level L, const float4* ref_v, const float4* ref_n, const float4* cur_v,
+                                             const float4* cur_n, TrackState* st, float* partials, float angle_th,
+                                             float dist_th, int record_inliers) {
+  __shared__ float pose[12];
+  __shared__ float lds[4 * 29];
+  load_pose_f(st->est, pose);
+  PoseF T;
+#pragma unroll
+  for (int i = 0; i < 9; ++i) T.r[i] = pose[i];
+  T.t[0] = pose[9]; T.t[1] = pose[10]; T.t[2] = pose[11];
+  float acc[29];
+#pragma unroll
+  for (int i = 0; i < 29; ++i) acc[i] = 0.0f;
+  const int N = L.w * L.h;
+  for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < N; k += gridDim.x * blockDim.x) {
+    const float4 vl = ref_v[k];
+    if (!(vl.w >= 0.0f)) continue;
+    float vx, vy, vz;
+    xform(T, vl.x, vl.y, vl.z, vx, vy, vz);
+    float invz = 1.0f / vz;
+    int u = floor_sat(L.fx * vx * invz + L.cx + 0.5f);
+    int v = floor_sat(L.fy * vy * invz + L.cy + 0.5f);
+    if (u < 0 || v < 0 || u >= L.w || v >= L.h) continue;
+    const float4 vc = cur_v[v * L.w + u];
+    if (vc.w < 0.0f) continue;
+    const float4 nl = ref_n[k];
+    float nx, ny, nz;
+    rot3(T, nl.x, nl.y, nl.z, nx, ny, nz);
+    const float4 nc = cur_n[v * L.w + u];
+    float dx = vx - vc.x, dy = vy - vc.y, dz = vz - vc.z;
+    float dist = sqrtf(dx * dx + dy * dy + dz * dz);
+    float c0 = nc.y * nz - nc.z * ny, c1 = nc.z * nx - nc.x * nz, c2 = nc.x * ny - nc.y * nx;
+    float ang = sqrtf(c0 * c0 + c1 * c1 + c2 * c2);
+    if (!(ang < angle_th && dist <= dist_th && nl.w > 0.0f && nc.w > 0.0f)) continue;
+    float row[7];
+    row[0] = nx; row[1] = ny; row[2] = nz;
+    row[3] = vy * nz - vz * ny;
+    row[4] = vz * nx - vx * nz;
+    row[5] = vx * ny - vy * nx;
+    row[6] = nx * (vc.x - vx) + ny * (vc.y - vy) + nz * (vc.z - vz);
+    int c = 0;
+#pragma unroll
+    for (int i = 0; i < 7; ++i)
+#pragma unroll
+      for (int j = i; j < 7; ++j) acc[c++] += row[i] * row[j];
+    acc[28] += 1.0f;
+  }
+  float out[29];
+  if (grid_sum<29>(acc, partials, &st->ticket[0], lds, out) && threadIdx.x == 0) {
+#pragma unroll
+    for (int i = 0; i < 29; ++i) st->sums_icp[i] = out[i];
+    if (record_inliers) st->icp_inliers = (int)out[28];
+  }
+}
+
+// ------------------------------------------------------------------ photometric
+// Warp parameters of ImageProc.cu:42-78: RK = R K^-1 (formed in double), t.
+__device__ inline void load_warp(const double* T, const double* Kd, float* lds /* 12 */) {
+  if (threadIdx.x < 12) {
+    int i = threadIdx.x / 4, j = threadIdx.x % 4;
+    float v;
+    if (j < 3) {
+      double Ki[9] = {1.0 / Kd[0], 0.0, -Kd[2] / Kd[0], 0.0, 1.0 / Kd[1], -Kd[3] / Kd[1], 0.0, 0.0, 1.0};
+      v = (float)(T[4 * i] * Ki[j] + T[4 * i + 1] * Ki[3 + j] + T[4 * i + 2] * Ki[6 + j]);
+      lds[3 * i + j] = v;
+    } else {
+      lds[9 + i] = (float)T[4 * i + 3];
+    }
+  }
+  __syncthreads();
+}
+
+__device__ inline float bilerp(const float* m, int w, float x, float y) {
+  int u = (int)floorf(x), v = (int)floorf(y);
+  float cx = x - (float)u, cy = y - (float)v;
+  const float* r0 = m + (size_t)v * w;
+  const float* r1 = r0 + w;
+  return (r0[u] * (1.0f - cx) + r0[u + 1] * cx) * (1.0f - cy) + (r1[u] * (1.0f - cx) + r1[u + 1] * cx) * cy;
+}
+
+// TrackingUtils.h:73-151 with the warp of ImageProc.cu:42-61 fused in.
+__global__ void __launch_bounds__(256) k_rgb_residual(Level L, const float* ref_inv, const float* ref_int,
+                                                      const float* cur_int, const float* gxs, const float* gys,
+                                                      float4* warped, float4* resid, TrackState* st, float* partials) {
+  __shared__ float wp[12];
+  __shared__ float lds[4 * 2];
+  load_warp(st->est, L.Kd, wp);
+  float acc[2] = {0.0f, 0.0f};
+  const int N = L.w * L.h;
+  for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < N; k += gridDim.x * blockDim.x) {
+    const int y = k / L.w, x = k - y * L.w;
+    const float iz = ref_inv[k];
+    float4 p = make_float4(0.f, 0.f, 0.f, -1.f);
+    if (iz > 0.0f) {
+      float z = 1.0f / iz;
+      float px = (wp[0] * (float)x + wp[1] * (float)y + wp[2]) * z + wp[9];
+      float py = (wp[3] * (float)x + wp[4] * (float)y + wp[5]) * z + wp[10];
+      float pz = (wp[6] * (float)x + wp[7] * (float)y + wp[8]) * z + wp[11];
+      p = make_float4(px, py, pz, 1.0f);
+    }
+    warped[k] = p;
+    float r = 0.0f, gx = 0.0f, gy = 0.0f;
+    bool ok = false;
+    if (p.w > 0.0f) {
+      float u = L.fx * p.x / p.z + L.cx;
+      float v = L.fy * p.y / p.z + L.cy;
+      if (u > 0.0f && v > 0.0f && u < (float)(L.w - 1) && v < (float)(L.h - 1)) {
+        float ref = ref_int[k];
+        float cur = bilerp(cur_int, L.w, u, v);
+        r = cur - ref;
+        gx = bilerp(gxs, L.w, u, v);
+        gy = bilerp(gys, L.w, u, v);
+        ok = (gx * gx + gy * gy) > 0.0f && isfinite(r);
+      }
+    }
+    resid[k] = make_float4(r, gx, gy, ok ? 0.5f : -0.5f);
+    if (ok) {
+      acc[0] += r;
+      acc[1] += 1.0f;
+    }
+  }
+  float out[2];
+  if (grid_sum<2>(acc, partials, &st->ticket[1], lds, out) && threadIdx.x == 0) {
+    st->rgb_stats[0] = out[0];
+    st->rgb_stats[1] = out[1];
+    st->rgb_stats[2] = out[0] / out[1];
+  }
+}
+
+// TrackingUtils.h:578-607, then sigma and the Huber threshold
+// 4.685 * sigma (RGBDTracking.cu:270-278).
+__global__ void __launch_bounds__(256) k_rgb_variance(Level L, const float4* resid, TrackState* st, float* partials) {
+  __shared__ float lds[4];
+  const float mean = st->rgb_stats[2];
+  float acc[1] = {0.0f};
+  const int N = L.w * L.h;
+  for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < N; k += gridDim.x * blockDim.x) {
+    float4 e = resid[k];
+    if (e.w > 0.0f) acc[0] += (e.x - mean) * (e.x - mean);
+  }
+  float out[1];
+  if (grid_sum<1>(acc, partials, &st->ticket[2], lds, out) && threadIdx.x == 0) {
+    float sigma = sqrtf(out[0] / (st->rgb_stats[1] - 1.0f));
+    st->rgb_stats[3] = (float)(4.685 * (double)sigma);
+  }
+}
+
+// TrackingUtils.h:153-220: Huber-weighted photometric rows.
+__global__ void __launch_bounds__(256) k_rgb_jacobian(Level L, const float4* warped, const float4* resid,
+                                                      TrackState* st, float* partials, int record_inliers) {
+  __shared__ float lds[4 * 29];
+  const float th = st->rgb_stats[3];
+  float acc[29];
+#pragma unroll
+  for (int i = 0; i < 29; ++i) acc[i] = 0.0f;
+  const int N = L.w * L.h;
+  for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < N; k += gridDim.x * blockDim.x) {
+    const float4 e = resid[k];
+    if (!(e.w > 0.0f)) continue;
+    const float4 p = warped[k];
+    float zinv = 1.0f / p.z;
+    float wt = 1.0f;
+    if (fabsf(e.x) > th) wt = th / fabsf(e.x);
+    float row[7];
+    row[0] = e.y * L.fx * zinv;
+    row[1] = e.z * L.fy * zinv;
+    row[2] = -(row[0] * p.x + row[1] * p.y) * zinv;
+    row[3] = row[2] * p.y - e.z * L.fy;
+    row[4] = -row[2] * p.x + e.y * L.fx;
+    row[5] = -row[0] * p.y + row[1] * p.x;
+    row[6] = -e.x;
+    int c = 0;
+#pragma unroll
+    for (int i = 0; i < 7; ++i)
+#pragma unroll
+      for (int j = i; j < 7; ++j) acc[c++] += wt * row[i] * row[j];
+    acc[28] += 1.0f;
+  }
+  float out[29];
+  if (grid_sum<29>(acc, partials, &st->ticket[3], lds, out) && threadIdx.x == 0) {
+#pragma unroll
+    for (int i = 0; i < 29; ++i) st->sums_rgb[i] = out[i];
+    if (record_inliers) st->rgb_inliers = (int)out[28];
+  }
+}
+
+// ------------------------------------------------------------------ GN update
+// Eigen 3.3 LDLT restated (diagonal pivoting, pseudo-inverse of D).
+__device__ inline void ldlt_solve6(double* A, double* y) {
+  const int n = 6;
+  int tr[6];
+  for (int k = 0; k < n; ++k) {
+    int big = k;
+    double bv = fabs(A[k * n + k]);
+    for (int i = k + 1; i < n; ++i)
+      if (fabs(A[i * n + i]) > bv) { bv = fabs(A[i * n + i]); big = i; }
+    tr[k] = big;
+    if (big != k) {
+      for (int j = 0; j < k; ++j) { double t = A[k * n + j]; A[k * n + j] = A[big * n + j]; A[big * n + j] = t; }
+      for (int i = big + 1; i < n; ++i) { double t = A[i * n + k]; A[i * n + k] = A[i * n + big]; A[i * n + big] = t; }
+      { double t = A[k * n + k]; A[k * n + k] = A[big * n + big]; A[big * n + big] = t; }
+      for (int i = k + 1; i < big; ++i) {
+        double t = A[i * n + k];
+        A[i * n + k] = A[big * n + i];
+        A[big * n + i] = t;
+      }
+    }
+    double temp[6];
+    if (k > 0) {
+      for (int j = 0; j < k; ++j) temp[j] = A[j * n + j] * A[k * n + j];
+      double s = 0.0;
+      for (int j = 0; j < k; ++j) s += A[k * n + j] * temp[j];
+      A[k * n + k] -= s;
+      for (int i = k + 1; i < n; ++i) {
+        double r = 0.0;
+        for (int j = 0; j < k; ++j) r += A[i * n + j] * temp[j];
+        A[i * n + k] -= r;
+      }
+    }
+    double akk = A[k * n + k];
+    bool valid = fabs(akk) > 0.0;
+    if (k == 0 && !valid) {
+      for (int j = 0; j < n; ++j) tr[j] = j;
+      break;
+    }
+    if (valid)
+      for (int i = k + 1; i < n; ++i) A[i * n + k] /= akk;
+  }
+  for (int k = 0; k < n; ++k) { double t = y[k]; y[k] = y[tr[k]]; y[tr[k]] = t; }
+  for (int i = 0; i < n; ++i)
+    for (int j = 0; j < i; ++j) y[i] -= A[i * n + j] * y[j];
+  for (int i = 0; i < n; ++i) {
+    double d = A[i * n + i];
+    y[i] = fabs(d) > DBL_MIN ? y[i] / d : 0.0;
+  }
+  for (int i = n - 1; i >= 0; --i)
+    for (int j = i + 1; j < n; ++j) y[i] -= A[j * n + i] * y[j];
+  for (int k = n - 1; k >= 0; --k) { double t = y[k]; y[k] = y[tr[k]]; y[tr[k]] = t; }
+}
+
+// SE(3) exponential, twist (rho, omega), Rodrigues' formula.
+__device__ inline void se3_exp(const double* xi, double* T) {
+  const double* w = xi + 3;
+  double th2 = w[0] * w[0] + w[1] * w[1] + w[2] * w[2];
+  double th = sqrt(th2);
+  double A, B, C;
+  if (th < 1e-8) {
+    A = 1.0 - th2 / 6.0;
+    B = 0.5 - th2 / 24.0;
+    C = 1.0 / 6.0 - th2 / 120.0;
+  } else {
+    A = sin(th) / th;
+    B = (1.0 - cos(th)) / th2;
+    C = (th - sin(th)) / (th2 * th);
+  }
+  double W[9] = {0, -w[2], w[1], w[2], 0, -w[0], -w[1], w[0], 0};
+  double W2[9], Rm[9], V[9];
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) W2[3 * i + j] = W[3 * i] * W[j] + W[3 * i + 1] * W[3 + j] + W[3 * i + 2] * W[6 + j];
+  for (int i = 0; i < 9; ++i) {
+    double I = (i % 4 == 0) ? 1.0 : 0.0;
+    Rm[i] = I + A * W[i] + B * W2[i];
+    V[i] = I + B * W[i] + C * W2[i];
+  }
+  for (int i = 0; i < 3; ++i) {
+    for (int j = 0; j < 3; ++j) T[4 * i + j] = Rm[3 * i + j];
+    T[4 * i + 3] = V[3 * i] * xi[0] + V[3 * i + 1] * xi[1] + V[3 * i + 2] * xi[2];
+  }
+  T[12] = T[13] = T[14] = 0.0;
+  T[15] = 1.0;
+}
+
+// RankUpdateHessian<6,7> (TrackingUtils.h:45-58).
+__device__ inline void unpack29(const float* s, float* H, float* b) {
+  int k = 0;
+  for (int i = 0; i < 6; ++i)
+    for (int j = i; j < 7; ++j) {
+      float v = s[k++];
+      if (j == 6) b[i] = v;
+      else H[6 * i + j] = H[6 * j + i] = v;
+    }
+}
+
+// One Gauss-Newton step of RGBDTracking.cu:161-204.
+__global__ void k_gn_update(TrackState* st, int mode, int first_of_level, float wrh, float wrb, float wih, float wib) {
+  if (threadIdx.x != 0) return;
+  if (first_of_level) st->last_err = FLT_MAX;
+  if (!st->good) return;
+  float H[36], b[6];
+  for (int i = 0; i < 36; ++i) H[i] = 0.0f;
+  for (int i = 0; i < 6; ++i) b[i] = 0.0f;
+  float rsum = 0.0f, nres = 0.0f;
+  if (mode != YAS_TRACK_DEPTH) {
+    float Hr[36], br[6];
+    unpack29(st->sums_rgb, Hr, br);
+    float wh = mode == YAS_TRACK_RGBD ? wrh : 1.0f, wb = mode == YAS_TRACK_RGBD ? wrb : 1.0f;
+    for (int i = 0; i < 36; ++i) H[i] = H[i] + wh * Hr[i];
+    for (int i = 0; i < 6; ++i) b[i] = b[i] + wb * br[i];
+    rsum = st->sums_rgb[27];
+    nres = st->sums_rgb[28];
+  }
+  if (mode != YAS_TRACK_RGB) {
+    float Hi[36], bi[6];
+    unpack29(st->sums_icp, Hi, bi);
+    float wh = mode == YAS_TRACK_RGBD ? wih : 1.0f, wb = mode == YAS_TRACK_RGBD ? wib : 1.0f;
+    for (int i = 0; i < 36; ++i) H[i] = H[i] + wh * Hi[i];
+    for (int i = 0; i < 6; ++i) b[i] = b[i] + wb * bi[i];
+    rsum = st->sums_icp[27];
+  }
+  float err = sqrtf(rsum) / (nres + 6.0f);
+  double A[36], x[6];
+  for (int i = 0; i < 36; ++i) A[i] = (double)H[i];
+  for (int i = 0; i < 6; ++i) x[i] = (double)b[i];
+  ldlt_solve6(A, x);
+  if (isnan(x[0])) {
+    st->good = 0;
+    return;
+  }
+  double E[16];
+  se3_exp(x, E);
+  mul44(E, st->est, st->est);
+  if (err < st->last_err) {
+    for (int i = 0; i < 16; ++i) st->last_ok[i] = st->est[i];
+    st->last_err = err;
+  }
+}
+
+__global__ void k_track_init(TrackState* st, Mat44 T) {
+  if (threadIdx.x != 0) return;
+  for (int i = 0; i < 16; ++i) st->est[i] = st->last_ok[i] = T.m[i];
+  st->good = 1;
+  st->last_err = FLT_MAX;
+}
+__global__ void k_track_init_identity(TrackState* st) {
+  if (threadIdx.x != 0) return;
+  for (int i = 0; i < 16; ++i) st->est[i] = st->last_ok[i] = (i % 5 == 0) ? 1.0 : 0.0;
+  st->good = 1;
+  st->last_err = FLT_MAX;
+}
+
+inline dim3 grid64x4(int W, int H) { return dim3((W + 63) / 64, (H + 3) / 4); }
+
+}  // namespace
+
+// ------------------------------------------------------------------ host
+struct yas_tracker {
+  yas_ctx* ctx;
+  int W, H;
+  yas_tracker_params P;
+  Level lv[kMaxLevels];
+  float *cur_int[kMaxLevels], *ref_int[kMaxLevels], *gx[kMaxLevels], *gy[kMaxLevels];
+  float *cur_inv[kMaxLevels], *ref_inv[kMaxLevels];
+  float4 *cur_v[kMaxLevels], *cur_n[kMaxLevels], *ref_v[kMaxLevels], *ref_n[kMaxLevels];
+  float4 *warped[kMaxLevels], *resid[kMaxLevels];
+  float* partials;
+  TrackState* st;
+  TrackState* host_st;
+  float angle_th, dist_th;
+};
+
+static int image_pyramid(yas_tracker* t, const float* g, size_t pitch, float** pyr, bool grads) {
+  hipStream_t s = t->ctx->stream;
+  hipLaunchKernelGGL(k_copy_f32, grid64x4(t->W, t->H), dim3(256), 0, s, g, pitch, pyr[0], t->W, t->H);
+  for (int l = 1; l < t->P.levels; ++l) {
+    const Level& a = t->lv[l - 1];
+    const Level& b = t->lv[l];
+    hipLaunchKernelGGL(k_pyrdown_gauss, dim3((b.w + 15) / 16, (b.h + 15) / 16), dim3(256), 0, s, pyr[l - 1],
+                       (size_t)a.w * 4, a.w, a.h, pyr[l], (size_t)b.w * 4, b.w, b.h);
+  }
+  if (grads)
+    for (int l = 0; l < t->P.levels; ++l) {
+      const Level& a = t->lv[l];
+      hipLaunchKernelGGL(k_gradient, grid64x4(a.w, a.h), dim3(256), 0, s, pyr[l], (size_t)a.w * 4, t->gx[l], t->gy[l],
+                         (size_t)a.w * 4, a.w, a.h);
+    }
+  YAS_LAUNCH_CHECK();
+  return YAS_OK;
+}
+
+static int depth_pyramid(yas_tracker* t, const float* d, size_t pitch, float** inv, float4** v, float4** n) {
+  hipStream_t s = t->ctx->stream;
+  for (int l = 0; l < t->P.levels; ++l) {
+    const Level& a = t->lv[l];
+    if (l == 0)
+      hipLaunchKernelGGL(k_invdepth, grid64x4(a.w, a.h), dim3(256), 0, s, d, pitch, inv[0], (size_t)a.w * 4, a.w, a.h);
+    else
+      hipLaunchKernelGGL(k_pyrdown_depth, grid64x4(a.w, a.h), dim3(256), 0, s, inv[l - 1], (size_t)t->lv[l - 1].w * 4,
+                         inv[l], (size_t)a.w * 4, a.w, a.h);
+    hipLaunchKernelGGL(k_vertex, grid64x4(a.w, a.h), dim3(256), 0, s, inv[l], (size_t)a.w * 4, v[l], (size_t)a.w * 16,
+                       a.w, a.h, a.invfx, a.invfy, a.cx, a.cy, 3.0f);
+    hipLaunchKernelGGL(k_normal, grid64x4(a.w, a.h), dim3(256), 0, s, v[l], (size_t)a.w * 16, n[l], (size_t)a.w * 16,
+                       a.w, a.h);
+  }
+  YAS_LAUNCH_CHECK();
+  return YAS_OK;
+}
+
+static int model_pyramid(yas_tracker* t) {
+  hipStream_t s = t->ctx->stream;
+  for (int l = 0; l < t->P.levels; ++l) {
+    const Level& a = t->lv[l];
+    if (l > 0)
+      hipLaunchKernelGGL(k_pyrdown_vmap, grid64x4(a.w, a.h), dim3(256), 0, s, t->ref_v[l - 1],
+                         (size_t)t->lv[l - 1].w * 16, t->ref_v[l], (size_t)a.w * 16, a.w, a.h);
+    hipLaunchKernelGGL(k_normal, grid64x4(a.w, a.h), dim3(256), 0, s, t->ref_v[l], (size_t)a.w * 16, t->ref_n[l],
+                       (size_t)a.w * 16, a.w, a.h);
+  }
+  YAS_LAUNCH_CHECK();
+  return YAS_OK;
+}
+
+static int launch_icp(yas_tracker* t, int l, bool rec) {
+  const Level& a = t->lv[l];
+  hipLaunchKernelGGL(k_icp, dim3(kRedGroups), dim3(256), 0, t->ctx->stream, a, (const float4*)t->ref_v[l],
+                     (const float4*)t->ref_n[l], (const float4*)t->cur_v[l], (const float4*)t->cur_n[l], t->st,
+                     t->partials, t->angle_th, t->dist_th, rec ? 1 : 0);
+  YAS_LAUNCH_CHECK();
+  return YAS_OK;
+}
+
+static int launch_rgb(yas_tracker* t, int l, bool rec) {
+  const Level& a = t->lv[l];
+  hipStream_t s = t->ctx->stream;
+  hipLaunchKernelGGL(k_rgb_residual, dim3(kRedGroups), dim3(256), 0, s, a, (const float*)t->ref_inv[l],
+                     (const float*)t->ref_int[l], (const float*)t->cur_int[l], (const float*)t->gx[l],
+                     (const float*)t->gy[l], t->warped[l], t->resid[l], t->st, t->partials);
+  hipLaunchKernelGGL(k_rgb_variance, dim3(kRedGroups), dim3(256), 0, s, a, (const float4*)t->resid[l], t->st,
+                     t->partials);
+  hipLaunchKernelGGL(k_rgb_jacobian, dim3(kRedGroups), dim3(256), 0, s, a, (const float4*)t->warped[l],
+                     (const float4*)t->resid[l], t->st, t->partials, rec ? 1 : 0);
+  YAS_LAUNCH_CHECK();
+  return YAS_OK;
+}
+
+// Coarse-to-fine Gauss-Newton from the pose already in the device state.
+int yas_tracker_solve_async(yas_tracker* t) {
+  hipStream_t s = t->ctx->stream;
+  for (int l = t->P.levels - 1; l >= 0; --l) {
+    for (int it = 0; it < t->P.iters[l]; ++it) {
+      int rc;
+      if (t->P.mode != YAS_TRACK_DEPTH && (rc = launch_rgb(t, l, l == 0))) return rc;
+      if (t->P.mode != YAS_TRACK_RGB && (rc = launch_icp(t, l, l == 0))) return rc;
+      hipLaunchKernelGGL(k_gn_update, dim3(1), dim3(64), 0, s, t->st, t->P.mode, it == 0 ? 1 : 0, t->P.w_rgb_h,
+                         t->P.w_rgb_b, t->P.w_icp_h, t->P.w_icp_b);
+      YAS_LAUNCH_CHECK();
+    }
+  }
+  return YAS_OK;
+}
+
+int yas_tracker_init_identity_async(yas_tracker* t) {
+  hipLaunchKernelGGL(k_track_init_identity, dim3(1), dim3(64), 0, t->ctx->stream, t->st);
+  YAS_LAUNCH_CHECK();
+  return YAS_OK;
+}
+
+// Device pointers of the solver's result (camera pose change and status).
+const double* yas_tracker_result_dev(yas_tracker* t) { return t->st->last_ok; }
+const int* yas_tracker_good_dev(yas_tracker* t) { return &t->st->good; }
+
+int yas_tracker_set_tracking_image_dev(yas_tracker* t, const float* g, size_t pitch) {
+  return image_pyramid(t, g, pitch, t->cur_int, true);
+}
+int yas_tracker_set_tracking_depth_dev(yas_tracker* t, const float* d, size_t pitch) {
+  return depth_pyramid(t, d, pitch, t->cur_inv, t->cur_v, t->cur_n);
+}
+
+int yas_rgb_to_gray_async(yas_ctx* ctx, const uint8_t* rgb, size_t sp, float* dst, size_t dp, int W, int H, int mode) {
+  hipLaunchKernelGGL(k_rgb_to_gray, grid64x4(W, H), dim3(256), 0, ctx->stream, rgb, sp, dst, dp, W, H, mode);
+  YAS_LAUNCH_CHECK();
+  return YAS_OK;
+}
+
+extern "C" {
+
+int yas_tracker_create(yas_ctx* ctx, int W, int H, const double K[4], const yas_tracker_params* P, yas_tracker** out) {
+  YAS_REQUIRE(ctx && K && P && out, "yas_tracker_create: null argument");
+  YAS_REQUIRE(P->levels >= 1 && P->levels <= kMaxLevels, "yas_tracker_create: levels must be 1..5");
+  YAS_REQUIRE(P->mode >= 0 && P->mode <= 2, "yas_tracker_create: unknown mode");
+  YAS_REQUIRE(W > 0 && H > 0 && W % (1 << (P->levels - 1)) == 0 && H % (1 << (P->levels - 1)) == 0,
+              "yas_tracker_create: width and height must be divisible by 2^(levels-1)");
+  YAS_REQUIRE((W >> (P->levels - 1)) >= 4 && (H >> (P->levels - 1)) >= 4,
+              "yas_tracker_create: the coarsest level must be at least 4x4");
+  YAS_HIP(hipSetDevice(ctx->device));
+  yas_tracker* t = new yas_tracker();
+  std::memset(t, 0, sizeof(*t));
+  t->ctx = ctx;
+  t->W = W;
+  t->H = H;
+  t->P = *P;
+  // RGBDTracking.cu:432-433: sin(20 deg) computed in float, 1 cm.
+  t->angle_th = (float)std::sin(20.f * 3.14159254f / 180.f);
+  t->dist_th = 0.01f;
+  hipError_t e = hipSuccess;
+  auto alloc = [&](void** p, size_t bytes) {
+    if (e == hipSuccess) e = hipMalloc(p, bytes);
+  };
+  for (int l = 0; l < P->levels; ++l) {
+    Level& a = t->lv[l];
+    double s = (double)(1 << l);
+    a.Kd[0] = K[0] / s; a.Kd[1] = K[1] / s; a.Kd[2] = K[2] / s; a.Kd[3] = K[3] / s;
+    a.w = W >> l;
+    a.h = H >> l;
+    a.fx = (float)a.Kd[0]; a.fy = (float)a.Kd[1]; a.cx = (float)a.Kd[2]; a.cy = (float)a.Kd[3];
+    a.invfx = (float)(1.0 / a.Kd[0]);
+    a.invfy = (float)(1.0 / a.Kd[1]);
+    size_t n = (size_t)a.w * a.h;
+    for (float** p : {&t->cur_int[l], &t->ref_int[l], &t->gx[l], &t->gy[l], &t->cur_inv[l], &t->ref_inv[l]})
+      alloc((void**)p, n * sizeof(float));
+    for (float4** p : {&t->cur_v[l], &t->cur_n[l], &t->ref_v[l], &t->ref_n[l], &t->warped[l], &t->resid[l]})
+      alloc((void**)p, n * sizeof(float4));
+  }
+  alloc((void**)&t->partials, sizeof(float) * 32 * kRedGroups);
+  alloc((void**)&t->st, sizeof(TrackState));
+  if (e == hipSuccess) e = hipHostMalloc((void**)&t->host_st, sizeof(TrackState), hipHostMallocDefault);
+  if (e != hipSuccess) {
+    yas_set_error("yas_tracker_create: %s", hipGetErrorString(e));
+    yas_tracker_destroy(t);
+    return YAS_E_HIP;
+  }
+  // zeroed images, zeroed state (tickets armed)
+  for (int l = 0; l < P->levels; ++l) {
+    size_t n = (size_t)t->lv[l].w * t->lv[l].h;
+    for (float* p : {t->cur_int[l], t->ref_int[l], t->gx[l], t->gy[l], t->cur_inv[l], t->ref_inv[l]})
+      YAS_HIP(hipMemsetAsync(p, 0, n * sizeof(float), ctx->stream));
+    for (float4* p : {t->cur_v[l], t->cur_n[l], t->ref_v[l], t->ref_n[l], t->warped[l], t->resid[l]})
+      YAS_HIP(hipMemsetAsync(p, 0, n * sizeof(float4), ctx->stream));
+  }
+  YAS_HIP(hipMemsetAsync(t->st, 0, sizeof(TrackState), ctx->stream));
+  int rc = yas_tracker_init_identity_async(t);
+  if (rc) {
+    yas_tracker_destroy(t);
+    return rc;
+  }
+  *out = t;
+  return YAS_OK;
+}
+
+int yas_tracker_destroy(yas_tracker* t) {
+  if (!t) return YAS_OK;
+  hipSetDevice(t->ctx->device);
+  hipStreamSynchronize(t->ctx->stream);
+  for (int l = 0; l < kMaxLevels; ++l) {
+    for (float* p : {t->cur_int[l], t->ref_int[l], t->gx[l], t->gy[l], t->cur_inv[l], t->ref_inv[l]}) hipFree(p);
+    for (float4* p : {t->cur_v[l], t->cur_n[l], t->ref_v[l], t->ref_n[l], t->warped[l], t->resid[l]}) hipFree(p);
+  }
+  hipFree(t->partials);
+  hipFree(t->st);
+  if (t->host_st) hipHostFree(t->host_st);
+  delete t;
+  return YAS_OK;
+}
+
+int yas_tracker_set_reference_image(yas_tracker* t, const float* g, size_t pitch) {
+  YAS_REQUIRE(t && g && pitch >= (size_t)t->W * 4, "yas_tracker_set_reference_image: bad argument");
+  return image_pyramid(t, g, pitch, t->ref_int, false);
+}
+int yas_tracker_set_tracking_image(yas_tracker* t, const float* g, size_t pitch) {
+  YAS_REQUIRE(t && g && pitch >= (size_t)t->W * 4, "yas_tracker_set_tracking_image: bad argument");
+  return image_pyramid(t, g, pitch, t->cur_int, true);
+}
+int yas_tracker_set_reference_depth(yas_tracker* t, const float* d, size_t pitch) {
+  YAS_REQUIRE(t && d && pitch >= (size_t)t->W * 4, "yas_tracker_set_reference_depth: bad argument");
+  return depth_pyramid(t, d, pitch, t->ref_inv, t->ref_v, t->ref_n);
+}
+int yas_tracker_set_tracking_depth(yas_tracker* t, const float* d, size_t pitch) {
+  YAS_REQUIRE(t && d && pitch >= (size_t)t->W * 4, "yas_tracker_set_tracking_depth: bad argument");
+  return depth_pyramid(t, d, pitch, t->cur_inv, t->cur_v, t->cur_n);
+}
+int yas_tracker_set_reference_model(yas_tracker* t, const float* vmap, size_t pitch) {
+  YAS_REQUIRE(t && vmap && pitch >= (size_t)t->W * 16, "yas_tracker_set_reference_model: bad argument");
+  hipLaunchKernelGGL(k_copy_f4, grid64x4(t->W, t->H), dim3(256), 0, t->ctx->stream, (const float4*)vmap, pitch,
+                     t->ref_v[0], t->W, t->H);
+  YAS_LAUNCH_CHECK();
+  return model_pyramid(t);
+}
+
+int yas_tracker_swap_frame_buffer(yas_tracker* t) {
+  YAS_REQUIRE(t, "null tracker");
+  for (int l = 0; l < kMaxLevels; ++l) {
+    std::swap(t->ref_v[l], t->cur_v[l]);
+    std::swap(t->ref_n[l], t->cur_n[l]);
+    std::swap(t->ref_inv[l], t->cur_inv[l]);
+    std::swap(t->ref_int[l], t->cur_int[l]);
+  }
+  return YAS_OK;
+}
+
+int yas_tracker_get_transform(yas_tracker* t, const double init[16], int swap, double out[16], yas_track_stats* stats) {
+  YAS_REQUIRE(t && out, "yas_tracker_get_transform: null argument");
+  YAS_HIP(hipSetDevice(t->ctx->device));
+  Mat44 M;
+  if (init) std::memcpy(M.m, init, sizeof(M.m));
+  else
+    for (int i = 0; i < 16; ++i) M.m[i] = (i % 5 == 0) ? 1.0 : 0.0;
+  hipLaunchKernelGGL(k_track_init, dim3(1), dim3(64), 0, t->ctx->stream, t->st, M);
+  YAS_LAUNCH_CHECK();
+  int rc = yas_tracker_solve_async(t);
+  if (rc) return rc;
+  YAS_HIP(hipMemcpyAsync(t->host_st, t->st, sizeof(TrackState), hipMemcpyDeviceToHost, t->ctx->stream));
+  YAS_HIP(hipStreamSynchronize(t->ctx->stream));
+  if (swap) yas_tracker_swap_frame_buffer(t);
+  const TrackState& h = *t->host_st;
+  if (h.good) std::memcpy(out, h.last_ok, sizeof(h.last_ok));
+  else
+    for (int i = 0; i < 16; ++i) out[i] = (i % 5 == 0) ? 1.0 : 0.0;
+  if (stats) {
+    stats->good = h.good;
+    stats->icp_inliers = h.icp_inliers;
+    stats->rgb_inliers = h.rgb_inliers;
+    stats->last_error = h.last_err;
+  }
+  return h.good ? YAS_OK : YAS_E_DIVERGED;
+}
+
+int yas_tracker_is_tracking_good(yas_tracker* t, int* good) {
+  YAS_REQUIRE(t && good, "null argument");
+  YAS_HIP(hipMemcpyAsync(t->host_st, t->st, sizeof(TrackState), hipMemcpyDeviceToHost, t->ctx->stream));
+  YAS_HIP(hipStreamSynchronize(t->ctx->stream));
+  *good = t->host_st->good;
+  return YAS_OK;
+}
+
+int yas_tracker_step_sums(yas_tracker* t, int level, const double T[16], int kind, float out[29], float stats[4]) {
+  YAS_REQUIRE(t && T && out, "yas_tracker_step_sums: null argument");
+  YAS_REQUIRE(level >= 0 && level < t->P.levels, "yas_tracker_step_sums: bad level");
+  YAS_HIP(hipSetDevice(t->ctx->device));
+  Mat44 M;
+  std::memcpy(M.m, T, sizeof(M.m));
+  hipLaunchKernelGGL(k_track_init, dim3(1), dim3(64), 0, t->ctx->stream, t->st, M);
+  int rc = kind == 0 ? launch_icp(t, level, false) : launch_rgb(t, level, false);
+  if (rc) return rc;
+  YAS_HIP(hipMemcpyAsync(t->host_st, t->st, sizeof(TrackState), hipMemcpyDeviceToHost, t->ctx->stream));
+  YAS_HIP(hipStreamSynchronize(t->ctx->stream));
+  const float* s = kind == 0 ? t->host_st->sums_icp : t->host_st->sums_rgb;
+  std::memcpy(out, s, sizeof(float) * 29);
+  if (stats) std::memcpy(stats, t->host_st->rgb_stats, sizeof(float) * 4);
+  return YAS_OK;
+}
+
+int yas_tracker_level(yas_tracker* t, int which, int level, void** ptr, size_t* pitch, int* w, int* h) {
+  YAS_REQUIRE(t && ptr, "null argument");
+  YAS_REQUIRE(level >= 0 && level < t->P.levels, "bad level");
+  const Level& a = t->lv[level];
+  void* one[6] = {t->cur_int[level], t->ref_int[level], t->gx[level], t->gy[level], t->cur_inv[level], t->ref_inv[level]};
+  void* four[6] = {t->cur_v[level], t->cur_n[level], t->ref_v[level], t->ref_n[level], t->warped[level], t->resid[level]};
+  YAS_REQUIRE(which >= 0 && which < 12, "bad buffer id");
+  *ptr = which < 6 ? one[which] : four[which - 6];
+  if (pitch) *pitch = (size_t)a.w * (which < 6 ? 4 : 16);
+  if (w) *w = a.w;
+  if (h) *h = a.h;
+  return YAS_OK;
+}
+
+int yas_rgb_to_intensity(yas_ctx* ctx, const uint8_t* rgb, size_t sp, float* dst, size_t dp, int W, int H, int mode) {
+  YAS_REQUIRE(ctx && rgb && dst && W > 0 && H > 0, "yas_rgb_to_intensity: bad argument");
+  return yas_rgb_to_gray_async(ctx, rgb, sp, dst, dp, W, H, mode);
+}
+int yas_depth_to_invdepth(yas_ctx* ctx, const float* d, size_t sp, float* inv, size_t dp, int W, int H) {
+  YAS_REQUIRE(ctx && d && inv && W > 0 && H > 0, "bad argument");
+  hipLaunchKernelGGL(k_invdepth, grid64x4(W, H), dim3(256), 0, ctx->stream, d, sp, inv, dp, W, H);
+  YAS_LAUNCH_CHECK();
+  return YAS_OK;
+}
+int yas_pyrdown_depth(yas_ctx* ctx, const float* s, size_t sp, float* d, size_t dp, int W, int H) {
+  YAS_REQUIRE(ctx && s && d && W > 0 && H > 0, "bad argument");
+  hipLaunchKernelGGL(k_pyrdown_depth, grid64x4(W, H), dim3(256), 0, ctx->stream, s, sp, d, dp, W, H);
+  YAS_LAUNCH_CHECK();
+  return YAS_OK;
+}
+int yas_pyrdown_gauss(yas_ctx* ctx, const float* s, size_t sp, int sw, int sh, float* d, size_t dp) {
+  YAS_REQUIRE(ctx && s && d && sw >= 4 && sh >= 4, "yas_pyrdown_gauss: source must be at least 4x4");
+  int dw = sw / 2, dh = sh / 2;
+  hipLaunchKernelGGL(k_pyrdown_gauss, dim3((dw + 15) / 16, (dh + 15) / 16), dim3(256), 0, ctx->stream, s, sp, sw, sh, d,
+                     dp, dw, dh);
+  YAS_LAUNCH_CHECK();
+  return YAS_OK;
+}
+int yas_compute_vertex_map(yas_ctx* ctx, const float* inv, size_t sp, float* v, size_t dp, int W, int H, const float K[4],
+                           float cutoff) {
+  YAS_REQUIRE(ctx && inv && v && K && W > 0 && H > 0, "bad argument");
+  Intr k = make_intr(K);
+  hipLaunchKernelGGL(k_vertex, grid64x4(W, H), dim3(256), 0, ctx->stream, inv, sp, (float4*)v, dp, W, H, k.invfx,
+                     k.invfy, k.cx, k.cy, cutoff);
+  YAS_LAUNCH_CHECK();
+  return YAS_OK;
+}
+int yas_compute_normal_map(yas_ctx* ctx, const float* v, size_t sp, float* n, size_t dp, int W, int H) {
+  YAS_REQUIRE(ctx && v && n && W > 0 && H > 0, "bad argument");
+  hipLaunchKernelGGL(k_normal, grid64x4(W, H), dim3(256), 0, ctx->stream, (const float4*)v, sp, (float4*)n, dp, W, H);
+  YAS_LAUNCH_CHECK();
+  return YAS_OK;
+}
+int yas_image_gradient(yas_ctx* ctx, const float* s, size_t sp, float* gx, float* gy, size_t dp, int W, int H) {
+  YAS_REQUIRE(ctx && s && gx && gy && W > 0 && H > 0, "bad argument");
+  hipLaunchKernelGGL(k_gradient, grid64x4(W, H), dim3(256), 0, ctx->stream, s, sp, gx, gy, dp, W, H);
+  YAS_LAUNCH_CHECK();
+  return YAS_OK;
+}
+int yas_vmap_to_depth(yas_ctx* ctx, const float* v, size_t sp, float* d, size_t dp, int W, int H) {
+  YAS_REQUIRE(ctx && v && d && W > 0 && H > 0, "bad argument");
+  hipLaunchKernelGGL(k_vmap_to_depth, grid64x4(W, H), dim3(256), 0, ctx->stream, (const float4*)v, sp, d, dp, W, H);
+  YAS_LAUNCH_CHECK();
+  return YAS_OK;
+}
+
+}  // extern "C"

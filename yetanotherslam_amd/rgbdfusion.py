@@ -1,0 +1,439 @@
+"""Python mirror of the reference's dense-fusion classes over the C ABI.
+
+Class and method names follow RGBDFusion/include (MapStruct, RayTraceEngine,
+MeshEngine, RGBDTracking and the ImageProc free functions) so that tests read
+like calls into the reference.  Device images are torch tensors on the
+context's device (the reference passes cv::cuda::GpuMat); poses are 4x4 numpy
+arrays (the reference passes Sophus::SE3d).  Work runs on torch's current
+stream of that device, so torch ops and library kernels are ordered.
+
+PyTorch is plumbing here (device memory and streams); every computation runs
+in libyas.so's HIP kernels, and nothing falls back to the CPU.
+"""
+import ctypes
+
+import numpy as np
+import torch
+
+from . import native
+from .native import MapParams, MapStats, TrackerParams, TrackStats, call
+
+VOXEL_I16_U8, VOXEL_F32_F32, VOXEL_F16_U16 = 0, 1, 2
+TRACK_RGB, TRACK_DEPTH, TRACK_RGBD = 0, 1, 2
+HIP_MEMCPY_D2H = 2
+
+_hip = None
+
+
+def _hiprt():
+    global _hip
+    if _hip is None:
+        native.lib()
+        _hip = ctypes.CDLL("libamdhip64.so")
+        _hip.hipMemcpy.restype = ctypes.c_int
+        _hip.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
+    return _hip
+
+
+def download(ptr, shape, dtype):
+    """Copy device memory at `ptr` into a new numpy array (synchronous)."""
+    out = np.empty(shape, dtype)
+    rc = _hiprt().hipMemcpy(out.ctypes.data, ctypes.c_void_p(ptr), out.nbytes, HIP_MEMCPY_D2H)
+    if rc != 0:
+        raise RuntimeError("hipMemcpy failed: %d" % rc)
+    return out
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _ptr(a):
+    return ctypes.c_void_p(a.ctypes.data)
+
+
+def _dev(t):
+    if not (isinstance(t, torch.Tensor) and t.is_cuda):
+        raise TypeError("expected a CUDA/HIP torch tensor")
+    return ctypes.c_void_p(t.data_ptr())
+
+
+def _pitch(t):
+    return t.stride(0) * t.element_size()
+
+
+class Context:
+    """yas_ctx bound to torch's current stream on `device`."""
+
+    def __init__(self, device=0):
+        self.device = device
+        with torch.cuda.device(device):
+            stream = torch.cuda.current_stream(device).cuda_stream
+        h = ctypes.c_void_p()
+        call("yas_ctx_create", device, ctypes.c_void_p(stream), ctypes.byref(h))
+        self.h = h
+
+    def synchronize(self):
+        call("yas_ctx_synchronize", self.h)
+
+    def close(self):
+        if getattr(self, "h", None):
+            native.lib().yas_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+_default_ctx = {}
+
+
+def default_context(device=0):
+    if device not in _default_ctx:
+        _default_ctx[device] = Context(device)
+    return _default_ctx[device]
+
+
+def decode_voxels(raw, fmt):
+    """Raw exported voxels (n, 512) -> (sdf float32, weight float32)."""
+    if fmt == VOXEL_F32_F32:
+        a = raw.view(np.float32).reshape(raw.shape[0], 512, 2)
+        return a[..., 0].copy(), a[..., 1].copy()
+    v = raw.view(np.uint32).reshape(raw.shape[0], 512)
+    lo = (v & 0xFFFF).astype(np.uint16)
+    if fmt == VOXEL_I16_U8:
+        sdf = lo.view(np.int16).astype(np.float32) / np.float32(32767.0)
+        wt = ((v >> 16) & 0xFF).astype(np.float32)
+    else:
+        sdf = lo.view(np.float16).astype(np.float32)
+        wt = (v >> 16).astype(np.float32)
+    return sdf, wt
+
+
+class MapStruct:
+    """MapStruct (RGBDFusion/include/VoxelMap.h:32-144)."""
+
+    def __init__(self, K, num_buckets, num_blocks, voxel_size, trunc_dist, voxel_format=VOXEL_F32_F32,
+                 max_weight=255, depth_min=0.1, depth_max=3.0, ctx=None):
+        self.ctx = ctx or default_context()
+        p = MapParams(num_buckets, num_blocks, voxel_size, trunc_dist, voxel_format, max_weight, depth_min, depth_max)
+        self.params = p
+        self.K = np.asarray(K, np.float32)
+        h = ctypes.c_void_p()
+        call("yas_map_create", self.ctx.h, ctypes.byref(p), _ptr(self.K), ctypes.byref(h))
+        self.h = h
+        self.voxel_format = voxel_format
+
+    def close(self):
+        if getattr(self, "h", None):
+            native.lib().yas_map_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def Reset(self):
+        call("yas_map_reset", self.h)
+
+    def SetPose(self, T):
+        call("yas_map_set_pose", self.h, _ptr(_f64(T)))
+
+    def GetPose(self):
+        T = np.zeros((4, 4))
+        call("yas_map_get_pose", self.h, _ptr(T))
+        return T
+
+    def Fuse(self, depth, T):
+        """Fuse(GpuMat depth_f32_metres, SE3d Tcm) (VoxelMap.cu:825-884)."""
+        assert depth.dtype == torch.float32 and depth.dim() == 2
+        call("yas_map_fuse", self.h, _dev(depth), _pitch(depth), depth.shape[1], depth.shape[0], _ptr(_f64(T)))
+
+    def FuseU16(self, depth, T, depth_scale=1000.0):
+        assert depth.dtype in (torch.int16, torch.uint16) and depth.dim() == 2
+        call("yas_map_fuse_u16", self.h, _dev(depth), _pitch(depth), depth.shape[1], depth.shape[0],
+             depth_scale, _ptr(_f64(T)))
+
+    def CheckNumVisibleBlocks(self, cols, rows, T):
+        n = ctypes.c_int()
+        call("yas_map_check_visible_blocks", self.h, cols, rows, _ptr(_f64(T)), ctypes.byref(n))
+        return n.value
+
+    def stats(self):
+        s = MapStats()
+        call("yas_map_get_stats", self.h, ctypes.byref(s))
+        return s
+
+    def export(self):
+        """(keys int32 (n,3), raw voxels (n, 512*bytes) uint8) in pool order."""
+        n = self.stats().num_allocated
+        vb = native.lib().yas_map_voxel_bytes(self.h)
+        keys = np.zeros((max(n, 1), 3), np.int32)
+        raw = np.zeros((max(n, 1), 512 * vb), np.uint8)
+        cnt = ctypes.c_int()
+        call("yas_map_export", self.h, _ptr(keys), _ptr(raw), n, ctypes.byref(cnt))
+        return keys[:n], raw[:n]
+
+    def export_sorted(self):
+        """Blocks sorted by (x, y, z): keys, sdf (n,512) float32, weight (n,512) float32."""
+        keys, raw = self.export()
+        order = np.lexsort((keys[:, 2], keys[:, 1], keys[:, 0]))
+        sdf, wt = decode_voxels(raw[order], self.voxel_format)
+        return keys[order], sdf, wt
+
+
+class RayTraceEngine:
+    """RayTraceEngine (RGBDFusion/include/RayTraceEngine.h:11-48)."""
+
+    def __init__(self, w, h, K, ctx=None):
+        self.ctx = ctx or default_context()
+        self.w, self.h_ = w, h
+        Kf = np.asarray(K, np.float32)
+        h_ = ctypes.c_void_p()
+        call("yas_raycaster_create", self.ctx.h, w, h, _ptr(Kf), ctypes.byref(h_))
+        self.h = h_
+
+    def close(self):
+        if getattr(self, "h", None):
+            native.lib().yas_raycaster_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def RayTrace(self, map_struct, T):
+        call("yas_raycast", self.h, map_struct.h, _ptr(_f64(T)))
+
+    def vmap_ptr(self):
+        p = ctypes.c_void_p()
+        pitch = ctypes.c_size_t()
+        call("yas_raycaster_vmap", self.h, ctypes.byref(p), ctypes.byref(pitch))
+        return p.value, pitch.value
+
+    def GetVMap(self):
+        """Host copy of the last vertex map, (h, w, 4) float32."""
+        self.ctx.synchronize()
+        p, _ = self.vmap_ptr()
+        return download(p, (self.h_, self.w, 4), np.float32)
+
+    def GetZRange(self):
+        self.ctx.synchronize()
+        zmin, zmax = ctypes.c_void_p(), ctypes.c_void_p()
+        cols, rows = ctypes.c_int(), ctypes.c_int()
+        call("yas_raycaster_zrange", self.h, ctypes.byref(zmin), ctypes.byref(zmax), ctypes.byref(cols),
+             ctypes.byref(rows))
+        shape = (rows.value, cols.value)
+        return download(zmin.value, shape, np.float32), download(zmax.value, shape, np.float32)
+
+    def GetNumVisibleBlock(self):
+        n = ctypes.c_int()
+        call("yas_raycaster_num_visible", self.h, ctypes.byref(n))
+        return n.value
+
+
+class MeshEngine:
+    """MeshEngine (RGBDFusion/include/MeshEngine.h:8-24)."""
+
+    def __init__(self, max_triangles, ctx=None):
+        self.ctx = ctx or default_context()
+        h = ctypes.c_void_p()
+        call("yas_mesher_create", self.ctx.h, max_triangles, ctypes.byref(h))
+        self.h = h
+        self.max_triangles = max_triangles
+
+    def close(self):
+        if getattr(self, "h", None):
+            native.lib().yas_mesher_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def Meshify(self, map_struct, with_cases=False):
+        n = ctypes.c_int()
+        call("yas_mesh_extract", self.h, map_struct.h, ctypes.byref(n), allow=(6,))
+        ntri = n.value
+        keep = min(ntri, self.max_triangles)
+        tris = np.zeros((max(keep, 1), 9), np.float32)
+        normals = np.zeros((max(keep, 1), 9), np.float32)
+        call("yas_mesher_download", self.h, _ptr(tris), _ptr(normals), keep)
+        nb = ctypes.c_int()
+        call("yas_mesher_blocks", self.h, None, None, None, None, 0, ctypes.byref(nb))
+        nbk = nb.value
+        keys = np.zeros((max(nbk, 1), 3), np.int32)
+        first = np.zeros(max(nbk, 1), np.int32)
+        count = np.zeros(max(nbk, 1), np.int32)
+        cases = np.zeros((max(nbk, 1), 512), np.int16) if with_cases else None
+        call("yas_mesher_blocks", self.h, _ptr(keys), _ptr(first), _ptr(count),
+             _ptr(cases) if with_cases else None, nbk, ctypes.byref(nb))
+        out = dict(num_triangles=ntri, tris=tris[:keep], normals=normals[:keep], keys=keys[:nbk],
+                   first=first[:nbk], count=count[:nbk])
+        if with_cases:
+            out["cases"] = cases[:nbk]
+        return out
+
+
+def canonical_mesh(m):
+    """Reorder a mesh's blocks by key (x, y, z); triangles keep their
+    within-block order.  Returns (keys, counts, cases|None, tris, normals)."""
+    keys = m["keys"]
+    order = np.lexsort((keys[:, 2], keys[:, 1], keys[:, 0]))
+    idx = np.concatenate([np.arange(m["first"][i], m["first"][i] + m["count"][i]) for i in order]) \
+        if len(order) else np.zeros(0, np.int64)
+    idx = idx.astype(np.int64)
+    cases = m["cases"][order] if "cases" in m else None
+    return keys[order], m["count"][order], cases, m["tris"][idx], m["normals"][idx]
+
+
+class RGBDTracking:
+    """RGBDTracking (RGBDFusion/include/RGBDTracking.h:9-107)."""
+
+    def __init__(self, w, h, K, levels=3, iters=(10, 5, 3), mode=TRACK_DEPTH, weights=(1.0, 1.0, 100.0, 10.0),
+                 ctx=None):
+        self.ctx = ctx or default_context()
+        self.w, self.h_ = w, h
+        it = (ctypes.c_int * 5)(*(list(iters) + [0] * (5 - len(iters))))
+        self.params = TrackerParams(levels, it, mode, *weights)
+        Kd = _f64(K)
+        h_ = ctypes.c_void_p()
+        call("yas_tracker_create", self.ctx.h, w, h, _ptr(Kd), ctypes.byref(self.params), ctypes.byref(h_))
+        self.h = h_
+
+    def close(self):
+        if getattr(self, "h", None):
+            native.lib().yas_tracker_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def SetReferenceImage(self, gray):
+        call("yas_tracker_set_reference_image", self.h, _dev(gray), _pitch(gray))
+
+    def SetTrackingImage(self, gray):
+        call("yas_tracker_set_tracking_image", self.h, _dev(gray), _pitch(gray))
+
+    def SetReferenceDepth(self, depth):
+        call("yas_tracker_set_reference_depth", self.h, _dev(depth), _pitch(depth))
+
+    def SetTrackingDepth(self, depth):
+        call("yas_tracker_set_tracking_depth", self.h, _dev(depth), _pitch(depth))
+
+    def SetReferenceModel(self, vmap):
+        """vmap: torch (h, w, 4) float32 on the device, or (ptr, pitch)."""
+        if isinstance(vmap, tuple):
+            call("yas_tracker_set_reference_model", self.h, ctypes.c_void_p(vmap[0]), vmap[1])
+        else:
+            call("yas_tracker_set_reference_model", self.h, _dev(vmap), _pitch(vmap))
+
+    def GetTransform(self, init=None, swap=True):
+        init = np.eye(4) if init is None else init
+        out = np.zeros((4, 4))
+        st = TrackStats()
+        call("yas_tracker_get_transform", self.h, _ptr(_f64(init)), 1 if swap else 0, _ptr(out), ctypes.byref(st),
+             allow=(5,))
+        return out, st
+
+    def SwapFrameBuffer(self):
+        call("yas_tracker_swap_frame_buffer", self.h)
+
+    def IsTrackingGood(self):
+        g = ctypes.c_int()
+        call("yas_tracker_is_tracking_good", self.h, ctypes.byref(g))
+        return bool(g.value)
+
+    def step_sums(self, level, T, kind):
+        out = np.zeros(29, np.float32)
+        stats = np.zeros(4, np.float32)
+        call("yas_tracker_step_sums", self.h, level, _ptr(_f64(T)), kind, _ptr(out), _ptr(stats))
+        return out, stats
+
+    def level(self, which, lvl):
+        self.ctx.synchronize()
+        p = ctypes.c_void_p()
+        pitch = ctypes.c_size_t()
+        w, h = ctypes.c_int(), ctypes.c_int()
+        call("yas_tracker_level", self.h, which, lvl, ctypes.byref(p), ctypes.byref(pitch), ctypes.byref(w),
+             ctypes.byref(h))
+        if which < 6:
+            return download(p.value, (h.value, w.value), np.float32)
+        return download(p.value, (h.value, w.value, 4), np.float32)
+
+
+class DensePipeline:
+    """Tracking::TrackRGBD's dense steps on one volume (yas_pipeline_*)."""
+
+    def __init__(self, w, h, K, map_params, tracker_params, depth_scale=1000.0, gray_mode=0, ctx=None):
+        self.ctx = ctx or default_context()
+        self.w, self.h_ = w, h
+        Kf = np.asarray(K, np.float32)
+        self.map_params = map_params
+        self.tracker_params = tracker_params
+        h_ = ctypes.c_void_p()
+        call("yas_pipeline_create", self.ctx.h, w, h, _ptr(Kf), ctypes.byref(map_params),
+             ctypes.byref(tracker_params), depth_scale, gray_mode, ctypes.byref(h_))
+        self.h = h_
+
+    def close(self):
+        if getattr(self, "h", None):
+            native.lib().yas_pipeline_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def step(self, depth_u16, rgb_u8):
+        call("yas_pipeline_step", self.h, _dev(depth_u16), _pitch(depth_u16), _dev(rgb_u8), _pitch(rgb_u8))
+
+    def pose(self, frame):
+        T = np.zeros((4, 4))
+        tr = ctypes.c_int()
+        call("yas_pipeline_pose", self.h, frame, _ptr(T), ctypes.byref(tr))
+        return T, bool(tr.value)
+
+    def num_frames(self):
+        n = ctypes.c_int()
+        call("yas_pipeline_num_frames", self.h, ctypes.byref(n))
+        return n.value
+
+    def map_handle(self):
+        m = ctypes.c_void_p()
+        call("yas_pipeline_map", self.h, ctypes.byref(m))
+        return m
+
+    def export_map_sorted(self, voxel_format):
+        """Sorted blocks of the pipeline's volume (see MapStruct.export_sorted)."""
+        view = MapStruct.__new__(MapStruct)
+        view.h = self.map_handle()
+        view.voxel_format = voxel_format
+        try:
+            return MapStruct.export_sorted(view)
+        finally:
+            view.h = None
+
+
+def map_params(num_buckets, num_blocks, voxel_size, trunc_dist, voxel_format=VOXEL_F32_F32, max_weight=255,
+               depth_min=0.1, depth_max=3.0):
+    return MapParams(num_buckets, num_blocks, voxel_size, trunc_dist, voxel_format, max_weight, depth_min, depth_max)
+
+
+def tracker_params(levels=3, iters=(10, 5, 3), mode=TRACK_DEPTH, weights=(1.0, 1.0, 100.0, 10.0)):
+    it = (ctypes.c_int * 5)(*(list(iters) + [0] * (5 - len(iters))))
+    return TrackerParams(levels, it, mode, *weights)
