@@ -253,6 +253,20 @@ int yas_pipeline_step(yas_pipeline* pl, const uint16_t* depth, size_t depth_pitc
  * (1) or rejected (0).  Synchronous. */
 int yas_pipeline_pose(yas_pipeline* pl, int frame, double T[16], int* tracked);
 int yas_pipeline_num_frames(yas_pipeline* pl, int* n);
+/* Per-stage device timing with HIP events on the context's stream (the
+ * reference only brackets TrackRGBD with std::chrono, Examples/tum_rgbd.cpp:56-60).
+ * While enabled, every frame after the first records its stage boundaries;
+ * yas_pipeline_timing synchronises and accumulates them: stage_ms[8] =
+ * {input conversion, visibility + z-range, ray march, tracker inputs
+ * (model + frame pyramids), Gauss-Newton solve, block allocation, block
+ * visibility, TSDF integration}, frame_ms = whole-frame time of each timed
+ * frame (at most `capacity`), *num_frames = frames timed so far. */
+int yas_pipeline_enable_timing(yas_pipeline* pl, int enable);
+int yas_pipeline_timing(yas_pipeline* pl, double stage_ms[8], double* frame_ms, int capacity, int* num_frames);
+int yas_pipeline_reset_timing(yas_pipeline* pl);
+/* Counters of frames [first, first+count): per frame {allocated blocks,
+ * visible blocks, capacity flags, Bresenham walk steps (cumulative)}. Synchronous. */
+int yas_pipeline_frame_stats(yas_pipeline* pl, int first, int count, int* out);
 int yas_pipeline_map(yas_pipeline* pl, yas_map** map);
 int yas_pipeline_raycaster(yas_pipeline* pl, yas_raycaster** rc);
 int yas_pipeline_tracker(yas_pipeline* pl, yas_tracker** trk);

@@ -393,17 +393,30 @@ static int launch_integrate(yas_map* map, const float* depth, size_t pitch, int 
 }
 
 // Allocation + visibility + integration with the pose already in pose_dev.
-// A non-null `gate` pointing at 0 turns the call into a no-op on the device.
-int yas_map_fuse_prepared(yas_map* map, const float* depth, size_t pitch, int W, int H, const int* gate) {
-  hipStream_t s = map->ctx->stream;
+// The three fusion stages with the pose already in pose_dev.  A non-null
+// `gate` pointing at 0 turns allocation and visibility into no-ops on the
+// device (integration then sees no visible blocks).
+int yas_map_alloc_stage(yas_map* map, const float* depth, size_t pitch, int W, int H, const int* gate) {
   dim3 g((W + 15) / 16, (H + 15) / 16);
-  hipLaunchKernelGGL(k_alloc, g, dim3(256), 0, s, map->v, depth, pitch, W, H, map->K, (const PosePair*)map->pose_dev,
-                     gate);
+  hipLaunchKernelGGL(k_alloc, g, dim3(256), 0, map->ctx->stream, map->v, depth, pitch, W, H, map->K,
+                     (const PosePair*)map->pose_dev, gate);
   YAS_LAUNCH_CHECK();
-  hipLaunchKernelGGL(k_visible, dim3(grid_for(map->P.num_blocks, 256, 4096)), dim3(256), 0, s, map->v,
+  return YAS_OK;
+}
+int yas_map_visible_stage(yas_map* map, int W, int H, const int* gate) {
+  hipLaunchKernelGGL(k_visible, dim3(grid_for(map->P.num_blocks, 256, 4096)), dim3(256), 0, map->ctx->stream, map->v,
                      (const PosePair*)map->pose_dev, map->K, W, H, gate);
   YAS_LAUNCH_CHECK();
+  return YAS_OK;
+}
+int yas_map_integrate_stage(yas_map* map, const float* depth, size_t pitch, int W, int H) {
   return launch_integrate(map, depth, pitch, W, H);
+}
+int yas_map_fuse_prepared(yas_map* map, const float* depth, size_t pitch, int W, int H, const int* gate) {
+  int rc = yas_map_alloc_stage(map, depth, pitch, W, H, gate);
+  if (!rc) rc = yas_map_visible_stage(map, W, H, gate);
+  if (!rc) rc = yas_map_integrate_stage(map, depth, pitch, W, H);
+  return rc;
 }
 
 int yas_map_prepare_pose_dev(yas_map* map, const double* T_dev) {

@@ -15,7 +15,11 @@ int yas_map_fuse_prepared(yas_map* map, const float* depth, size_t pitch, int W,
 int yas_map_prepare_pose_dev(yas_map* map, const double* T_dev);
 int yas_convert_depth_u16(yas_ctx* ctx, const uint16_t* src, size_t sp, float* dst, size_t dp, int W, int H,
                           float depth_scale);
-int yas_raycast_dev(yas_raycaster* rc, yas_map* map, const double* T_dev);
+int yas_raycast_dev(yas_raycaster* rc, yas_map* map, const double* T_dev, hipEvent_t before_march);
+int yas_map_alloc_stage(yas_map* map, const float* depth, size_t pitch, int W, int H, const int* gate);
+int yas_map_visible_stage(yas_map* map, int W, int H, const int* gate);
+int yas_map_integrate_stage(yas_map* map, const float* depth, size_t pitch, int W, int H);
+MapView yas_map_view(yas_map* map);
 float4* yas_raycaster_vmap_dev(yas_raycaster* rc);
 int yas_tracker_solve_async(yas_tracker* t);
 int yas_tracker_init_identity_async(yas_tracker* t);
@@ -57,8 +61,17 @@ __global__ void k_set_identity(double* T) {
 
 }  // namespace
 
+// Stage boundaries recorded per frame while timing is on.
+constexpr int kStages = 8;  // conversion, visibility+z-range, march, tracker inputs, solve, alloc, visible, integrate
+
 struct yas_pipeline {
   yas_ctx* ctx;
+  bool timing;
+  std::vector<std::vector<hipEvent_t>> pending;  // kStages + 1 events per timed frame
+  std::vector<std::vector<hipEvent_t>> spare;
+  double stage_ms[kStages];
+  std::vector<double> frame_ms;
+  int* frame_stats;  // [cap][4]: allocated, visible, flags, walk steps
   int W, H;
   float depth_scale;
   int gray_mode;
@@ -88,6 +101,15 @@ static int grow(yas_pipeline* p) {
     hipFree(p->poses);
     hipFree(p->tracked);
   }
+  int* ns = nullptr;
+  YAS_HIP(hipMalloc((void**)&ns, sizeof(int) * 4 * (size_t)ncap));
+  YAS_HIP(hipMemsetAsync(ns, 0, sizeof(int) * 4 * (size_t)ncap, p->ctx->stream));
+  if (p->cap) {
+    YAS_HIP(hipMemcpyAsync(ns, p->frame_stats, sizeof(int) * 4 * p->cap, hipMemcpyDeviceToDevice, p->ctx->stream));
+    YAS_HIP(hipStreamSynchronize(p->ctx->stream));
+    hipFree(p->frame_stats);
+  }
+  p->frame_stats = ns;
   p->poses = np;
   p->tracked = nt;
   p->cap = ncap;
@@ -102,8 +124,14 @@ int yas_pipeline_create(yas_ctx* ctx, int W, int H, const float K[4], const yas_
   YAS_REQUIRE(depth_scale > 0.f, "yas_pipeline_create: depth_scale must be > 0");
   YAS_HIP(hipSetDevice(ctx->device));
   yas_pipeline* p = new yas_pipeline();
-  std::memset(p, 0, sizeof(*p));
   p->ctx = ctx;
+  p->timing = false;
+  std::memset(p->stage_ms, 0, sizeof(p->stage_ms));
+  p->map = nullptr; p->rc = nullptr; p->trk = nullptr;
+  p->depth = p->gray = nullptr;
+  p->T_last = p->poses = nullptr;
+  p->tracked = p->gate = p->frame_stats = nullptr;
+  p->frames = p->cap = 0;
   p->W = W;
   p->H = H;
   p->depth_scale = depth_scale;
@@ -148,6 +176,10 @@ int yas_pipeline_destroy(yas_pipeline* p) {
   hipFree(p->poses);
   hipFree(p->tracked);
   hipFree(p->gate);
+  hipFree(p->frame_stats);
+  for (auto* v : {&p->pending, &p->spare})
+    for (auto& ev : *v)
+      for (hipEvent_t e : ev) hipEventDestroy(e);
   delete p;
   return YAS_OK;
 }
@@ -159,26 +191,49 @@ int yas_pipeline_step(yas_pipeline* p, const uint16_t* depth, size_t dpitch, con
   if (p->frames >= p->cap && (rc = grow(p))) return rc;
   const int W = p->W, H = p->H;
   const size_t fp = (size_t)W * sizeof(float);
+  hipStream_t s = p->ctx->stream;
+  // events of this frame (timed frames after the first only)
+  hipEvent_t ev[kStages + 1] = {};
+  const bool timed = p->timing && p->frames > 0;
+  if (timed) {
+    std::vector<hipEvent_t> set;
+    if (!p->spare.empty()) {
+      set = std::move(p->spare.back());
+      p->spare.pop_back();
+    } else {
+      set.resize(kStages + 1);
+      for (auto& e : set) YAS_HIP(hipEventCreate(&e));
+    }
+    for (int i = 0; i <= kStages; ++i) ev[i] = set[i];
+    p->pending.push_back(std::move(set));
+  }
+  auto mark = [&](int i) -> int {
+    if (timed) YAS_HIP(hipEventRecord(ev[i], s));
+    return YAS_OK;
+  };
+  if ((rc = mark(0))) return rc;
   if ((rc = yas_convert_depth_u16(p->ctx, depth, dpitch, p->depth, fp, W, H, p->depth_scale))) return rc;
   if ((rc = yas_rgb_to_gray_async(p->ctx, rgb, cpitch, p->gray, fp, W, H, p->gray_mode))) return rc;
-  hipStream_t s = p->ctx->stream;
+  if ((rc = mark(1))) return rc;
   double* pose_out = p->poses + 16 * (size_t)p->frames;
   int* tracked_out = p->tracked + p->frames;
+  const int* gate = nullptr;
   if (p->frames == 0) {
     // StereoInitialization (Tracking.cpp:139-155)
     if ((rc = yas_tracker_set_reference_image(p->trk, p->gray, fp))) return rc;
     if ((rc = yas_tracker_set_reference_depth(p->trk, p->depth, fp))) return rc;
     if ((rc = yas_map_prepare_pose_dev(p->map, p->T_last))) return rc;
-    if ((rc = yas_map_fuse_prepared(p->map, p->depth, fp, W, H, nullptr))) return rc;
     hipLaunchKernelGGL(k_record_first, dim3(1), dim3(64), 0, s, (const double*)p->T_last, pose_out, tracked_out);
     YAS_LAUNCH_CHECK();
   } else {
     // TrackRGBD (Tracking.cpp:162-197)
-    if ((rc = yas_raycast_dev(p->rc, p->map, p->T_last))) return rc;
+    if ((rc = yas_raycast_dev(p->rc, p->map, p->T_last, ev[2]))) return rc;
+    if ((rc = mark(3))) return rc;
     if ((rc = yas_tracker_set_reference_model(p->trk, (const float*)yas_raycaster_vmap_dev(p->rc), (size_t)W * 16)))
       return rc;
     if ((rc = yas_tracker_set_tracking_image_dev(p->trk, p->gray, fp))) return rc;
     if ((rc = yas_tracker_set_tracking_depth_dev(p->trk, p->depth, fp))) return rc;
+    if ((rc = mark(4))) return rc;
     if ((rc = yas_tracker_init_identity_async(p->trk))) return rc;
     if ((rc = yas_tracker_solve_async(p->trk))) return rc;
     yas_tracker_swap_frame_buffer(p->trk);
@@ -186,8 +241,18 @@ int yas_pipeline_step(yas_pipeline* p, const uint16_t* depth, size_t dpitch, con
                        p->T_last, pose_out, tracked_out, p->gate);
     YAS_LAUNCH_CHECK();
     if ((rc = yas_map_prepare_pose_dev(p->map, p->T_last))) return rc;
-    if ((rc = yas_map_fuse_prepared(p->map, p->depth, fp, W, H, p->gate))) return rc;
+    gate = p->gate;
   }
+  if ((rc = mark(5))) return rc;
+  if ((rc = yas_map_alloc_stage(p->map, p->depth, fp, W, H, gate))) return rc;
+  if ((rc = mark(6))) return rc;
+  if ((rc = yas_map_visible_stage(p->map, W, H, gate))) return rc;
+  if ((rc = mark(7))) return rc;
+  if ((rc = yas_map_integrate_stage(p->map, p->depth, fp, W, H))) return rc;
+  if ((rc = mark(8))) return rc;
+  // per-frame counters: allocated, visible, flags, walk steps
+  MapView v = yas_map_view(p->map);
+  YAS_HIP(hipMemcpyAsync(p->frame_stats + 4 * (size_t)p->frames, v.ctr, sizeof(int) * 4, hipMemcpyDeviceToDevice, s));
   p->frames++;
   return YAS_OK;
 }
@@ -199,6 +264,55 @@ int yas_pipeline_pose(yas_pipeline* p, int frame, double T[16], int* tracked) {
   YAS_HIP(hipStreamSynchronize(p->ctx->stream));
   YAS_HIP(hipMemcpy(T, p->poses + 16 * (size_t)frame, sizeof(double) * 16, hipMemcpyDeviceToHost));
   if (tracked) YAS_HIP(hipMemcpy(tracked, p->tracked + frame, sizeof(int), hipMemcpyDeviceToHost));
+  return YAS_OK;
+}
+
+int yas_pipeline_enable_timing(yas_pipeline* p, int enable) {
+  YAS_REQUIRE(p, "null pipeline");
+  p->timing = enable != 0;
+  return YAS_OK;
+}
+
+int yas_pipeline_timing(yas_pipeline* p, double stage_ms[8], double* frame_ms, int capacity, int* num_frames) {
+  YAS_REQUIRE(p, "null pipeline");
+  YAS_HIP(hipStreamSynchronize(p->ctx->stream));
+  for (auto& set : p->pending) {
+    double total = 0.0;
+    for (int i = 0; i < kStages; ++i) {
+      float ms = 0.0f;
+      YAS_HIP(hipEventElapsedTime(&ms, set[i], set[i + 1]));
+      p->stage_ms[i] += ms;
+    }
+    float whole = 0.0f;
+    YAS_HIP(hipEventElapsedTime(&whole, set[0], set[kStages]));
+    total = whole;
+    p->frame_ms.push_back(total);
+    p->spare.push_back(std::move(set));
+  }
+  p->pending.clear();
+  if (stage_ms) std::memcpy(stage_ms, p->stage_ms, sizeof(p->stage_ms));
+  int n = (int)p->frame_ms.size();
+  if (frame_ms)
+    for (int i = 0; i < n && i < capacity; ++i) frame_ms[i] = p->frame_ms[i];
+  if (num_frames) *num_frames = n;
+  return YAS_OK;
+}
+
+int yas_pipeline_reset_timing(yas_pipeline* p) {
+  YAS_REQUIRE(p, "null pipeline");
+  double tmp[8];
+  int rc = yas_pipeline_timing(p, tmp, nullptr, 0, nullptr);
+  if (rc) return rc;
+  std::memset(p->stage_ms, 0, sizeof(p->stage_ms));
+  p->frame_ms.clear();
+  return YAS_OK;
+}
+
+int yas_pipeline_frame_stats(yas_pipeline* p, int first, int count, int* out) {
+  YAS_REQUIRE(p && out, "null argument");
+  YAS_REQUIRE(first >= 0 && count >= 0 && first + count <= p->frames, "yas_pipeline_frame_stats: bad frame range");
+  YAS_HIP(hipStreamSynchronize(p->ctx->stream));
+  if (count) YAS_HIP(hipMemcpy(out, p->frame_stats + 4 * (size_t)first, sizeof(int) * 4 * count, hipMemcpyDeviceToHost));
   return YAS_OK;
 }
 

@@ -202,7 +202,7 @@ __global__ void __launch_bounds__(256) k_march(MapView m, const PosePair* pose, 
   vmap[(size_t)y * W + x] = out;
 }
 
-static int launch_raycast(yas_raycaster* rc, yas_map* map) {
+static int launch_raycast(yas_raycaster* rc, yas_map* map, hipEvent_t before_march = nullptr) {
   hipStream_t s = rc->ctx->stream;
   MapView v = yas_map_view(map);
   const PosePair* pose = yas_map_pose_dev(map);
@@ -212,6 +212,7 @@ static int launch_raycast(yas_raycaster* rc, yas_map* map) {
   if (g > 2048) g = 2048;
   hipLaunchKernelGGL(k_zrange, dim3(g), dim3(256), 0, s, v, pose, rc->K, rc->cols, rc->rows, rc->zmin, rc->zmax);
   YAS_LAUNCH_CHECK();
+  if (before_march) YAS_HIP(hipEventRecord(before_march, s));
   const float vsinv = (float)(1.0 / (double)v.voxel_size);
   const float step_scale = v.trunc / v.voxel_size;
   dim3 grid((rc->W + 15) / 16, (rc->H + 15) / 16);
@@ -234,10 +235,11 @@ static int launch_raycast(yas_raycaster* rc, yas_map* map) {
 }
 
 // Ray cast with the pose held in device memory (pipeline).
-int yas_raycast_dev(yas_raycaster* rc, yas_map* map, const double* T_dev) {
+// `before_march` (optional) is recorded between the z-range and march kernels.
+int yas_raycast_dev(yas_raycaster* rc, yas_map* map, const double* T_dev, hipEvent_t before_march) {
   int r = yas_map_visible_async_dev(map, T_dev, rc->W, rc->H);
   if (r) return r;
-  return launch_raycast(rc, map);
+  return launch_raycast(rc, map, before_march);
 }
 
 float4* yas_raycaster_vmap_dev(yas_raycaster* rc) { return rc->vmap; }

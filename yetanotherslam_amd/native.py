@@ -92,6 +92,10 @@ SIGNATURES = {
     "yas_pipeline_step": (I, [P, P, SZ, P, SZ]),
     "yas_pipeline_pose": (I, [P, I, P, PI]),
     "yas_pipeline_num_frames": (I, [P, PI]),
+    "yas_pipeline_enable_timing": (I, [P, I]),
+    "yas_pipeline_timing": (I, [P, P, P, I, PI]),
+    "yas_pipeline_reset_timing": (I, [P]),
+    "yas_pipeline_frame_stats": (I, [P, I, I, P]),
     "yas_pipeline_map": (I, [P, PP]),
     "yas_pipeline_raycaster": (I, [P, PP]),
     "yas_pipeline_tracker": (I, [P, PP]),
