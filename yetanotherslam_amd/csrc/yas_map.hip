@@ -215,6 +215,9 @@ __device__ inline void pixel_block(int x, int y, float z, const Intr& K, const P
 __global__ void __launch_bounds__(256) k_alloc(MapView m, const float* depth, size_t pitch, int W, int H, Intr K,
                                                const PosePair* pose, const int* gate) {
   if (gate && *gate == 0) return;
+  __shared__ int walked;
+  if (threadIdx.x == 0) walked = 0;
+  __syncthreads();
   const int x = blockIdx.x * 16 + (threadIdx.x & 15);
   const int y = blockIdx.y * 16 + (threadIdx.x >> 4);
   const PoseF T = pose->T;
@@ -265,7 +268,7 @@ __global__ void __launch_bounds__(256) k_alloc(MapView m, const float* depth, si
     if (has && !ok) atomicOr(&m.ctr[kCtrFlags], kFlagKeyRange);
     uint64_t key = ok ? pack_key(px, py, pz) : 0ull;
     int home = ok ? ref_hash(px, py, pz, m.num_buckets) : 0;
-    if (lane_id() == __ffsll(act) - 1) atomicAdd(&m.ctr[kCtrWalk], __popcll(act));
+    if (lane_id() == __ffsll(act) - 1) atomicAdd(&walked, __popcll(act));
     wave_insert(m, key, ok, home);
     if (has) {
       --remaining;
@@ -278,6 +281,8 @@ __global__ void __launch_bounds__(256) k_alloc(MapView m, const float* depth, si
       }
     }
   }
+  __syncthreads();
+  if (threadIdx.x == 0 && walked) atomicAdd(&m.ctr[kCtrWalk], walked);
 }
 
 // VoxelStructUtils.h:112-147.

@@ -2,16 +2,22 @@
 // (RGBDFusion/src/RGBDTracking.cu, RGBDFusion/include/TrackingUtils.h) and the
 // image operations it uses (RGBDFusion/src/ImageProc.cu).
 //
-// MI355X design: the whole Gauss-Newton loop stays on the device.  Each
-// linearisation is one grid-stride kernel whose workgroups reduce their
-// per-pixel products (28 upper-triangle entries of [J|r]^T[J|r] plus a count)
-// through wavefront butterflies and LDS; the last workgroup to finish (atomic
-// ticket) folds the per-workgroup partials in a fixed order, so results are
-// deterministic.  A one-workgroup kernel then solves the 6x6 system in FP64
-// (the reference's Eigen LDLT, restated), applies T <- exp(xi) T in FP64 and
-// keeps the best pose, all in device memory.  The host only enqueues kernels:
-// no per-iteration download (the reference blocks on ~4 downloads per
-// iteration, RGBDTracking.cu:252-288,434-437).
+// MI355X design: the whole Gauss-Newton loop stays on the device, with no
+// host round trip and no grid-wide synchronisation inside a kernel (on the
+// multi-XCD gfx950 an agent-scope fence writes back the L2, so kernel
+// boundaries are the cheaper barrier).  One iteration is:
+//   pass 1   grid-stride over the level's pixels: ICP point-to-plane rows
+//            (28 upper-triangle products + count) and photometric residuals
+//            with per-thread Welford statistics; each workgroup reduces its
+//            values (transpose butterfly across the 64 lanes, LDS across
+//            waves) into one row of partials.
+//   pass 2   (photometric) every workgroup folds pass 1's statistics rows in
+//            the same fixed order, so all agree on mean, sigma and the Huber
+//            threshold, then reduces the weighted photometric rows.
+//   update   one workgroup folds the partial rows in a fixed order, solves
+//            the 6x6 system in FP64 (the reference's Eigen LDLT, restated),
+//            applies T <- exp(xi) T in FP64 and keeps the best pose.
+// Results are deterministic run to run.
 #include <cfloat>
 #include <cmath>
 #include <cstring>
@@ -41,63 +47,92 @@ struct TrackState {
   int good;
   int icp_inliers;
   int rgb_inliers;
-  unsigned ticket[4];
 };
 
 // ------------------------------------------------------------------ reductions
-// Sum N floats over the workgroup; the result is valid in thread 0.
-template <int N>
-__device__ inline void block_sum(float (&v)[N], float* lds /* [4 * N] */) {
+// Wave-wide sums of 32 values by a transpose butterfly: at each step a lane
+// keeps half of its values and trades the other half with its partner, so
+// 32 values need 16+8+4+2+1+1 shuffles.  On return, lanes 2i and 2i+1 hold
+// the total of value i.  The step size is a template constant so every
+// register index is static.
+template <int S>
+__device__ inline void butterfly_step(float (&v)[32], int lane) {
+  const bool hi = (lane & (2 * S)) != 0;
+#pragma unroll
+  for (int i = 0; i < S; ++i) {
+    const float send = hi ? v[i] : v[i + S];
+    const float keep = hi ? v[i + S] : v[i];
+    v[i] = keep + __shfl_xor(send, 2 * S, 64);
+  }
+}
+__device__ inline float wave_sum32(float (&v)[32]) {
+  const int lane = threadIdx.x & 63;
+  butterfly_step<16>(v, lane);
+  butterfly_step<8>(v, lane);
+  butterfly_step<4>(v, lane);
+  butterfly_step<2>(v, lane);
+  butterfly_step<1>(v, lane);
+  return v[0] + __shfl_xor(v[0], 1, 64);
+}
+
+// Workgroup sums of 32 values; thread t < 32 returns the total of value t.
+__device__ inline float block_sum32(float (&v)[32], float* lds /* [nwaves * 32] */) {
+  const float w = wave_sum32(v);
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  if ((lane & 1) == 0) lds[wid * 32 + (lane >> 1)] = w;
+  __syncthreads();
+  float t = 0.0f;
+  if (threadIdx.x < 32) {
+    const int nw = blockDim.x >> 6;
+    t = lds[threadIdx.x];
+    for (int k = 1; k < nw; ++k) t += lds[k * 32 + threadIdx.x];
+  }
+  __syncthreads();
+  return t;
+}
+
+// Chan et al.'s pairwise update of (count, mean, M2) statistics.
+__device__ inline void chan_merge(float& n, float& mean, float& m2, float nb, float meanb, float m2b) {
+  const float nn = n + nb;
+  if (nb == 0.0f) return;
+  if (n == 0.0f) {
+    n = nb; mean = meanb; m2 = m2b;
+    return;
+  }
+  const float d = meanb - mean;
+  mean = mean + d * (nb / nn);
+  m2 = m2 + m2b + d * d * (n * nb / nn);
+  n = nn;
+}
+
+// Workgroup merge of per-thread statistics; the result is broadcast to all threads.
+__device__ inline void block_stats(float& n, float& mean, float& m2, float* lds /* [3 * nwaves] */) {
 #pragma unroll
-  for (int i = 0; i < N; ++i) v[i] = wave_sum(v[i]);
-  if (lane == 0)
-#pragma unroll
-    for (int i = 0; i < N; ++i) lds[wid * N + i] = v[i];
+  for (int o = 1; o < 64; o <<= 1) {
+    float nb = __shfl_xor(n, o, 64), mb = __shfl_xor(mean, o, 64), qb = __shfl_xor(m2, o, 64);
+    // merge in a lane-independent order so both partners get identical values
+    if ((threadIdx.x & o) == 0) {
+      chan_merge(n, mean, m2, nb, mb, qb);
+    } else {
+      float n0 = nb, m0 = mb, q0 = qb;
+      chan_merge(n0, m0, q0, n, mean, m2);
+      n = n0; mean = m0; m2 = q0;
+    }
+  }
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  if (lane == 0) {
+    lds[3 * wid] = n; lds[3 * wid + 1] = mean; lds[3 * wid + 2] = m2;
+  }
   __syncthreads();
   if (threadIdx.x == 0) {
     const int nw = blockDim.x >> 6;
-#pragma unroll
-    for (int i = 0; i < N; ++i) {
-      float s = lds[i];
-      for (int w = 1; w < nw; ++w) s += lds[w * N + i];
-      v[i] = s;
-    }
+    float a = lds[0], b = lds[1], c = lds[2];
+    for (int k = 1; k < nw; ++k) chan_merge(a, b, c, lds[3 * k], lds[3 * k + 1], lds[3 * k + 2]);
+    lds[0] = a; lds[1] = b; lds[2] = c;
   }
   __syncthreads();
-}
-
-// Store this workgroup's N sums; the last workgroup to arrive folds all
-// partials (row r = workgroup r) and returns true with the totals in `out`
-// (thread 0).  The ticket is re-armed for the next pass.
-template <int N>
-__device__ inline bool grid_sum(float (&v)[N], float* partials, unsigned* ticket, float* lds, float (&out)[N]) {
-  __shared__ bool last;
-  block_sum<N>(v, lds);
-  if (threadIdx.x == 0) {
-#pragma unroll
-    for (int i = 0; i < N; ++i) partials[blockIdx.x * 32 + i] = v[i];
-    __threadfence();
-    unsigned t = atomicAdd(ticket, 1u);
-    last = (t == gridDim.x - 1);
-  }
+  n = lds[0]; mean = lds[1]; m2 = lds[2];
   __syncthreads();
-  if (!last) return false;
-  __threadfence();
-  float r[N];
-#pragma unroll
-  for (int i = 0; i < N; ++i) r[i] = 0.0f;
-  for (int g = threadIdx.x; g < (int)gridDim.x; g += blockDim.x)
-#pragma unroll
-    for (int i = 0; i < N; ++i)
-      r[i] += __hip_atomic_load(&partials[g * 32 + i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  block_sum<N>(r, lds);
-  if (threadIdx.x == 0) {
-#pragma unroll
-    for (int i = 0; i < N; ++i) out[i] = r[i];
-    *ticket = 0u;
-  }
-  return true;
 }
 
 __device__ inline void load_pose_f(const double* T, float* lds_pose /* 12 */) {
@@ -106,7 +141,6 @@ __device__ inline void load_pose_f(const double* T, float* lds_pose /* 12 */) {
     int k = j < 3 ? 3 * i + j : 9 + i;
     lds_pose[k] = (float)T[4 * i + j];
   }
-  __syncthreads();
 }
 
 // ------------------------------------------------------------------ image ops
@@ -250,79 +284,71 @@ __global__ void __launch_bounds__(256) k_vmap_to_depth(const float4* v, size_t s
   ((float*)((char*)d + (size_t)y * dp))[x] = a.w > 0.f ? a.z : 0.0f;
 }
 
-// ------------------------------------------------------------------ ICP
-// RGBDTracking.cu:305-406: projective association of each reference vertex
-// into the current frame, gated at sin(20 deg) normal angle and 1 cm.
-__global__ void __launch_bounds__(256) k_icp(Level L, const float4* ref_v, const float4* ref_n, const float4* cur_v,
-                                             const float4* cur_n, TrackState* st, float* partials, float angle_th,
-                                             float dist_th, int record_inliers) {
-  __shared__ float pose[12];
-  __shared__ float lds[4 * 29];
-  load_pose_f(st->est, pose);
-  PoseF T;
+// ------------------------------------------------------------------ per-pixel terms
+struct TrackBufs {
+  const float4* ref_v;
+  const float4* ref_n;
+  const float4* cur_v;
+  const float4* cur_n;
+  const float* ref_inv;
+  const float* ref_int;
+  const float* cur_int;
+  const float* gx;
+  const float* gy;
+  float4* warped;
+  float4* resid;
+};
+
+// RGBDTracking.cu:316-372: projective association of reference pixel k into
+// the current frame, gated at sin(20 deg) normal angle and 1 cm; on a match
+// the point-to-plane row [n, v x n, n.(v_c - v)] is accumulated.
+__device__ inline void icp_pixel(const Level& L, const TrackBufs& B, const PoseF& T, float angle_th, float dist_th,
+                                 int k, float (&acc)[32]) {
+  const float4 vl = B.ref_v[k];
+  if (!(vl.w >= 0.0f)) return;
+  float vx, vy, vz;
+  xform(T, vl.x, vl.y, vl.z, vx, vy, vz);
+  const float invz = 1.0f / vz;
+  const int u = floor_sat(L.fx * vx * invz + L.cx + 0.5f);
+  const int v = floor_sat(L.fy * vy * invz + L.cy + 0.5f);
+  if (u < 0 || v < 0 || u >= L.w || v >= L.h) return;
+  const float4 vc = B.cur_v[v * L.w + u];
+  if (vc.w < 0.0f) return;
+  const float4 nl = B.ref_n[k];
+  float nx, ny, nz;
+  rot3(T, nl.x, nl.y, nl.z, nx, ny, nz);
+  const float4 nc = B.cur_n[v * L.w + u];
+  const float dx = vx - vc.x, dy = vy - vc.y, dz = vz - vc.z;
+  const float dist = sqrtf(dx * dx + dy * dy + dz * dz);
+  const float c0 = nc.y * nz - nc.z * ny, c1 = nc.z * nx - nc.x * nz, c2 = nc.x * ny - nc.y * nx;
+  const float ang = sqrtf(c0 * c0 + c1 * c1 + c2 * c2);
+  if (!(ang < angle_th && dist <= dist_th && nl.w > 0.0f && nc.w > 0.0f)) return;
+  float row[7];
+  row[0] = nx; row[1] = ny; row[2] = nz;
+  row[3] = vy * nz - vz * ny;
+  row[4] = vz * nx - vx * nz;
+  row[5] = vx * ny - vy * nx;
+  row[6] = nx * (vc.x - vx) + ny * (vc.y - vy) + nz * (vc.z - vz);
+  int c = 0;
 #pragma unroll
-  for (int i = 0; i < 9; ++i) T.r[i] = pose[i];
-  T.t[0] = pose[9]; T.t[1] = pose[10]; T.t[2] = pose[11];
-  float acc[29];
+  for (int i = 0; i < 7; ++i)
 #pragma unroll
-  for (int i = 0; i < 29; ++i) acc[i] = 0.0f;
-  const int N = L.w * L.h;
-  for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < N; k += gridDim.x * blockDim.x) {
-    const float4 vl = ref_v[k];
-    if (!(vl.w >= 0.0f)) continue;
-    float vx, vy, vz;
-    xform(T, vl.x, vl.y, vl.z, vx, vy, vz);
-    float invz = 1.0f / vz;
-    int u = floor_sat(L.fx * vx * invz + L.cx + 0.5f);
-    int v = floor_sat(L.fy * vy * invz + L.cy + 0.5f);
-    if (u < 0 || v < 0 || u >= L.w || v >= L.h) continue;
-    const float4 vc = cur_v[v * L.w + u];
-    if (vc.w < 0.0f) continue;
-    const float4 nl = ref_n[k];
-    float nx, ny, nz;
-    rot3(T, nl.x, nl.y, nl.z, nx, ny, nz);
-    const float4 nc = cur_n[v * L.w + u];
-    float dx = vx - vc.x, dy = vy - vc.y, dz = vz - vc.z;
-    float dist = sqrtf(dx * dx + dy * dy + dz * dz);
-    float c0 = nc.y * nz - nc.z * ny, c1 = nc.z * nx - nc.x * nz, c2 = nc.x * ny - nc.y * nx;
-    float ang = sqrtf(c0 * c0 + c1 * c1 + c2 * c2);
-    if (!(ang < angle_th && dist <= dist_th && nl.w > 0.0f && nc.w > 0.0f)) continue;
-    float row[7];
-    row[0] = nx; row[1] = ny; row[2] = nz;
-    row[3] = vy * nz - vz * ny;
-    row[4] = vz * nx - vx * nz;
-    row[5] = vx * ny - vy * nx;
-    row[6] = nx * (vc.x - vx) + ny * (vc.y - vy) + nz * (vc.z - vz);
-    int c = 0;
-#pragma unroll
-    for (int i = 0; i < 7; ++i)
-#pragma unroll
-      for (int j = i; j < 7; ++j) acc[c++] += row[i] * row[j];
-    acc[28] += 1.0f;
-  }
-  float out[29];
-  if (grid_sum<29>(acc, partials, &st->ticket[0], lds, out) && threadIdx.x == 0) {
-#pragma unroll
-    for (int i = 0; i < 29; ++i) st->sums_icp[i] = out[i];
-    if (record_inliers) st->icp_inliers = (int)out[28];
-  }
+    for (int j = i; j < 7; ++j) acc[c++] += row[i] * row[j];
+  acc[28] += 1.0f;
 }
 
-// ------------------------------------------------------------------ photometric
 // Warp parameters of ImageProc.cu:42-78: RK = R K^-1 (formed in double), t.
 __device__ inline void load_warp(const double* T, const double* Kd, float* lds /* 12 */) {
-  if (threadIdx.x < 12) {
-    int i = threadIdx.x / 4, j = threadIdx.x % 4;
-    float v;
+  if (threadIdx.x >= 16 && threadIdx.x < 28) {
+    int q = threadIdx.x - 16;
+    int i = q / 4, j = q % 4;
     if (j < 3) {
       double Ki[9] = {1.0 / Kd[0], 0.0, -Kd[2] / Kd[0], 0.0, 1.0 / Kd[1], -Kd[3] / Kd[1], 0.0, 0.0, 1.0};
-      v = (float)(T[4 * i] * Ki[j] + T[4 * i + 1] * Ki[3 + j] + T[4 * i + 2] * Ki[6 + j]);
-      lds[3 * i + j] = v;
+      lds[3 * i + j] = (float)(T[4 * i] * Ki[j] + T[4 * i + 1] * Ki[3 + j] + T[4 * i + 2] * Ki[6 + j]);
     } else {
       lds[9 + i] = (float)T[4 * i + 3];
     }
   }
-  __syncthreads();
 }
 
 __device__ inline float bilerp(const float* m, int w, float x, float y) {
@@ -333,87 +359,108 @@ __device__ inline float bilerp(const float* m, int w, float x, float y) {
   return (r0[u] * (1.0f - cx) + r0[u + 1] * cx) * (1.0f - cy) + (r1[u] * (1.0f - cx) + r1[u + 1] * cx) * cy;
 }
 
-// TrackingUtils.h:73-151 with the warp of ImageProc.cu:42-61 fused in.
-__global__ void __launch_bounds__(256) k_rgb_residual(Level L, const float* ref_inv, const float* ref_int,
-                                                      const float* cur_int, const float* gxs, const float* gys,
-                                                      float4* warped, float4* resid, TrackState* st, float* partials) {
+// TrackingUtils.h:73-151 with the warp of ImageProc.cu:42-61 fused in; the
+// residual statistics are accumulated with Welford's update.
+__device__ inline void rgb_pixel(const Level& L, const TrackBufs& B, const float* wp, int k, float& n, float& mean,
+                                 float& m2) {
+  const int y = k / L.w, x = k - y * L.w;
+  const float iz = B.ref_inv[k];
+  float4 p = make_float4(0.f, 0.f, 0.f, -1.f);
+  if (iz > 0.0f) {
+    float z = 1.0f / iz;
+    float px = (wp[0] * (float)x + wp[1] * (float)y + wp[2]) * z + wp[9];
+    float py = (wp[3] * (float)x + wp[4] * (float)y + wp[5]) * z + wp[10];
+    float pz = (wp[6] * (float)x + wp[7] * (float)y + wp[8]) * z + wp[11];
+    p = make_float4(px, py, pz, 1.0f);
+  }
+  B.warped[k] = p;
+  float r = 0.0f, gx = 0.0f, gy = 0.0f;
+  bool ok = false;
+  if (p.w > 0.0f) {
+    float u = L.fx * p.x / p.z + L.cx;
+    float v = L.fy * p.y / p.z + L.cy;
+    if (u > 0.0f && v > 0.0f && u < (float)(L.w - 1) && v < (float)(L.h - 1)) {
+      float ref = B.ref_int[k];
+      float cur = bilerp(B.cur_int, L.w, u, v);
+      r = cur - ref;
+      gx = bilerp(B.gx, L.w, u, v);
+      gy = bilerp(B.gy, L.w, u, v);
+      ok = (gx * gx + gy * gy) > 0.0f && isfinite(r);
+    }
+  }
+  B.resid[k] = make_float4(r, gx, gy, ok ? 0.5f : -0.5f);
+  if (ok) {
+    n += 1.0f;
+    float d = r - mean;
+    mean = mean + d / n;
+    m2 = m2 + d * (r - mean);
+  }
+}
+
+// Pass 1: ICP rows and/or photometric residuals of one linearisation.
+template <bool ICP, bool RGB>
+__global__ void __launch_bounds__(256) k_pass1(Level L, TrackBufs B, const TrackState* st, float* part_icp,
+                                               float* part_res, float angle_th, float dist_th) {
+  __shared__ float pose[12];
   __shared__ float wp[12];
-  __shared__ float lds[4 * 2];
-  load_warp(st->est, L.Kd, wp);
-  float acc[2] = {0.0f, 0.0f};
-  const int N = L.w * L.h;
-  for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < N; k += gridDim.x * blockDim.x) {
-    const int y = k / L.w, x = k - y * L.w;
-    const float iz = ref_inv[k];
-    float4 p = make_float4(0.f, 0.f, 0.f, -1.f);
-    if (iz > 0.0f) {
-      float z = 1.0f / iz;
-      float px = (wp[0] * (float)x + wp[1] * (float)y + wp[2]) * z + wp[9];
-      float py = (wp[3] * (float)x + wp[4] * (float)y + wp[5]) * z + wp[10];
-      float pz = (wp[6] * (float)x + wp[7] * (float)y + wp[8]) * z + wp[11];
-      p = make_float4(px, py, pz, 1.0f);
-    }
-    warped[k] = p;
-    float r = 0.0f, gx = 0.0f, gy = 0.0f;
-    bool ok = false;
-    if (p.w > 0.0f) {
-      float u = L.fx * p.x / p.z + L.cx;
-      float v = L.fy * p.y / p.z + L.cy;
-      if (u > 0.0f && v > 0.0f && u < (float)(L.w - 1) && v < (float)(L.h - 1)) {
-        float ref = ref_int[k];
-        float cur = bilerp(cur_int, L.w, u, v);
-        r = cur - ref;
-        gx = bilerp(gxs, L.w, u, v);
-        gy = bilerp(gys, L.w, u, v);
-        ok = (gx * gx + gy * gy) > 0.0f && isfinite(r);
-      }
-    }
-    resid[k] = make_float4(r, gx, gy, ok ? 0.5f : -0.5f);
-    if (ok) {
-      acc[0] += r;
-      acc[1] += 1.0f;
-    }
-  }
-  float out[2];
-  if (grid_sum<2>(acc, partials, &st->ticket[1], lds, out) && threadIdx.x == 0) {
-    st->rgb_stats[0] = out[0];
-    st->rgb_stats[1] = out[1];
-    st->rgb_stats[2] = out[0] / out[1];
-  }
-}
-
-// TrackingUtils.h:578-607, then sigma and the Huber threshold
-// 4.685 * sigma (RGBDTracking.cu:270-278).
-__global__ void __launch_bounds__(256) k_rgb_variance(Level L, const float4* resid, TrackState* st, float* partials) {
-  __shared__ float lds[4];
-  const float mean = st->rgb_stats[2];
-  float acc[1] = {0.0f};
-  const int N = L.w * L.h;
-  for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < N; k += gridDim.x * blockDim.x) {
-    float4 e = resid[k];
-    if (e.w > 0.0f) acc[0] += (e.x - mean) * (e.x - mean);
-  }
-  float out[1];
-  if (grid_sum<1>(acc, partials, &st->ticket[2], lds, out) && threadIdx.x == 0) {
-    float sigma = sqrtf(out[0] / (st->rgb_stats[1] - 1.0f));
-    st->rgb_stats[3] = (float)(4.685 * (double)sigma);
-  }
-}
-
-// TrackingUtils.h:153-220: Huber-weighted photometric rows.
-__global__ void __launch_bounds__(256) k_rgb_jacobian(Level L, const float4* warped, const float4* resid,
-                                                      TrackState* st, float* partials, int record_inliers) {
-  __shared__ float lds[4 * 29];
-  const float th = st->rgb_stats[3];
-  float acc[29];
+  __shared__ float lds[4 * 32];
+  load_pose_f(st->est, pose);
+  if (RGB) load_warp(st->est, L.Kd, wp);
+  __syncthreads();
+  PoseF T;
 #pragma unroll
-  for (int i = 0; i < 29; ++i) acc[i] = 0.0f;
+  for (int i = 0; i < 9; ++i) T.r[i] = pose[i];
+  T.t[0] = pose[9]; T.t[1] = pose[10]; T.t[2] = pose[11];
+  float acc[32];
+#pragma unroll
+  for (int i = 0; i < 32; ++i) acc[i] = 0.0f;
+  float n = 0.0f, mean = 0.0f, m2 = 0.0f;
   const int N = L.w * L.h;
   for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < N; k += gridDim.x * blockDim.x) {
-    const float4 e = resid[k];
+    if (ICP) icp_pixel(L, B, T, angle_th, dist_th, k, acc);
+    if (RGB) rgb_pixel(L, B, wp, k, n, mean, m2);
+  }
+  if (ICP) {
+    float t = block_sum32(acc, lds);
+    if (threadIdx.x < 32) part_icp[blockIdx.x * 32 + threadIdx.x] = t;
+  }
+  if (RGB) {
+    block_stats(n, mean, m2, lds);
+    if (threadIdx.x == 0) {
+      part_res[blockIdx.x * 4] = n;
+      part_res[blockIdx.x * 4 + 1] = mean;
+      part_res[blockIdx.x * 4 + 2] = m2;
+    }
+  }
+}
+
+// Pass 2 (TrackingUtils.h:153-220, RGBDTracking.cu:262-290): fold the
+// residual statistics of pass 1 (identically in every workgroup), derive
+// sigma and the Huber threshold 4.685 sigma, and reduce the weighted rows.
+__global__ void __launch_bounds__(256) k_pass2(Level L, TrackBufs B, TrackState* st, const float* part_res, int g1,
+                                               float* part_rgb) {
+  __shared__ float lds[4 * 32];
+  float n = 0.0f, mean = 0.0f, m2 = 0.0f;
+  for (int g = threadIdx.x; g < g1; g += blockDim.x)
+    chan_merge(n, mean, m2, part_res[g * 4], part_res[g * 4 + 1], part_res[g * 4 + 2]);
+  block_stats(n, mean, m2, lds);
+  const float sigma = sqrtf(m2 / (n - 1.0f));
+  const float th = (float)(4.685 * (double)sigma);
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    st->rgb_stats[0] = n * mean;
+    st->rgb_stats[1] = n;
+    st->rgb_stats[2] = mean;
+    st->rgb_stats[3] = th;
+  }
+  float acc[32];
+#pragma unroll
+  for (int i = 0; i < 32; ++i) acc[i] = 0.0f;
+  const int N = L.w * L.h;
+  for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < N; k += gridDim.x * blockDim.x) {
+    const float4 e = B.resid[k];
     if (!(e.w > 0.0f)) continue;
-    const float4 p = warped[k];
-    float zinv = 1.0f / p.z;
+    const float4 p = B.warped[k];
+    const float zinv = 1.0f / p.z;
     float wt = 1.0f;
     if (fabsf(e.x) > th) wt = th / fabsf(e.x);
     float row[7];
@@ -431,19 +478,14 @@ __global__ void __launch_bounds__(256) k_rgb_jacobian(Level L, const float4* war
       for (int j = i; j < 7; ++j) acc[c++] += wt * row[i] * row[j];
     acc[28] += 1.0f;
   }
-  float out[29];
-  if (grid_sum<29>(acc, partials, &st->ticket[3], lds, out) && threadIdx.x == 0) {
-#pragma unroll
-    for (int i = 0; i < 29; ++i) st->sums_rgb[i] = out[i];
-    if (record_inliers) st->rgb_inliers = (int)out[28];
-  }
+  float t = block_sum32(acc, lds);
+  if (threadIdx.x < 32) part_rgb[blockIdx.x * 32 + threadIdx.x] = t;
 }
 
 // ------------------------------------------------------------------ GN update
 // Eigen 3.3 LDLT restated (diagonal pivoting, pseudo-inverse of D).
-__device__ inline void ldlt_solve6(double* A, double* y) {
+__device__ inline void ldlt_solve6(double* A, double* y, int* tr) {
   const int n = 6;
-  int tr[6];
   for (int k = 0; k < n; ++k) {
     int big = k;
     double bv = fabs(A[k * n + k]);
@@ -536,46 +578,92 @@ __device__ inline void unpack29(const float* s, float* H, float* b) {
     }
 }
 
-// One Gauss-Newton step of RGBDTracking.cu:161-204.
-__global__ void k_gn_update(TrackState* st, int mode, int first_of_level, float wrh, float wrb, float wih, float wib) {
+// Fold the partial rows of one linearisation and, when `do_step`, take one
+// Gauss-Newton step of RGBDTracking.cu:161-204.  One workgroup.
+__global__ void __launch_bounds__(256) k_update(TrackState* st, const float* part_icp, int gi, const float* part_rgb,
+                                                int gr, int mode, int first_of_level, int do_step, int record,
+                                                float wrh, float wrb, float wih, float wib) {
+  __shared__ float lds[4 * 32];
+  __shared__ float sums[2][32];
+  __shared__ double A[36];
+  __shared__ double y[6];
+  __shared__ int tr[6];
+  for (int which = 0; which < 2; ++which) {
+    const float* part = which == 0 ? part_icp : part_rgb;
+    const int g = which == 0 ? gi : gr;
+    if (g == 0) continue;
+    float v[32];
+#pragma unroll
+    for (int i = 0; i < 32; ++i) v[i] = 0.0f;
+    for (int r = threadIdx.x; r < g; r += blockDim.x)
+#pragma unroll
+      for (int i = 0; i < 32; ++i) v[i] += part[r * 32 + i];
+    float t = block_sum32(v, lds);
+    if (threadIdx.x < 32) sums[which][threadIdx.x] = t;
+  }
+  __syncthreads();
   if (threadIdx.x != 0) return;
+  if (gi) {
+    for (int i = 0; i < 29; ++i) st->sums_icp[i] = sums[0][i];
+    if (record) st->icp_inliers = (int)sums[0][28];
+  }
+  if (gr) {
+    for (int i = 0; i < 29; ++i) st->sums_rgb[i] = sums[1][i];
+    if (record) st->rgb_inliers = (int)sums[1][28];
+  }
+  if (!do_step) return;
   if (first_of_level) st->last_err = FLT_MAX;
   if (!st->good) return;
   float H[36], b[6];
+#pragma unroll
   for (int i = 0; i < 36; ++i) H[i] = 0.0f;
+#pragma unroll
   for (int i = 0; i < 6; ++i) b[i] = 0.0f;
   float rsum = 0.0f, nres = 0.0f;
   if (mode != YAS_TRACK_DEPTH) {
     float Hr[36], br[6];
-    unpack29(st->sums_rgb, Hr, br);
-    float wh = mode == YAS_TRACK_RGBD ? wrh : 1.0f, wb = mode == YAS_TRACK_RGBD ? wrb : 1.0f;
+    unpack29(sums[1], Hr, br);
+    const float wh = mode == YAS_TRACK_RGBD ? wrh : 1.0f, wb = mode == YAS_TRACK_RGBD ? wrb : 1.0f;
+#pragma unroll
     for (int i = 0; i < 36; ++i) H[i] = H[i] + wh * Hr[i];
+#pragma unroll
     for (int i = 0; i < 6; ++i) b[i] = b[i] + wb * br[i];
-    rsum = st->sums_rgb[27];
-    nres = st->sums_rgb[28];
+    rsum = sums[1][27];
+    nres = sums[1][28];
   }
   if (mode != YAS_TRACK_RGB) {
     float Hi[36], bi[6];
-    unpack29(st->sums_icp, Hi, bi);
-    float wh = mode == YAS_TRACK_RGBD ? wih : 1.0f, wb = mode == YAS_TRACK_RGBD ? wib : 1.0f;
+    unpack29(sums[0], Hi, bi);
+    const float wh = mode == YAS_TRACK_RGBD ? wih : 1.0f, wb = mode == YAS_TRACK_RGBD ? wib : 1.0f;
+#pragma unroll
     for (int i = 0; i < 36; ++i) H[i] = H[i] + wh * Hi[i];
+#pragma unroll
     for (int i = 0; i < 6; ++i) b[i] = b[i] + wb * bi[i];
-    rsum = st->sums_icp[27];
+    rsum = sums[0][27];
   }
-  float err = sqrtf(rsum) / (nres + 6.0f);
-  double A[36], x[6];
+  const float err = sqrtf(rsum) / (nres + 6.0f);
+#pragma unroll
   for (int i = 0; i < 36; ++i) A[i] = (double)H[i];
-  for (int i = 0; i < 6; ++i) x[i] = (double)b[i];
-  ldlt_solve6(A, x);
-  if (isnan(x[0])) {
+#pragma unroll
+  for (int i = 0; i < 6; ++i) y[i] = (double)b[i];
+  ldlt_solve6(A, y, tr);
+  double xi[6];
+#pragma unroll
+  for (int i = 0; i < 6; ++i) xi[i] = y[i];
+  if (isnan(xi[0])) {
     st->good = 0;
     return;
   }
-  double E[16];
-  se3_exp(x, E);
-  mul44(E, st->est, st->est);
+  double E[16], est[16];
+  se3_exp(xi, E);
+#pragma unroll
+  for (int i = 0; i < 16; ++i) est[i] = st->est[i];
+  mul44(E, est, est);
+#pragma unroll
+  for (int i = 0; i < 16; ++i) st->est[i] = est[i];
   if (err < st->last_err) {
-    for (int i = 0; i < 16; ++i) st->last_ok[i] = st->est[i];
+#pragma unroll
+    for (int i = 0; i < 16; ++i) st->last_ok[i] = est[i];
     st->last_err = err;
   }
 }
@@ -607,7 +695,9 @@ struct yas_tracker {
   float *cur_inv[kMaxLevels], *ref_inv[kMaxLevels];
   float4 *cur_v[kMaxLevels], *cur_n[kMaxLevels], *ref_v[kMaxLevels], *ref_n[kMaxLevels];
   float4 *warped[kMaxLevels], *resid[kMaxLevels];
-  float* partials;
+  float* part_icp;  // [kRedGroups][32]
+  float* part_res;  // [kRedGroups][4]
+  float* part_rgb;  // [kRedGroups][32]
   TrackState* st;
   TrackState* host_st;
   float angle_th, dist_th;
@@ -664,42 +754,52 @@ static int model_pyramid(yas_tracker* t) {
   return YAS_OK;
 }
 
-static int launch_icp(yas_tracker* t, int l, bool rec) {
-  const Level& a = t->lv[l];
-  hipLaunchKernelGGL(k_icp, dim3(kRedGroups), dim3(256), 0, t->ctx->stream, a, (const float4*)t->ref_v[l],
-                     (const float4*)t->ref_n[l], (const float4*)t->cur_v[l], (const float4*)t->cur_n[l], t->st,
-                     t->partials, t->angle_th, t->dist_th, rec ? 1 : 0);
-  YAS_LAUNCH_CHECK();
-  return YAS_OK;
+static int groups_for(const Level& a) {
+  int n = (a.w * a.h + 1023) / 1024;  // about 4 pixels per thread
+  return n < 1 ? 1 : (n > kRedGroups ? kRedGroups : n);
 }
 
-static int launch_rgb(yas_tracker* t, int l, bool rec) {
+static TrackBufs bufs(yas_tracker* t, int l) {
+  TrackBufs B;
+  B.ref_v = t->ref_v[l]; B.ref_n = t->ref_n[l]; B.cur_v = t->cur_v[l]; B.cur_n = t->cur_n[l];
+  B.ref_inv = t->ref_inv[l]; B.ref_int = t->ref_int[l]; B.cur_int = t->cur_int[l];
+  B.gx = t->gx[l]; B.gy = t->gy[l]; B.warped = t->warped[l]; B.resid = t->resid[l];
+  return B;
+}
+
+// One linearisation at level l (icp / rgb terms as requested) followed by the
+// fold and, when do_step, one Gauss-Newton step.
+static int launch_iteration(yas_tracker* t, int l, bool icp, bool rgb, bool do_step, bool first_of_level) {
   const Level& a = t->lv[l];
   hipStream_t s = t->ctx->stream;
-  hipLaunchKernelGGL(k_rgb_residual, dim3(kRedGroups), dim3(256), 0, s, a, (const float*)t->ref_inv[l],
-                     (const float*)t->ref_int[l], (const float*)t->cur_int[l], (const float*)t->gx[l],
-                     (const float*)t->gy[l], t->warped[l], t->resid[l], t->st, t->partials);
-  hipLaunchKernelGGL(k_rgb_variance, dim3(kRedGroups), dim3(256), 0, s, a, (const float4*)t->resid[l], t->st,
-                     t->partials);
-  hipLaunchKernelGGL(k_rgb_jacobian, dim3(kRedGroups), dim3(256), 0, s, a, (const float4*)t->warped[l],
-                     (const float4*)t->resid[l], t->st, t->partials, rec ? 1 : 0);
+  const int G = groups_for(a);
+  TrackBufs B = bufs(t, l);
+  if (icp && rgb)
+    hipLaunchKernelGGL((k_pass1<true, true>), dim3(G), dim3(256), 0, s, a, B, (const TrackState*)t->st, t->part_icp,
+                       t->part_res, t->angle_th, t->dist_th);
+  else if (icp)
+    hipLaunchKernelGGL((k_pass1<true, false>), dim3(G), dim3(256), 0, s, a, B, (const TrackState*)t->st, t->part_icp,
+                       t->part_res, t->angle_th, t->dist_th);
+  else
+    hipLaunchKernelGGL((k_pass1<false, true>), dim3(G), dim3(256), 0, s, a, B, (const TrackState*)t->st, t->part_icp,
+                       t->part_res, t->angle_th, t->dist_th);
+  if (rgb)
+    hipLaunchKernelGGL(k_pass2, dim3(G), dim3(256), 0, s, a, B, t->st, (const float*)t->part_res, G, t->part_rgb);
+  hipLaunchKernelGGL(k_update, dim3(1), dim3(256), 0, s, t->st, (const float*)t->part_icp, icp ? G : 0,
+                     (const float*)t->part_rgb, rgb ? G : 0, t->P.mode, first_of_level ? 1 : 0, do_step ? 1 : 0,
+                     l == 0 ? 1 : 0, t->P.w_rgb_h, t->P.w_rgb_b, t->P.w_icp_h, t->P.w_icp_b);
   YAS_LAUNCH_CHECK();
   return YAS_OK;
 }
 
 // Coarse-to-fine Gauss-Newton from the pose already in the device state.
 int yas_tracker_solve_async(yas_tracker* t) {
-  hipStream_t s = t->ctx->stream;
-  for (int l = t->P.levels - 1; l >= 0; --l) {
+  const bool icp = t->P.mode != YAS_TRACK_RGB, rgb = t->P.mode != YAS_TRACK_DEPTH;
+  for (int l = t->P.levels - 1; l >= 0; --l)
     for (int it = 0; it < t->P.iters[l]; ++it) {
-      int rc;
-      if (t->P.mode != YAS_TRACK_DEPTH && (rc = launch_rgb(t, l, l == 0))) return rc;
-      if (t->P.mode != YAS_TRACK_RGB && (rc = launch_icp(t, l, l == 0))) return rc;
-      hipLaunchKernelGGL(k_gn_update, dim3(1), dim3(64), 0, s, t->st, t->P.mode, it == 0 ? 1 : 0, t->P.w_rgb_h,
-                         t->P.w_rgb_b, t->P.w_icp_h, t->P.w_icp_b);
-      YAS_LAUNCH_CHECK();
+      int rc = launch_iteration(t, l, icp, rgb, true, it == 0);
+      if (rc) return rc;
     }
-  }
   return YAS_OK;
 }
 
@@ -765,7 +865,9 @@ int yas_tracker_create(yas_ctx* ctx, int W, int H, const double K[4], const yas_
     for (float4** p : {&t->cur_v[l], &t->cur_n[l], &t->ref_v[l], &t->ref_n[l], &t->warped[l], &t->resid[l]})
       alloc((void**)p, n * sizeof(float4));
   }
-  alloc((void**)&t->partials, sizeof(float) * 32 * kRedGroups);
+  alloc((void**)&t->part_icp, sizeof(float) * 32 * kRedGroups);
+  alloc((void**)&t->part_res, sizeof(float) * 4 * kRedGroups);
+  alloc((void**)&t->part_rgb, sizeof(float) * 32 * kRedGroups);
   alloc((void**)&t->st, sizeof(TrackState));
   if (e == hipSuccess) e = hipHostMalloc((void**)&t->host_st, sizeof(TrackState), hipHostMallocDefault);
   if (e != hipSuccess) {
@@ -799,7 +901,9 @@ int yas_tracker_destroy(yas_tracker* t) {
     for (float* p : {t->cur_int[l], t->ref_int[l], t->gx[l], t->gy[l], t->cur_inv[l], t->ref_inv[l]}) hipFree(p);
     for (float4* p : {t->cur_v[l], t->cur_n[l], t->ref_v[l], t->ref_n[l], t->warped[l], t->resid[l]}) hipFree(p);
   }
-  hipFree(t->partials);
+  hipFree(t->part_icp);
+  hipFree(t->part_res);
+  hipFree(t->part_rgb);
   hipFree(t->st);
   if (t->host_st) hipHostFree(t->host_st);
   delete t;
@@ -883,7 +987,7 @@ int yas_tracker_step_sums(yas_tracker* t, int level, const double T[16], int kin
   Mat44 M;
   std::memcpy(M.m, T, sizeof(M.m));
   hipLaunchKernelGGL(k_track_init, dim3(1), dim3(64), 0, t->ctx->stream, t->st, M);
-  int rc = kind == 0 ? launch_icp(t, level, false) : launch_rgb(t, level, false);
+  int rc = launch_iteration(t, level, kind == 0, kind != 0, false, false);
   if (rc) return rc;
   YAS_HIP(hipMemcpyAsync(t->host_st, t->st, sizeof(TrackState), hipMemcpyDeviceToHost, t->ctx->stream));
   YAS_HIP(hipStreamSynchronize(t->ctx->stream));
