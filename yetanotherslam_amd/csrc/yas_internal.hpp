@@ -252,9 +252,20 @@ inline int voxel_bytes(int fmt) { return fmt == YAS_VOXEL_F32_F32 ? 8 : 4; }
 // ------------------------------------------------------------------ map view
 // Device view of one volume: open-addressed hash (keys/vals), the block
 // pool (block_keys/voxels) and counters.
+// One hash slot: the packed block key and its pool index side by side, so a
+// probe is one 16-byte load.
+struct alignas(16) HashSlot {
+  unsigned long long key;  // kEmptyKey when free
+  int val;                 // pool index, -1 if the pool overflowed
+  int flags;               // kBlock* summary of the block's voxels
+};
+// Exact summaries of a block's 512 voxels, maintained by allocation (a new
+// block is all unobserved) and integration (recomputed after every update).
+enum : int { kBlockUnobserved = 1, kBlockFree = 2 };  // all w == 0 / all w > 0 with sdf == 1
+
 struct MapView {
-  unsigned long long* keys;        // [num_buckets], kEmptyKey when free
-  int* vals;                       // [num_buckets], pool index or -1
+  HashSlot* slots;                 // [num_buckets]
+  int* block_slot;                 // [num_blocks] hash slot of each pool block
   unsigned long long* block_keys;  // [num_blocks]
   void* voxels;                    // [num_blocks * 512] of the storage type
   int* visible;                    // [num_blocks] pool indices of visible blocks
@@ -268,21 +279,35 @@ struct MapView {
   int format;
 };
 
-// Linear-probe lookup from the reference hash's home slot.
-__device__ inline int find_block(const MapView& m, uint64_t key, int home) {
+// Linear-probe lookup from the reference hash's home slot: the pool index
+// (-1 if absent) and the block's summary flags.
+__device__ inline int find_block(const MapView& m, uint64_t key, int home, int* flags = nullptr) {
   int slot = home;
   const int lim = m.num_buckets < kMaxProbe ? m.num_buckets : kMaxProbe;
   for (int p = 0; p < lim; ++p) {
-    uint64_t k = m.keys[slot];
-    if (k == key) return m.vals[slot];
-    if (k == kEmptyKey) return -1;
+    const HashSlot e = m.slots[slot];
+    if (e.key == key) {
+      if (flags) *flags = e.flags;
+      return e.val;
+    }
+    if (e.key == kEmptyKey) return -1;
     slot = slot + 1 == m.num_buckets ? 0 : slot + 1;
   }
   return -1;
 }
-__device__ inline int find_block_xyz(const MapView& m, int bx, int by, int bz) {
+__device__ inline int find_block_xyz(const MapView& m, int bx, int by, int bz, int* flags = nullptr) {
   if (!key_in_range(bx, by, bz)) return -1;
-  return find_block(m, pack_key(bx, by, bz), ref_hash(bx, by, bz, m.num_buckets));
+  return find_block(m, pack_key(bx, by, bz), ref_hash(bx, by, bz, m.num_buckets), flags);
+}
+
+// ------------------------------------------------------------------ XCD mapping
+// Workgroups are dealt to the 8 XCDs round-robin (workgroup b runs on XCD
+// b % 8).  This bijection from launch order to tile index gives XCD j one
+// contiguous run of tiles, so neighbouring image tiles share that XCD's L2.
+__device__ inline int xcd_tile(int b, int T) {
+  const int j = b & 7, k = b >> 3;
+  const int q = T >> 3, r = T & 7;
+  return j * q + (j < r ? j : r) + k;
 }
 
 // ------------------------------------------------------------------ wave64

@@ -3,8 +3,8 @@
 // gfx950.
 //
 // Layout in HBM:
-//   keys[num_buckets]        u64 packed block coordinate, ~0 = free slot
-//   vals[num_buckets]        i32 pool index of the slot's block
+//   slots[num_buckets]       16 B: u64 packed block coordinate (~0 = free)
+//                            and the i32 pool index of its block
 //   block_keys[num_blocks]   u64 key of each pool block (dense list of the
 //                            allocated blocks; visibility scans this list
 //                            instead of the whole table)
@@ -102,10 +102,13 @@ void* yas_ctx_stream(yas_ctx* ctx) { return ctx ? (void*)ctx->stream : nullptr; 
 
 // ------------------------------------------------------------------ kernels
 
-__global__ void __launch_bounds__(256) k_reset_hash(unsigned long long* keys, int* vals, int n) {
+__global__ void __launch_bounds__(256) k_reset_hash(HashSlot* slots, int n) {
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-    keys[i] = kEmptyKey;
-    vals[i] = -1;
+    HashSlot e;
+    e.key = kEmptyKey;
+    e.val = -1;
+    e.flags = 0;
+    slots[i] = e;
   }
 }
 
@@ -163,10 +166,10 @@ __device__ inline void wave_insert(const MapView& m, uint64_t key, bool has, int
     const int lim = m.num_buckets < kMaxProbe ? m.num_buckets : kMaxProbe;
     int p = 0;
     for (; p < lim; ++p) {
-      unsigned long long k = __hip_atomic_load(&m.keys[slot], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      unsigned long long k = __hip_atomic_load(&m.slots[slot].key, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       if (k == key) break;
       if (k == kEmptyKey) {
-        unsigned long long old = atomicCAS(&m.keys[slot], kEmptyKey, (unsigned long long)key);
+        unsigned long long old = atomicCAS(&m.slots[slot].key, kEmptyKey, (unsigned long long)key);
         if (old == kEmptyKey) {
           won = true;
           break;
@@ -187,9 +190,11 @@ __device__ inline void wave_insert(const MapView& m, uint64_t key, bool has, int
       int idx = base + __popcll(wm & lane_mask_lt());
       if (idx < m.num_blocks) {
         m.block_keys[idx] = key;
-        m.vals[slot] = idx;
+        m.block_slot[idx] = slot;
+        m.slots[slot].flags = kBlockUnobserved;
+        m.slots[slot].val = idx;
       } else {
-        m.vals[slot] = -1;
+        m.slots[slot].val = -1;
         atomicOr(&m.ctr[kCtrFlags], kFlagPoolOverflow);
       }
     }
@@ -332,7 +337,9 @@ __global__ void __launch_bounds__(256) k_visible(MapView m, const PosePair* pose
 }
 
 // K5 (VoxelMap.cu:719-786): one wavefront per visible block, lane = (x, y),
-// looping over the 8 z-slices; each slice is 64 consecutive voxels.
+// looping over the 8 z-slices; each slice is 64 consecutive voxels.  After
+// the update the wavefront re-derives the block's summary flags from the
+// final voxel values and stores them in the block's hash slot.
 template <class VT>
 __global__ void __launch_bounds__(256) k_integrate(MapView m, const PosePair* pose, Intr K, const float* depth,
                                                    size_t pitch, int W, int H) {
@@ -351,21 +358,35 @@ __global__ void __launch_bounds__(256) k_integrate(MapView m, const PosePair* po
     typename VT::T* bv = vox + (size_t)blk * 512;
     const float wx = (float)(bx * 8 + lx) * vs;
     const float wy = (float)(by * 8 + ly) * vs;
+    bool unobserved = true, free_space = true;
 #pragma unroll 2
     for (int lz = 0; lz < 8; ++lz) {
+      const int li = lz * 64 + lane;
+      typename VT::T cur = bv[li];
       const float wz = (float)(bz * 8 + lz) * vs;
       float px, py, pz;
       xform(Ti, wx, wy, wz, px, py, pz);
       int u = floor_sat(K.fx * px / pz + K.cx + 0.5f);
       int v = floor_sat(K.fy * py / pz + K.cy + 0.5f);
-      if (u < 0 || v < 0 || u > W - 1 || v > H - 1) continue;
-      float d = ((const float*)((const char*)depth + (size_t)v * pitch))[u];
-      if (d != d || d > m.depth_max || d < m.depth_min) continue;
-      float sdf = d - pz;
-      if (sdf < -mu) continue;
-      sdf = fminf(1.0f, sdf / mu);
-      const int li = lz * 64 + lane;
-      bv[li] = VT::update(bv[li], sdf, m.max_weight);
+      if (!(u < 0 || v < 0 || u > W - 1 || v > H - 1)) {
+        float d = ((const float*)((const char*)depth + (size_t)v * pitch))[u];
+        if (!(d != d || d > m.depth_max || d < m.depth_min)) {
+          float sdf = d - pz;
+          if (!(sdf < -mu)) {
+            sdf = fminf(1.0f, sdf / mu);
+            cur = VT::update(cur, sdf, m.max_weight);
+            bv[li] = cur;
+          }
+        }
+      }
+      const bool obs = VT::observed(cur);
+      unobserved = unobserved && !obs;
+      free_space = free_space && obs && VT::sdf(cur) == 1.0f;
+    }
+    const unsigned long long un = __ballot(unobserved), fr = __ballot(free_space);
+    if (lane == 0) {
+      const int f = (un == ~0ull ? kBlockUnobserved : 0) | (fr == ~0ull ? kBlockFree : 0);
+      m.slots[m.block_slot[blk]].flags = f;
     }
   }
 }
@@ -500,9 +521,9 @@ int yas_map_create(yas_ctx* ctx, const yas_map_params* P, const float K[4], yas_
   auto alloc = [&](void** p, size_t bytes) {
     if (e == hipSuccess) e = hipMalloc(p, bytes);
   };
-  alloc((void**)&v.keys, sizeof(unsigned long long) * P->num_buckets);
-  alloc((void**)&v.vals, sizeof(int) * P->num_buckets);
+  alloc((void**)&v.slots, sizeof(HashSlot) * P->num_buckets);
   alloc((void**)&v.block_keys, sizeof(unsigned long long) * P->num_blocks);
+  alloc((void**)&v.block_slot, sizeof(int) * P->num_blocks);
   alloc((void**)&v.voxels, vb * 512 * (size_t)P->num_blocks);
   alloc((void**)&v.visible, sizeof(int) * P->num_blocks);
   alloc((void**)&v.ctr, sizeof(int) * kCtrCount);
@@ -525,7 +546,7 @@ int yas_map_create(yas_ctx* ctx, const yas_map_params* P, const float K[4], yas_
 int yas_map_reset(yas_map* m) {
   YAS_REQUIRE(m, "null map");
   hipStream_t s = m->ctx->stream;
-  hipLaunchKernelGGL(k_reset_hash, dim3(grid_for(m->P.num_buckets, 256, 8192)), dim3(256), 0, s, m->v.keys, m->v.vals,
+  hipLaunchKernelGGL(k_reset_hash, dim3(grid_for(m->P.num_buckets, 256, 8192)), dim3(256), 0, s, m->v.slots,
                      m->P.num_buckets);
   YAS_LAUNCH_CHECK();
   size_t words = (size_t)voxel_bytes(m->P.voxel_format) * 512 * (size_t)m->P.num_blocks / 4;
@@ -540,9 +561,9 @@ int yas_map_destroy(yas_map* m) {
   if (!m) return YAS_OK;
   hipSetDevice(m->ctx->device);
   hipStreamSynchronize(m->ctx->stream);
-  hipFree(m->v.keys);
-  hipFree(m->v.vals);
+  hipFree(m->v.slots);
   hipFree(m->v.block_keys);
+  hipFree(m->v.block_slot);
   hipFree(m->v.voxels);
   hipFree(m->v.visible);
   hipFree(m->v.ctr);

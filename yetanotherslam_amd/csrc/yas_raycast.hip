@@ -94,17 +94,25 @@ template <class VT>
 struct VoxelReader {
   const MapView& m;
   const typename VT::T* vox;
-  int cbx, cby, cbz, cblk;
-  __device__ VoxelReader(const MapView& mv) : m(mv), vox((const typename VT::T*)mv.voxels), cbx(INT32_MIN), cby(0), cbz(0), cblk(-1) {}
+  int cbx, cby, cbz, cblk, cflags;
+  __device__ VoxelReader(const MapView& mv)
+      : m(mv), vox((const typename VT::T*)mv.voxels), cbx(INT32_MIN), cby(0), cbz(0), cblk(-1), cflags(0) {}
   // RayTraceEngine.cu:279-292: nearest read at floor(p), valid iff observed.
+  // Blocks summarised as all-unobserved or all-free answer without a voxel
+  // load (the values they would return are exactly (1, invalid) / (1, valid)).
   __device__ inline float read(float x, float y, float z, bool& valid) {
     int vx = floor_sat(x), vy = floor_sat(y), vz = floor_sat(z);
     int bx = vox_to_block(vx), by = vox_to_block(vy), bz = vox_to_block(vz);
     if (bx != cbx || by != cby || bz != cbz) {
       cbx = bx; cby = by; cbz = bz;
-      cblk = find_block_xyz(m, bx, by, bz);
+      cflags = 0;
+      cblk = find_block_xyz(m, bx, by, bz, &cflags);
     }
-    if (cblk >= 0) {
+    if (cblk >= 0 && cflags == kBlockFree) {
+      valid = true;
+      return 1.0f;
+    }
+    if (cblk >= 0 && cflags != kBlockUnobserved) {
       typename VT::T v = vox[(size_t)cblk * 512 + (vox_to_local(vz) * 8 + vox_to_local(vy)) * 8 + vox_to_local(vx)];
       if (VT::observed(v)) {
         valid = true;
@@ -143,8 +151,10 @@ template <class VT>
 __global__ void __launch_bounds__(256) k_march(MapView m, const PosePair* pose, Intr K, int W, int H, int cols,
                                                const float* zmin, const float* zmax, float4* vmap, float vsinv,
                                                float step_scale) {
-  const int x = blockIdx.x * 16 + (threadIdx.x & 15);
-  const int y = blockIdx.y * 16 + (threadIdx.x >> 4);
+  const int tiles_x = (W + 15) >> 4;
+  const int tile = xcd_tile(blockIdx.x, tiles_x * ((H + 15) >> 4));
+  const int x = (tile % tiles_x) * 16 + (threadIdx.x & 15);
+  const int y = (tile / tiles_x) * 16 + (threadIdx.x >> 4);
   if (x >= W || y >= H) return;
   float4 out = make_float4(__int_as_float(0x7fffffff), 0.0f, 0.0f, -1.0f);
   const float zn = zmin[(y >> 3) * cols + (x >> 3)];
@@ -215,7 +225,7 @@ static int launch_raycast(yas_raycaster* rc, yas_map* map, hipEvent_t before_mar
   if (before_march) YAS_HIP(hipEventRecord(before_march, s));
   const float vsinv = (float)(1.0 / (double)v.voxel_size);
   const float step_scale = v.trunc / v.voxel_size;
-  dim3 grid((rc->W + 15) / 16, (rc->H + 15) / 16);
+  dim3 grid(((rc->W + 15) / 16) * ((rc->H + 15) / 16));
   switch (v.format) {
     case YAS_VOXEL_I16_U8:
       hipLaunchKernelGGL(k_march<VoxI16>, grid, dim3(256), 0, s, v, pose, rc->K, rc->W, rc->H, rc->cols,
