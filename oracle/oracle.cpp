@@ -283,6 +283,85 @@ template <> struct Vox<VF64> {
   }
 };
 
+// ---------------------------------------------------------------- marching cubes
+static const uint64_t* decoded_tri_table();
+
+// MeshEngine.cu:114-123 - the edge crossing parameter (thresholds compared
+// in double, as the reference's 1e-6 literal is a double).
+template <class R> inline R mc_edge_t(R v1, R v2) {
+  if (std::fabs((double)(0 - v1)) < 1e-6) return 0;
+  if (std::fabs((double)(0 - v2)) < 1e-6) return 1;
+  if (std::fabs((double)(v1 - v2)) < 1e-6) return 0;
+  return (0 - v1) / (v2 - v1);
+}
+
+// MeshEngine.cu:125-269 for one cube whose corner 0 is the voxel p (integer
+// voxel coordinates) with corner values s in Bourke order
+// (0,0,0),(1,0,0),(1,1,0),(0,1,0),(0,0,1),(1,0,1),(1,1,1),(0,1,1).
+// Returns the cube index; writes its triangles (9 values each, metres) and
+// face normals normalize((b-a) x (c-a)), and the triangle count.
+template <class R> inline int mc_cube(const int p[3], const R s[8], R vs, R* tris, R* normals, int* ntri) {
+  *ntri = 0;
+  int cube = 0;
+  for (int c = 0; c < 8; ++c)
+    if (s[c] < 0) cube |= 1 << c;
+  if (cube <= 0) return cube;
+  const uint64_t word = decoded_tri_table()[cube];
+  const int nt = (int)(word & 0xF);
+  if (nt == 0) return cube;
+  const R pf[3] = {(R)p[0], (R)p[1], (R)p[2]};
+  static const int ea[12] = {0, 1, 2, 3, 4, 5, 6, 7, 0, 1, 2, 3};
+  static const int eb[12] = {1, 2, 3, 0, 5, 6, 7, 4, 4, 5, 6, 7};
+  R vert[12][3];
+  uint32_t used = 0;
+  for (int i = 0; i < 3 * nt; ++i) used |= 1u << ((word >> (4 + 4 * i)) & 0xF);
+  for (int e = 0; e < 12; ++e) {
+    if (!(used & (1u << e))) continue;
+    R t = mc_edge_t(s[ea[e]], s[eb[e]]);
+    R o[3];
+    switch (e) {
+      case 0: o[0] = t; o[1] = 0; o[2] = 0; break;
+      case 1: o[0] = 1; o[1] = t; o[2] = 0; break;
+      case 2: o[0] = 1 - t; o[1] = 1; o[2] = 0; break;
+      case 3: o[0] = 0; o[1] = 1 - t; o[2] = 0; break;
+      case 4: o[0] = t; o[1] = 0; o[2] = 1; break;
+      case 5: o[0] = 1; o[1] = t; o[2] = 1; break;
+      case 6: o[0] = 1 - t; o[1] = 1; o[2] = 1; break;
+      case 7: o[0] = 0; o[1] = 1 - t; o[2] = 1; break;
+      case 8: o[0] = 0; o[1] = 0; o[2] = t; break;
+      case 9: o[0] = 1; o[1] = 0; o[2] = t; break;
+      case 10: o[0] = 1; o[1] = 1; o[2] = t; break;
+      default: o[0] = 0; o[1] = 1; o[2] = t; break;
+    }
+    for (int k = 0; k < 3; ++k) vert[e][k] = pf[k] + o[k];
+  }
+  for (int i = 0; i < nt; ++i) {
+    R a[3], b[3], c[3];
+    int e0 = (word >> (4 + 12 * i)) & 0xF, e1 = (word >> (8 + 12 * i)) & 0xF, e2 = (word >> (12 + 12 * i)) & 0xF;
+    for (int k = 0; k < 3; ++k) {
+      a[k] = vert[e0][k] * vs;
+      b[k] = vert[e1][k] * vs;
+      c[k] = vert[e2][k] * vs;
+    }
+    R u[3] = {b[0] - a[0], b[1] - a[1], b[2] - a[2]};
+    R w[3] = {c[0] - a[0], c[1] - a[1], c[2] - a[2]};
+    R n[3] = {u[1] * w[2] - u[2] * w[1], u[2] * w[0] - u[0] * w[2], u[0] * w[1] - u[1] * w[0]};
+    R n2 = n[0] * n[0] + n[1] * n[1] + n[2] * n[2];
+    if (n2 > 0) {
+      R nn = std::sqrt(n2);
+      for (int k = 0; k < 3; ++k) n[k] = n[k] / nn;
+    }
+    for (int k = 0; k < 3; ++k) {
+      tris[9 * i + k] = a[k];
+      tris[9 * i + 3 + k] = b[k];
+      tris[9 * i + 6 + k] = c[k];
+      normals[9 * i + k] = normals[9 * i + 3 + k] = normals[9 * i + 6 + k] = n[k];
+    }
+  }
+  *ntri = nt;
+  return cube;
+}
+
 // ---------------------------------------------------------------- map
 struct MapBase {
   virtual ~MapBase() {}
@@ -647,20 +726,10 @@ template <class V> struct Map : MapBase {
   }
 
   // ------------------------------------------------------------ mesh
-  // MeshEngine.cu:114-123 - the edge crossing parameter (thresholds compared
-  // in double, as the reference's 1e-6 literal is a double).
-  static R edge_t(R v1, R v2) {
-    if (std::fabs((double)(0 - v1)) < 1e-6) return 0;
-    if (std::fabs((double)(0 - v2)) < 1e-6) return 1;
-    if (std::fabs((double)(v1 - v2)) < 1e-6) return 0;
-    return (0 - v1) / (v2 - v1);
-  }
-
   // MeshEngine.cu:43-269 - per allocated block, per voxel: the 8 corners in
-  // Bourke order must all be observed (w > 0); cube index from sdf < 0;
-  // vertices on the crossed edges in voxel units, scaled by the voxel size;
-  // face normal normalize((b-a) x (c-a)).  Output order: blocks sorted by
-  // key, voxels by local index z*64+y*8+x, triangles in table order.
+  // Bourke order must all be observed (w > 0); then mc_cube.  Output order:
+  // blocks sorted by key, voxels by local index z*64+y*8+x, triangles in
+  // table order.
   long mesh(int32_t* bkeys, int32_t* tcounts, int16_t* cases, double* tris, double* normals, long tri_cap,
             int block_cap) override {
     static const int off[8][3] = {{0, 0, 0}, {1, 0, 0}, {1, 1, 0}, {0, 1, 0},
@@ -671,9 +740,9 @@ template <class V> struct Map : MapBase {
       keys.push_back({b.x, b.y, b.z});
     }
     std::sort(keys.begin(), keys.end());
-    const uint64_t* tt = mc_table();
     long ntri = 0;
     int bi = 0;
+    R t9[5 * 9], n9[5 * 9];
     for (const auto& bk : keys) {
       int cnt = 0;
       for (int l = 0; l < 512; ++l) {
@@ -686,72 +755,17 @@ template <class V> struct Map : MapBase {
           if (!v || Vox<V>::wt(*v) == 0) ok = false;
           else s[c] = Vox<V>::sdf(*v);
         }
-        int cube = -1;
-        if (ok) {
-          cube = 0;
-          for (int c = 0; c < 8; ++c)
-            if (s[c] < 0) cube |= 1 << c;
-        }
+        int cube = -1, nt = 0;
+        if (ok) cube = mc_cube<R>(p, s, vs, t9, n9, &nt);
         if (cases && bi < block_cap) cases[(size_t)bi * 512 + l] = (int16_t)cube;
-        if (cube <= 0) continue;
-        uint64_t word = tt[cube];
-        int nt = (int)(word & 0xF);
-        if (nt == 0) continue;
-        R pf[3] = {(R)p[0], (R)p[1], (R)p[2]};
-        R vert[12][3];
-        // Edge e joins corners (a,b); the vertex lies at corner a plus t
-        // times the unit step along the edge's axis (MeshEngine.cu:150-224).
-        static const int ea[12] = {0, 1, 2, 3, 4, 5, 6, 7, 0, 1, 2, 3};
-        static const int eb[12] = {1, 2, 3, 0, 5, 6, 7, 4, 4, 5, 6, 7};
-        uint32_t used = 0;
-        for (int i = 0; i < 3 * nt; ++i) used |= 1u << ((word >> (4 + 4 * i)) & 0xF);
-        for (int e = 0; e < 12; ++e) {
-          if (!(used & (1u << e))) continue;
-          R t = edge_t(s[ea[e]], s[eb[e]]);
-          R o[3];
-          switch (e) {
-            case 0: o[0] = t; o[1] = 0; o[2] = 0; break;
-            case 1: o[0] = 1; o[1] = t; o[2] = 0; break;
-            case 2: o[0] = 1 - t; o[1] = 1; o[2] = 0; break;
-            case 3: o[0] = 0; o[1] = 1 - t; o[2] = 0; break;
-            case 4: o[0] = t; o[1] = 0; o[2] = 1; break;
-            case 5: o[0] = 1; o[1] = t; o[2] = 1; break;
-            case 6: o[0] = 1 - t; o[1] = 1; o[2] = 1; break;
-            case 7: o[0] = 0; o[1] = 1 - t; o[2] = 1; break;
-            case 8: o[0] = 0; o[1] = 0; o[2] = t; break;
-            case 9: o[0] = 1; o[1] = 0; o[2] = t; break;
-            case 10: o[0] = 1; o[1] = 1; o[2] = t; break;
-            default: o[0] = 0; o[1] = 1; o[2] = t; break;
-          }
-          for (int k = 0; k < 3; ++k) vert[e][k] = pf[k] + o[k];
-        }
-        for (int i = 0; i < nt; ++i) {
-          R a[3], b[3], c[3];
-          int e0 = (word >> (4 + 12 * i)) & 0xF, e1 = (word >> (8 + 12 * i)) & 0xF, e2 = (word >> (12 + 12 * i)) & 0xF;
-          for (int k = 0; k < 3; ++k) {
-            a[k] = vert[e0][k] * vs;
-            b[k] = vert[e1][k] * vs;
-            c[k] = vert[e2][k] * vs;
-          }
-          R u[3] = {b[0] - a[0], b[1] - a[1], b[2] - a[2]};
-          R w[3] = {c[0] - a[0], c[1] - a[1], c[2] - a[2]};
-          R n[3] = {u[1] * w[2] - u[2] * w[1], u[2] * w[0] - u[0] * w[2], u[0] * w[1] - u[1] * w[0]};
-          R n2 = n[0] * n[0] + n[1] * n[1] + n[2] * n[2];
-          if (n2 > 0) {
-            R nn = std::sqrt(n2);
-            for (int k = 0; k < 3; ++k) n[k] = n[k] / nn;
-          }
-          if (ntri < tri_cap) {
-            for (int k = 0; k < 3; ++k) {
-              tris[ntri * 9 + k] = a[k];
-              tris[ntri * 9 + 3 + k] = b[k];
-              tris[ntri * 9 + 6 + k] = c[k];
-              if (normals) normals[ntri * 9 + k] = normals[ntri * 9 + 3 + k] = normals[ntri * 9 + 6 + k] = n[k];
+        for (int i = 0; i < nt; ++i, ++ntri) {
+          if (ntri < tri_cap)
+            for (int k = 0; k < 9; ++k) {
+              tris[ntri * 9 + k] = (double)t9[9 * i + k];
+              if (normals) normals[ntri * 9 + k] = (double)n9[9 * i + k];
             }
-          }
-          ++ntri;
-          ++cnt;
         }
+        cnt += nt;
       }
       if (bi < block_cap) {
         if (bkeys) { bkeys[3 * bi] = bk[0]; bkeys[3 * bi + 1] = bk[1]; bkeys[3 * bi + 2] = bk[2]; }
@@ -761,9 +775,8 @@ template <class V> struct Map : MapBase {
     }
     return ntri;
   }
-
-  static const uint64_t* mc_table();
 };
+
 
 // The triangulation table, decoded once from its hex-string form into
 // (count | edges << 4) words.
@@ -772,7 +785,7 @@ static const char* kTriHex[256] = {
 };
 static uint64_t g_tri_words[256];
 static bool g_tri_init = false;
-static const uint64_t* decoded_tri_table() {
+const uint64_t* decoded_tri_table() {
   if (!g_tri_init) {
     for (int i = 0; i < 256; ++i) {
       const char* s = kTriHex[i];
@@ -788,7 +801,6 @@ static const uint64_t* decoded_tri_table() {
   }
   return g_tri_words;
 }
-template <class V> const uint64_t* Map<V>::mc_table() { return decoded_tri_table(); }
 
 // ---------------------------------------------------------------- images
 template <class R> struct Img {
@@ -1423,6 +1435,16 @@ int16_t orc_pack_sdf(float f) { return Vox<VRef>::pack(f); }
 uint16_t orc_f32_to_f16(float f) { return f32_to_f16(f); }
 float orc_f16_to_f32(uint16_t h) { return f16_to_f32(h); }
 const uint64_t* orc_mc_table(void) { return decoded_tri_table(); }
+// One marching cube (precision 32 or 64): returns the cube index, writes up
+// to 5 triangles (45 values) and normals, *ntri their count.
+int orc_mc_cube(int precision, const int p[3], const double s[8], double vs, double* tris, double* normals, int* ntri) {
+  if (precision == 64) return mc_cube<double>(p, s, vs, tris, normals, ntri);
+  float sf[8], tf[45], nf[45];
+  for (int i = 0; i < 8; ++i) sf[i] = (float)s[i];
+  int c = mc_cube<float>(p, sf, (float)vs, tf, nf, ntri);
+  for (int i = 0; i < 9 * *ntri; ++i) { tris[i] = tf[i]; normals[i] = nf[i]; }
+  return c;
+}
 
 void orc_se3_exp(const double xi[6], double T[16]) { se3_exp(xi, T); }
 void orc_ldlt_solve6(const double A[36], const double b[6], double x[6]) { ldlt_solve6(A, b, x); }

@@ -44,6 +44,7 @@ def lib():
         "orc_f32_to_f16": (ctypes.c_uint16, [ctypes.c_float]),
         "orc_f16_to_f32": (ctypes.c_float, [ctypes.c_uint16]),
         "orc_mc_table": (ctypes.POINTER(ctypes.c_uint64), []),
+        "orc_mc_cube": (ctypes.c_int, [ctypes.c_int, P, P, ctypes.c_double, P, P, ctypes.POINTER(ctypes.c_int)]),
         "orc_se3_exp": (None, [P, P]),
         "orc_ldlt_solve6": (None, [P, P, P]),
         "orc_image_op": (None, [ctypes.c_int, ctypes.c_int, P, ctypes.c_int, ctypes.c_int, P, P]),
@@ -195,6 +196,17 @@ def ldlt_solve6(A, b):
 def mc_table():
     t = lib().orc_mc_table()
     return np.array([t[i] for i in range(256)], dtype=np.uint64)
+
+
+def mc_cube(p, s, vs=1.0, precision=64):
+    """One marching cube: (cube index, triangles (n, 9), normals (n, 9))."""
+    pp = np.ascontiguousarray(p, np.int32)
+    ss = _f64(s)
+    tris = np.zeros(45, np.float64)
+    nrm = np.zeros(45, np.float64)
+    n = ctypes.c_int()
+    c = lib().orc_mc_cube(precision, _p(pp), _p(ss), vs, _p(tris), _p(nrm), ctypes.byref(n))
+    return c, tris[:9 * n.value].reshape(-1, 9), nrm[:9 * n.value].reshape(-1, 9)
 
 
 def depth_u16_to_f32(d, scale=1000.0):
