@@ -69,6 +69,48 @@ def cpu_baseline(p, K, depth, rgb, budget_s=20.0, max_frames=40):
     return n / dt, n, dt
 
 
+def rooflines(stats, stage_ms, steps):
+    """Achieved algorithmic HBM bandwidth of the single-kernel stages (bytes
+    per launch as defined for each kernel below, divided by the kernel's
+    average duration measured with HIP events around its launches)."""
+    sv, se, sd = 8, 16, 4            # voxel bytes (f32 sdf + f32 weight), hash slot bytes, depth bytes
+    nvis_fuse = stats[:, 1].astype(np.float64)
+    nvis_ray = stats[:, 4].astype(np.float64)
+    cells = (W // 8) * (H // 8)
+    defs = {
+        # every voxel of every visible block read and written, the depth
+        # image, and the block's list entry + key + slot index
+        "k_integrate": (7, float(np.sum(nvis_fuse * (512 * 2 * sv + 4 + 8 + 4) + W * H * sd))),
+        # compulsory ray-cast traffic: each visible block's voxels and hash
+        # slot once, the vertex map written, the two z-range maps read
+        "k_march": (2, float(np.sum(nvis_ray * (512 * sv + se) + W * H * 16 + 2 * cells * 4))),
+        # the depth image read and one hash-slot probe per Bresenham visit
+        "k_alloc": (5, float(steps * W * H * sd + (stats[-1, 3] - stats[0, 3]) * se)),
+    }
+    out = {}
+    for name, (i, nbytes) in defs.items():
+        t = stage_ms[i] / 1e3
+        ach = nbytes / t / 1e9 if t > 0 else 0.0
+        out[name] = {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
+                     "algorithmic_bytes_per_launch": round(nbytes / steps),
+                     "avg_launch_ms": round(stage_ms[i] / steps, 5)}
+    return out
+
+
+def copy_probe():
+    """A plain device copy of known size (512 MiB read + 512 MiB written),
+    used to calibrate rocprofv3's FETCH_SIZE / WRITE_SIZE counters."""
+    import torch
+    n = 128 << 20
+    a = torch.ones(n, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(5):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_bytes_each_way": n * 4, "launches": 5}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -76,7 +118,11 @@ def main():
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
+    ap.add_argument("--copy-probe", action="store_true", help="run only the counter-calibration copy")
     args = ap.parse_args()
+    if args.copy_probe:
+        copy_probe()
+        return
     if args.gpus != 1:
         print("bench.py measures one GPU: the dense path is a per-stream, per-volume workload "
               "(one process per GPU); --gpus %d is not supported" % args.gpus, file=sys.stderr)
@@ -120,23 +166,17 @@ def main():
     frame_ms = np.zeros(args.steps)
     nt = rf.ctypes.c_int()
     lib.call("yas_pipeline_timing", pl.h, stage.ctypes.data, frame_ms.ctypes.data, args.steps, rf.ctypes.byref(nt))
-    stats = np.zeros((args.steps, 4), np.int32)
+    stats = np.zeros((args.steps, 8), np.int32)
     lib.call("yas_pipeline_frame_stats", pl.h, args.warmup + 1, args.steps, stats.ctypes.data)
     tracked = sum(pl.pose(t)[1] for t in range(args.warmup + 1, n))
 
     fps = args.steps / wall
     per_stage = {k: round(v / args.steps, 4) for k, v in zip(STAGES, stage)}
-    # roofline of TSDF integration: every voxel of every visible block is read
-    # and written (8 B each), plus the depth image (4 B/pixel) and 12 B of
-    # block list per block (visible index + block key).
-    n_vis = stats[:, 1].astype(np.float64)
-    bytes_int = float(np.sum(n_vis * (512 * 2 * 8 + 12) + W * H * 4))
-    t_int = stage[7] / 1e3
-    ach = bytes_int / t_int / 1e9 if t_int > 0 else 0.0
-    dominant = max(range(8), key=lambda i: stage[i])
+    rl = rooflines(stats, stage, args.steps)
+    dominant = max(rl, key=lambda k: rl[k]["avg_launch_ms"])
     log("stages ms/frame:", per_stage)
-    log("visible blocks/frame: mean %.0f; allocated at end %d; tracked %d/%d" % (
-        n_vis.mean(), stats[-1, 0], tracked, args.steps))
+    log("visible blocks/frame: fuse %.0f, ray cast %.0f; allocated at end %d; tracked %d/%d" % (
+        stats[:, 1].mean(), stats[:, 4].mean(), stats[-1, 0], tracked, args.steps))
 
     result = {
         "metric": "frames/s (integrate+raycast+ICP) 640x480@5mm; p50/p99 frame latency",
@@ -161,12 +201,9 @@ def main():
                        "p99": round(float(np.percentile(frame_ms, 99)), 4),
                        "mean_device": round(float(frame_ms.mean()), 4)},
         "stage_ms_per_frame": per_stage,
-        "dominant_stage": STAGES[dominant],
         "tracked_frames": int(tracked),
-        "roofline": {"kernel": "k_integrate", "bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
-                     "algorithmic_bytes_per_launch": round(bytes_int / args.steps),
-                     "avg_launch_ms": round(stage[7] / args.steps, 5)},
+        "roofline": dict(rl[dominant], kernel=dominant),
+        "roofline_by_kernel": rl,
     }
     if not args.no_cpu_baseline:
         log("cpu baseline (oracle, 1 thread, <= %.0f s)" % args.cpu_budget)

@@ -264,8 +264,9 @@ int yas_pipeline_num_frames(yas_pipeline* pl, int* n);
 int yas_pipeline_enable_timing(yas_pipeline* pl, int enable);
 int yas_pipeline_timing(yas_pipeline* pl, double stage_ms[8], double* frame_ms, int capacity, int* num_frames);
 int yas_pipeline_reset_timing(yas_pipeline* pl);
-/* Counters of frames [first, first+count): per frame {allocated blocks,
- * visible blocks, capacity flags, Bresenham walk steps (cumulative)}. Synchronous. */
+/* Counters of frames [first, first+count): 8 ints per frame {allocated
+ * blocks, visible blocks at fusion, capacity flags, Bresenham walk steps
+ * (cumulative), visible blocks at the ray cast, 0, 0, 0}. Synchronous. */
 int yas_pipeline_frame_stats(yas_pipeline* pl, int first, int count, int* out);
 int yas_pipeline_map(yas_pipeline* pl, yas_map** map);
 int yas_pipeline_raycaster(yas_pipeline* pl, yas_raycaster** rc);

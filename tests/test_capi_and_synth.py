@@ -98,3 +98,17 @@ def test_depth_noise_and_dropout_statistics():
     z = d[ok] / 1000.0
     resid = (z - clean[ok]) / (0.0012 * clean[ok] ** 2)
     assert abs(resid.mean()) < 0.1 and 0.8 < resid.std() < 1.3  # N(0, (k z^2)^2) plus mm rounding
+
+
+def test_cpp_facade_compiles_and_links(tmp_path):
+    """The reference-shaped C++ facade (include/yas_rgbdfusion.hpp) compiles
+    against yas.h and links against libyas.so; the GPU test runs it."""
+    import shutil
+    import subprocess
+    hipcc = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
+    out = tmp_path / "facade_replay"
+    r = subprocess.run([hipcc, "--offload-arch=gfx950", "-O1", "-std=c++17", os.path.join(REPO, "tests", "cpp",
+                        "facade_replay.cpp"), "-L" + os.path.dirname(native.LIB_PATH), "-lyas",
+                        "-Wl,-rpath," + os.path.dirname(native.LIB_PATH), "-o", str(out)],
+                       capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr[-2000:]

@@ -246,3 +246,15 @@ def test_empty_depth_frame_allocates_nothing_and_tracks_identity():
     T, st = trk.GetTransform()
     assert st.good
     np.testing.assert_array_equal(T, np.eye(4))
+
+
+def test_cpp_facade_replays_track_rgbd():
+    """The reference-shaped C++ facade, driven like Tracking::TrackRGBD on a
+    plane: tracks identity, fuses, meshes (built by __graft_entry__.build())."""
+    import os
+    import subprocess
+    exe = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "tests", "cpp", "facade_replay")
+    assert os.path.exists(exe), "run __graft_entry__.build() first"
+    r = subprocess.run([exe], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "tracked=1" in r.stdout

@@ -86,12 +86,24 @@ def build_oracle(verbose=False):
     return out
 
 
+def build_facade_replay(verbose=False):
+    """tests/cpp/facade_replay: the C++ facade driven like FullSystem's Tracking."""
+    out = os.path.join(REPO, "tests", "cpp", "facade_replay")
+    _run([_hipcc(), "--offload-arch=gfx950", "-O1", "-std=c++17", os.path.join(REPO, "tests", "cpp", "facade_replay.cpp"),
+          "-L" + HERE, "-lyas", "-Wl,-rpath," + HERE, "-o", out])
+    if verbose:
+        print("built", out)
+    return out
+
+
 def build_all(oracle=True, verbose=False):
     with ThreadPoolExecutor(max_workers=3) as ex:
         futs = [ex.submit(build_hip, verbose), ex.submit(build_synth, verbose)]
         if oracle:
             futs.append(ex.submit(build_oracle, verbose))
-        return [f.result() for f in futs]
+        outs = [f.result() for f in futs]
+    outs.append(build_facade_replay(verbose))
+    return outs
 
 
 if __name__ == "__main__":

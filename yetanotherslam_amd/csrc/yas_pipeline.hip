@@ -35,8 +35,9 @@ namespace {
 // T_last <- T_last * DT^-1.  Records the frame's pose and outcome and sets
 // the gate that lets the fusion kernels run.
 __global__ void k_post_track(const double* DT, const int* good, double* T_last, double* pose_out, int* tracked,
-                             int* gate) {
+                             int* gate, const int* ray_visible, int* ray_visible_out) {
   if (threadIdx.x != 0) return;
+  *ray_visible_out = *ray_visible;
   double tn = sqrt(DT[3] * DT[3] + DT[7] * DT[7] + DT[11] * DT[11]);
   int ok = (*good != 0) && !(tn > 0.1);
   if (ok) {
@@ -47,6 +48,11 @@ __global__ void k_post_track(const double* DT, const int* good, double* T_last, 
   for (int i = 0; i < 16; ++i) pose_out[i] = T_last[i];
   *tracked = ok;
   *gate = ok;
+}
+
+// Per-frame counters of the map: allocated, visible, flags, walk steps.
+__global__ void k_frame_stats(const int* ctr, int* out) {
+  if (threadIdx.x < 4) out[threadIdx.x] = ctr[threadIdx.x];
 }
 
 __global__ void k_record_first(const double* T_last, double* pose_out, int* tracked) {
@@ -71,7 +77,7 @@ struct yas_pipeline {
   std::vector<std::vector<hipEvent_t>> spare;
   double stage_ms[kStages];
   std::vector<double> frame_ms;
-  int* frame_stats;  // [cap][4]: allocated, visible, flags, walk steps
+  int* frame_stats;  // [cap][8]: allocated, visible (fuse), flags, walk steps, visible (ray cast)
   int W, H;
   float depth_scale;
   int gray_mode;
@@ -102,10 +108,10 @@ static int grow(yas_pipeline* p) {
     hipFree(p->tracked);
   }
   int* ns = nullptr;
-  YAS_HIP(hipMalloc((void**)&ns, sizeof(int) * 4 * (size_t)ncap));
-  YAS_HIP(hipMemsetAsync(ns, 0, sizeof(int) * 4 * (size_t)ncap, p->ctx->stream));
+  YAS_HIP(hipMalloc((void**)&ns, sizeof(int) * 8 * (size_t)ncap));
+  YAS_HIP(hipMemsetAsync(ns, 0, sizeof(int) * 8 * (size_t)ncap, p->ctx->stream));
   if (p->cap) {
-    YAS_HIP(hipMemcpyAsync(ns, p->frame_stats, sizeof(int) * 4 * p->cap, hipMemcpyDeviceToDevice, p->ctx->stream));
+    YAS_HIP(hipMemcpyAsync(ns, p->frame_stats, sizeof(int) * 8 * p->cap, hipMemcpyDeviceToDevice, p->ctx->stream));
     YAS_HIP(hipStreamSynchronize(p->ctx->stream));
     hipFree(p->frame_stats);
   }
@@ -238,7 +244,8 @@ int yas_pipeline_step(yas_pipeline* p, const uint16_t* depth, size_t dpitch, con
     if ((rc = yas_tracker_solve_async(p->trk))) return rc;
     yas_tracker_swap_frame_buffer(p->trk);
     hipLaunchKernelGGL(k_post_track, dim3(1), dim3(64), 0, s, yas_tracker_result_dev(p->trk), yas_tracker_good_dev(p->trk),
-                       p->T_last, pose_out, tracked_out, p->gate);
+                       p->T_last, pose_out, tracked_out, p->gate, (const int*)yas_map_view(p->map).ctr + kCtrVisible,
+                       p->frame_stats + 8 * (size_t)p->frames + 4);
     YAS_LAUNCH_CHECK();
     if ((rc = yas_map_prepare_pose_dev(p->map, p->T_last))) return rc;
     gate = p->gate;
@@ -250,9 +257,9 @@ int yas_pipeline_step(yas_pipeline* p, const uint16_t* depth, size_t dpitch, con
   if ((rc = mark(7))) return rc;
   if ((rc = yas_map_integrate_stage(p->map, p->depth, fp, W, H))) return rc;
   if ((rc = mark(8))) return rc;
-  // per-frame counters: allocated, visible, flags, walk steps
-  MapView v = yas_map_view(p->map);
-  YAS_HIP(hipMemcpyAsync(p->frame_stats + 4 * (size_t)p->frames, v.ctr, sizeof(int) * 4, hipMemcpyDeviceToDevice, s));
+  hipLaunchKernelGGL(k_frame_stats, dim3(1), dim3(64), 0, s, (const int*)yas_map_view(p->map).ctr,
+                     p->frame_stats + 8 * (size_t)p->frames);
+  YAS_LAUNCH_CHECK();
   p->frames++;
   return YAS_OK;
 }
@@ -312,7 +319,7 @@ int yas_pipeline_frame_stats(yas_pipeline* p, int first, int count, int* out) {
   YAS_REQUIRE(p && out, "null argument");
   YAS_REQUIRE(first >= 0 && count >= 0 && first + count <= p->frames, "yas_pipeline_frame_stats: bad frame range");
   YAS_HIP(hipStreamSynchronize(p->ctx->stream));
-  if (count) YAS_HIP(hipMemcpy(out, p->frame_stats + 4 * (size_t)first, sizeof(int) * 4 * count, hipMemcpyDeviceToHost));
+  if (count) YAS_HIP(hipMemcpy(out, p->frame_stats + 8 * (size_t)first, sizeof(int) * 8 * count, hipMemcpyDeviceToHost));
   return YAS_OK;
 }
 
