@@ -10,7 +10,7 @@ the synthetic room sequence at 640x480, voxel 5 mm, truncation 4 voxels,
 pyramid levels with 10/5/3 Gauss-Newton iterations.  All frames are uploaded
 to HBM before the timed region.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline]
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline] [--no-mesh]
 Prints one JSON line on stdout; progress goes to stderr.
 """
 import argparse
@@ -69,6 +69,14 @@ def cpu_baseline(p, K, depth, rgb, budget_s=20.0, max_frames=40):
     return n / dt, n, dt
 
 
+def walk_steps(stats):
+    """Bresenham block visits over the timed frames, from the cumulative
+    uint32 counter (frame 0's visits are estimated by the mean of the rest)."""
+    c = stats[:, 3].astype(np.int64) & 0xFFFFFFFF
+    d = np.diff(c) % (1 << 32)
+    return float(d.sum() * len(c) / max(len(d), 1))
+
+
 def rooflines(stats, stage_ms, steps):
     """Achieved algorithmic HBM bandwidth of the single-kernel stages (bytes
     per launch as defined for each kernel below, divided by the kernel's
@@ -85,7 +93,8 @@ def rooflines(stats, stage_ms, steps):
         # slot once, the vertex map written, the two z-range maps read
         "k_march": (2, float(np.sum(nvis_ray * (512 * sv + se) + W * H * 16 + 2 * cells * 4))),
         # the depth image read and one hash-slot probe per Bresenham visit
-        "k_alloc": (5, float(steps * W * H * sd + (stats[-1, 3] - stats[0, 3]) * se)),
+        # (the walk counter is cumulative and wraps at 2^32)
+        "k_alloc": (5, float(steps * W * H * sd + walk_steps(stats) * se)),
     }
     out = {}
     for name, (i, nbytes) in defs.items():
@@ -95,6 +104,46 @@ def rooflines(stats, stage_ms, steps):
                      "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
                      "algorithmic_bytes_per_launch": round(nbytes / steps),
                      "avg_launch_ms": round(stage_ms[i] / steps, 5)}
+    return out
+
+
+def mesh_bench(pl, dev_depth, K, nframes, reps=3):
+    """Marching cubes over the fused 5 mm volume (float32 sdf, the timed
+    run's own map) and over the same frames fused at the tracked poses into a
+    float16 sdf + uint16 weight map: indexed output (float32 vertices, int32
+    triangles), and the reference's triangle soup beside it.  Wall time of
+    the synchronous extraction call (kernels + one 8-byte readback), median
+    of `reps` after one untimed call."""
+    import ctypes
+    import torch
+    from yetanotherslam_amd import rgbdfusion as rf
+
+    f16 = rf.MapStruct(np.asarray(K, np.float32), BUCKETS, BLOCKS, VOXEL, TRUNC, voxel_format=rf.VOXEL_F16_U16)
+    for t in range(nframes):
+        T, ok = pl.pose(t)
+        if ok:
+            f16.FuseU16(dev_depth[t], T)
+    torch.cuda.synchronize()
+    me = rf.MeshEngine(8_000_000)
+    nv, nt, nb = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+    out = {}
+    for name, h, sv in (("f32", pl.map_handle(), 8), ("f16u16", f16.h, 4)):
+        ms = {}
+        for kind in ("indexed", "soup"):
+            ts = []
+            for _ in range(reps + 1):
+                torch.cuda.synchronize()
+                t0 = time.perf_counter()
+                if kind == "indexed":
+                    rf.native.call("yas_mesh_extract_indexed", me.h, h, ctypes.byref(nv), ctypes.byref(nt))
+                else:
+                    rf.native.call("yas_mesh_extract", me.h, h, ctypes.byref(nt))
+                ts.append(time.perf_counter() - t0)
+            ms[kind] = float(np.median(ts[1:])) * 1e3
+        rf.native.call("yas_mesher_blocks", me.h, None, None, None, None, 0, ctypes.byref(nb))
+        out[name] = {"blocks": nb.value, "triangles": nt.value, "vertices": nv.value,
+                     "ms_indexed": round(ms["indexed"], 3), "ms_soup": round(ms["soup"], 3),
+                     "voxel_read_GBps_indexed": round(2 * nb.value * 512 * sv / (ms["indexed"] * 1e-3) / 1e9, 1)}
     return out
 
 
@@ -119,6 +168,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--copy-probe", action="store_true", help="run only the counter-calibration copy")
+    ap.add_argument("--no-mesh", action="store_true", help="skip the marching-cubes measurement")
     args = ap.parse_args()
     if args.copy_probe:
         copy_probe()
@@ -205,6 +255,10 @@ def main():
         "roofline": dict(rl[dominant], kernel=dominant),
         "roofline_by_kernel": rl,
     }
+    if not args.no_mesh:
+        log("marching cubes over the fused volume")
+        result["mesh"] = mesh_bench(pl, dev_depth, K, n)
+        log("mesh:", result["mesh"])
     if not args.no_cpu_baseline:
         log("cpu baseline (oracle, 1 thread, <= %.0f s)" % args.cpu_budget)
         v, nf, dt = cpu_baseline(p, K, depth, rgb, args.cpu_budget)

@@ -168,6 +168,25 @@ int yas_mesh_extract(yas_mesher* me, yas_map* map, int* num_triangles);
 /* Download the last extraction: 9 floats (3 vertices) per triangle for
  * vertices and for face normals (either may be NULL). Synchronous. */
 int yas_mesher_download(yas_mesher* me, float* vertices, float* normals, int capacity);
+/* Marching cubes as an indexed mesh (no counterpart in the reference, whose
+ * Meshify emits a soup): float32 xyz vertices and int32 triangle corners.
+ * Each vertex sits on one cube edge and is shared by every triangle that
+ * uses the edge; a vertex on an edge whose neighbouring cubes are all
+ * incomplete is emitted but unreferenced.  Triangles come in the same order
+ * as yas_mesh_extract's; their corners match its positions to a few ulps of
+ * the coordinate (the interpolation runs along +axis, not along the cube's
+ * edge direction).  Capacity: max_triangles
+ * triangles and 3 * max_triangles vertices; the counts are returned even when
+ * they exceed it (then YAS_E_CAPACITY).  The mesher holds one extraction at
+ * a time.  Synchronous. */
+int yas_mesh_extract_indexed(yas_mesher* me, yas_map* map, int* num_vertices, int* num_triangles);
+/* Download the last indexed extraction (3 floats per vertex, 3 int32 per
+ * triangle); either pointer may be NULL. Synchronous. */
+int yas_mesher_download_indexed(yas_mesher* me, float* vertices, int32_t* triangles, int vertex_capacity,
+                                int triangle_capacity);
+/* Device pointers of the last indexed extraction (valid until the next
+ * extraction or yas_mesher_destroy). */
+int yas_mesher_indexed_device(yas_mesher* me, const float** vertices, const int32_t** triangles);
 /* Per-block detail of the last extraction, in pool order: block keys
  * (int32 triples), first triangle index, triangle count and, when `cases`
  * is not NULL, the cube index of every voxel (-1 where a corner is

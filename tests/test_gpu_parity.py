@@ -195,6 +195,27 @@ def test_mesh_cases_and_triangles_bit_exact(fmt):
     assert_bits_equal(mo["normals"], normals, "normals")
 
 
+@pytest.mark.parametrize("fmt", FMTS, ids=["i16u8", "f32", "f16u16"])
+def test_indexed_mesh_matches_the_soup_and_shares_vertices(fmt):
+    """The indexed mesh expands to the reference-form soup (same triangle
+    order, corners within 1e-5 m); every triangle edge is used by at most two
+    triangles, and most by exactly two (shared vertices, no duplicates)."""
+    _, _, _, _, _, gm = fuse_both(fmt)
+    me = rf.MeshEngine(2_000_000)
+    soup = me.Meshify(gm)
+    ix = me.MeshifyIndexed(gm)
+    V, T = ix["vertices"], ix["triangles"]
+    assert ix["num_triangles"] == soup["num_triangles"] == len(T) > 1000
+    assert T.min() >= 0 and T.max() < len(V)
+    assert np.all((T[:, 0] != T[:, 1]) & (T[:, 1] != T[:, 2]) & (T[:, 0] != T[:, 2]))
+    np.testing.assert_allclose(V[T].reshape(-1, 9), soup["tris"], rtol=0, atol=1e-5)
+    e = np.sort(np.concatenate([T[:, [0, 1]], T[:, [1, 2]], T[:, [2, 0]]]), axis=1).astype(np.int64)
+    _, uses = np.unique(e[:, 0] * len(V) + e[:, 1], return_counts=True)
+    assert uses.max() <= 2
+    assert (uses == 2).mean() > 0.9
+    assert len(np.unique(T)) > 0.9 * len(V)  # few unreferenced vertices
+
+
 def test_pipeline_matches_oracle_pipeline():
     """The dense half of Tracking::TrackRGBD over 6 frames: GPU poses within
     1e-4 m / 1e-4 rad of the oracle's, same accept/reject decisions."""

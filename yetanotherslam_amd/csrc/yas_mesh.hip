@@ -4,11 +4,23 @@
 // MI355X design: one 512-thread workgroup per allocated block, one thread per
 // voxel.  The +1 corners of a block's voxels live in up to seven neighbour
 // blocks; their pool indices are resolved once per workgroup (seven hash
-// probes) into LDS instead of eight probes per voxel.  Extraction is two
-// passes with a device prefix scan in between, so every block's triangles
-// are contiguous and ordered by voxel index, then table order: the output is
-// deterministic, unlike the reference's atomic triangle counter
-// (MeshEngine.cu:246).
+// probes) into LDS instead of eight probes per voxel.  Extraction is a count
+// pass, a three-phase device scan of the per-block counts and an emit pass,
+// so every block's output is contiguous and ordered by voxel index, then
+// table order: the output is deterministic, unlike the reference's atomic
+// triangle counter (MeshEngine.cu:246).
+//
+// Two output forms share the count pass:
+//   * the reference's triangle soup (9 positions + 9 face-normal floats per
+//     triangle, MeshEngine.cu:246-264), bit-identical to the scalar
+//     restatement;
+//   * an indexed mesh (float32 xyz vertices, int32 triangle corners).  Each
+//     voxel owns the three cube edges leaving its corner along +x, +y, +z;
+//     an owned edge with a sign change between two observed end voxels gets
+//     one vertex, placed by the reference's interpolation evaluated from the
+//     lower end.  Triangles of a cube reference the owners' vertices, so a
+//     vertex is shared by every triangle that touches its edge.
+
 #include <cstring>
 #include <vector>
 
@@ -57,24 +69,53 @@ __device__ inline void edge_offset(int e, float t, float& ox, float& oy, float& 
   }
 }
 
-// Cube index of this thread's voxel (-1 if a corner is unobserved) and the
-// 8 corner values.  nb[k] = pool index of the block at offset
-// (k&1, k>>1&1, k>>2&1), -1 if absent.
+// Owner of each cube edge: the voxel offset (from the cube's corner 0) of
+// the edge's lower end and the edge's axis (0 = x, 1 = y, 2 = z).
+__constant__ int kOwner[12][4] = {{0, 0, 0, 0}, {1, 0, 0, 1}, {0, 1, 0, 0}, {0, 0, 0, 1}, {0, 0, 1, 0}, {1, 0, 1, 1},
+                                  {0, 1, 1, 0}, {0, 0, 1, 1}, {0, 0, 0, 2}, {1, 0, 0, 2}, {1, 1, 0, 2}, {0, 1, 0, 2}};
+
+// The 8 corner values of this thread's voxel and a bit mask of the corners
+// that are observed (in an allocated block with weight > 0).  nb[k] = pool
+// index of the block at offset (k&1, k>>1&1, k>>2&1), -1 if absent.
 template <class VT>
-__device__ inline int voxel_case(const typename VT::T* vox, const int* nb, int lx, int ly, int lz, float* s) {
-  int cube = 0;
+__device__ inline int read_corners(const typename VT::T* vox, const int* nb, int lx, int ly, int lz, float* s) {
+  int valid = 0;
 #pragma unroll
   for (int c = 0; c < 8; ++c) {
     int x = lx + kCorner[c][0], y = ly + kCorner[c][1], z = lz + kCorner[c][2];
     int k = (x >> 3) | ((y >> 3) << 1) | ((z >> 3) << 2);
     int blk = nb[k];
-    if (blk < 0) return -1;
+    s[c] = 0.0f;
+    if (blk < 0) continue;
     typename VT::T v = vox[(size_t)blk * 512 + ((z & 7) * 8 + (y & 7)) * 8 + (x & 7)];
-    if (!VT::observed(v)) return -1;
+    if (!VT::observed(v)) continue;
     s[c] = VT::sdf(v);
-    if (s[c] < 0.0f) cube |= 1 << c;
+    valid |= 1 << c;
   }
+  return valid;
+}
+
+// Cube index (MeshEngine.cu:76-112): -1 unless all 8 corners are observed.
+__device__ inline int cube_index(int valid, const float* s) {
+  if (valid != 0xFF) return -1;
+  int cube = 0;
+#pragma unroll
+  for (int c = 0; c < 8; ++c)
+    if (s[c] < 0.0f) cube |= 1 << c;
   return cube;
+}
+
+// Owned edges of the voxel (bit a = edge along axis a from corner 0 to
+// corner 1, 3 or 4) that carry a vertex.
+__device__ inline int owned_edges(int valid, const float* s) {
+  const int far[3] = {1, 3, 4};
+  int m = 0;
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    const int c = far[a];
+    if ((valid & 1) && (valid & (1 << c)) && ((s[0] < 0.0f) != (s[c] < 0.0f))) m |= 1 << a;
+  }
+  return m;
 }
 
 __device__ inline void resolve_neighbours(const MapView& m, int blk, int* nb, int& bx, int& by, int& bz) {
@@ -86,9 +127,34 @@ __device__ inline void resolve_neighbours(const MapView& m, int blk, int* nb, in
   __syncthreads();
 }
 
-// Pass 1: triangles per block (and optional cube indices).
+// Exclusive prefix of v over the threads of a workgroup (thread order), and
+// the workgroup total.  `wsum` holds one int per wavefront.
+__device__ inline int block_excl_scan(int v, int* wsum, int& total) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  int inc = v;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    int t = __shfl_up(inc, o, 64);
+    if (lane >= o) inc += t;
+  }
+  if (lane == 63) wsum[w] = inc;
+  __syncthreads();
+  int before = 0;
+  total = 0;
+  for (int i = 0; i < nw; ++i) {
+    const int x = wsum[i];
+    before += i < w ? x : 0;
+    total += x;
+  }
+  __syncthreads();
+  return before + inc - v;
+}
+
+// Count pass: per block, triangles and owned vertices; per voxel, the cube
+// index and (local vertex prefix << 3 | owned edge mask).
 template <class VT>
-__global__ void __launch_bounds__(512) k_mc_count(MapView m, int nblocks, int* counts, int16_t* cases) {
+__global__ void __launch_bounds__(512) k_mc_count(MapView m, int nblocks, int2* counts, int16_t* cases,
+                                                  uint16_t* vinfo) {
   __shared__ int nb[8];
   __shared__ int wsum[8];
   const int blk = blockIdx.x;
@@ -97,50 +163,75 @@ __global__ void __launch_bounds__(512) k_mc_count(MapView m, int nblocks, int* c
   resolve_neighbours(m, blk, nb, bx, by, bz);
   const int l = threadIdx.x, lx = l & 7, ly = (l >> 3) & 7, lz = l >> 6;
   float s[8];
-  int cube = voxel_case<VT>((const typename VT::T*)m.voxels, nb, lx, ly, lz, s);
-  if (cases) cases[(size_t)blk * 512 + l] = (int16_t)cube;
-  int nt = cube > 0 ? (int)(kTriTable[cube] & 0xF) : 0;
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) nt += __shfl_xor(nt, o, 64);
-  if ((l & 63) == 0) wsum[l >> 6] = nt;
-  __syncthreads();
-  if (l == 0) {
-    int t = 0;
-    for (int w = 0; w < 8; ++w) t += wsum[w];
-    counts[blk] = t;
-  }
+  const int valid = read_corners<VT>((const typename VT::T*)m.voxels, nb, lx, ly, lz, s);
+  const int cube = cube_index(valid, s);
+  const int om = owned_edges(valid, s);
+  cases[(size_t)blk * 512 + l] = (int16_t)cube;
+  const int nt = cube > 0 ? (int)(kTriTable[cube] & 0xF) : 0;
+  int ttot, vtot;
+  block_excl_scan(nt, wsum, ttot);
+  const int vpre = block_excl_scan(__popc(om), wsum, vtot);
+  vinfo[(size_t)blk * 512 + l] = (uint16_t)((vpre << 3) | om);
+  if (l == 0) counts[blk] = make_int2(ttot, vtot);
 }
 
-// Exclusive scan of the block counts: 1024-thread single workgroup, chunks
-// of 1024 with a running carry.  `total` receives the sum.
-__global__ void __launch_bounds__(1024) k_scan(const int* in, int* out, int n, int* total) {
-  __shared__ int buf[1024];
-  __shared__ int carry;
-  if (threadIdx.x == 0) carry = 0;
+// Three-phase exclusive scan of int2 counts: 1024-element chunks scanned in
+// parallel, the chunk totals scanned by one workgroup, then added back.
+__global__ void __launch_bounds__(1024) k_scan_local(const int2* in, int2* out, int n, int2* chunk) {
+  __shared__ int wsum[16];
+  const int i = blockIdx.x * 1024 + threadIdx.x;
+  const int2 v = i < n ? in[i] : make_int2(0, 0);
+  int tx, ty;
+  const int ex = block_excl_scan(v.x, wsum, tx);
+  const int ey = block_excl_scan(v.y, wsum, ty);
+  if (i < n) out[i] = make_int2(ex, ey);
+  if (threadIdx.x == 0) chunk[blockIdx.x] = make_int2(tx, ty);
+}
+
+__global__ void __launch_bounds__(1024) k_scan_chunks(int2* chunk, int nchunks, int2* total) {
+  __shared__ int wsum[16];
+  __shared__ int carry[2];
+  if (threadIdx.x == 0) carry[0] = carry[1] = 0;
   __syncthreads();
-  for (int base = 0; base < n; base += 1024) {
-    int i = base + threadIdx.x;
-    int v = i < n ? in[i] : 0;
-    buf[threadIdx.x] = v;
+  for (int base = 0; base < nchunks; base += 1024) {
+    const int i = base + threadIdx.x;
+    const int2 v = i < nchunks ? chunk[i] : make_int2(0, 0);
+    int tx, ty;
+    const int ex = block_excl_scan(v.x, wsum, tx);
+    const int ey = block_excl_scan(v.y, wsum, ty);
+    if (i < nchunks) chunk[i] = make_int2(carry[0] + ex, carry[1] + ey);
     __syncthreads();
-    for (int o = 1; o < 1024; o <<= 1) {
-      int t = threadIdx.x >= o ? buf[threadIdx.x - o] : 0;
-      __syncthreads();
-      buf[threadIdx.x] += t;
-      __syncthreads();
+    if (threadIdx.x == 0) {
+      carry[0] += tx;
+      carry[1] += ty;
     }
-    if (i < n) out[i] = carry + buf[threadIdx.x] - v;
-    __syncthreads();
-    if (threadIdx.x == 0) carry += buf[1023];
     __syncthreads();
   }
-  if (threadIdx.x == 0) *total = carry;
+  if (threadIdx.x == 0) *total = make_int2(carry[0], carry[1]);
 }
 
-// Pass 2: emit.  Triangle slot = block offset + exclusive prefix (in voxel
-// index order) of the per-voxel counts.
+__global__ void __launch_bounds__(1024) k_scan_add(int2* out, const int2* chunk, int n) {
+  const int i = blockIdx.x * 1024 + threadIdx.x;
+  if (i >= n) return;
+  const int2 c = chunk[blockIdx.x], v = out[i];
+  out[i] = make_int2(v.x + c.x, v.y + c.y);
+}
+
+// Vertex on cube edge e of the voxel at integer position p (voxel units),
+// MeshEngine.cu:114-224: corner A + t * (edge direction), scaled by vs.
+__device__ inline void edge_vertex(int e, const float* s, float px, float py, float pz, float vs, float* out) {
+  float t = edge_t(s[kEdgeA[e]], s[kEdgeB[e]]);
+  float ox, oy, oz;
+  edge_offset(e, t, ox, oy, oz);
+  out[0] = (px + ox) * vs;
+  out[1] = (py + oy) * vs;
+  out[2] = (pz + oz) * vs;
+}
+
+// Emit, reference form: triangle slot = block offset + exclusive prefix (in
+// voxel index order) of the per-voxel counts.
 template <class VT>
-__global__ void __launch_bounds__(512) k_mc_emit(MapView m, int nblocks, const int* offsets, float* tris,
+__global__ void __launch_bounds__(512) k_mc_emit(MapView m, int nblocks, const int2* offsets, float* tris,
                                                  float* normals, int cap) {
   __shared__ int nb[8];
   __shared__ int wsum[8];
@@ -150,39 +241,19 @@ __global__ void __launch_bounds__(512) k_mc_emit(MapView m, int nblocks, const i
   resolve_neighbours(m, blk, nb, bx, by, bz);
   const int l = threadIdx.x, lx = l & 7, ly = (l >> 3) & 7, lz = l >> 6;
   float s[8];
-  int cube = voxel_case<VT>((const typename VT::T*)m.voxels, nb, lx, ly, lz, s);
-  uint64_t word = cube > 0 ? kTriTable[cube] : 0ull;
-  int nt = (int)(word & 0xF);
-  // inclusive scan within the wavefront
-  int inc = nt;
-  const int lane = l & 63;
-#pragma unroll
-  for (int o = 1; o < 64; o <<= 1) {
-    int t = __shfl_up(inc, o, 64);
-    if (lane >= o) inc += t;
-  }
-  if (lane == 63) wsum[l >> 6] = inc;
-  __syncthreads();
-  int before = 0;
-  for (int w = 0; w < (l >> 6); ++w) before += wsum[w];
-  if (nt == 0) return;
-  int slot = offsets[blk] + before + inc - nt;
+  const int cube = cube_index(read_corners<VT>((const typename VT::T*)m.voxels, nb, lx, ly, lz, s), s);
+  const uint64_t word = cube > 0 ? kTriTable[cube] : 0ull;
+  const int nt = (int)(word & 0xF);
+  int total;
+  int slot = offsets[blk].x + block_excl_scan(nt, wsum, total);
   const float vs = m.voxel_size;
-  float px = (float)(bx * 8 + lx), py = (float)(by * 8 + ly), pz = (float)(bz * 8 + lz);
+  const float px = (float)(bx * 8 + lx), py = (float)(by * 8 + ly), pz = (float)(bz * 8 + lz);
   for (int i = 0; i < nt; ++i, ++slot) {
     if (slot >= cap) return;
     float a[3], b[3], c[3];
-    float* dst[3] = {a, b, c};
-#pragma unroll
-    for (int k = 0; k < 3; ++k) {
-      int e = (int)((word >> (4 + 12 * i + 4 * k)) & 0xF);
-      float t = edge_t(s[kEdgeA[e]], s[kEdgeB[e]]);
-      float ox, oy, oz;
-      edge_offset(e, t, ox, oy, oz);
-      dst[k][0] = (px + ox) * vs;
-      dst[k][1] = (py + oy) * vs;
-      dst[k][2] = (pz + oz) * vs;
-    }
+    edge_vertex((int)((word >> (4 + 12 * i)) & 0xF), s, px, py, pz, vs, a);
+    edge_vertex((int)((word >> (8 + 12 * i)) & 0xF), s, px, py, pz, vs, b);
+    edge_vertex((int)((word >> (12 + 12 * i)) & 0xF), s, px, py, pz, vs, c);
     float u0 = b[0] - a[0], u1 = b[1] - a[1], u2 = b[2] - a[2];
     float w0 = c[0] - a[0], w1 = c[1] - a[1], w2 = c[2] - a[2];
     float n0 = u1 * w2 - u2 * w1, n1 = u2 * w0 - u0 * w2, n2 = u0 * w1 - u1 * w0;
@@ -204,20 +275,79 @@ __global__ void __launch_bounds__(512) k_mc_emit(MapView m, int nblocks, const i
   }
 }
 
+// Emit, indexed form: the voxel's owned vertices, then its triangles as
+// indices of the owners' vertices.
+template <class VT>
+__global__ void __launch_bounds__(512) k_mc_emit_indexed(MapView m, int nblocks, const int2* offsets,
+                                                         const uint16_t* vinfo, float* verts, int vcap, int* tris,
+                                                         int tcap) {
+  __shared__ int nb[8];
+  __shared__ int wsum[8];
+  __shared__ int vbase[8];
+  const int blk = blockIdx.x;
+  if (blk >= nblocks) return;
+  int bx, by, bz;
+  resolve_neighbours(m, blk, nb, bx, by, bz);
+  if (threadIdx.x < 8) vbase[threadIdx.x] = nb[threadIdx.x] >= 0 ? offsets[nb[threadIdx.x]].y : 0;
+  const int l = threadIdx.x, lx = l & 7, ly = (l >> 3) & 7, lz = l >> 6;
+  float s[8];
+  const int valid = read_corners<VT>((const typename VT::T*)m.voxels, nb, lx, ly, lz, s);
+  const int cube = cube_index(valid, s);
+  const uint64_t word = cube > 0 ? kTriTable[cube] : 0ull;
+  const int nt = (int)(word & 0xF);
+  int total;
+  // (the scan's barriers also publish vbase)
+  int slot = offsets[blk].x + block_excl_scan(nt, wsum, total);
+  const float vs = m.voxel_size;
+  const float px = (float)(bx * 8 + lx), py = (float)(by * 8 + ly), pz = (float)(bz * 8 + lz);
+  // owned vertices, interpolated from this voxel towards +x, +y, +z
+  const int own = vinfo[(size_t)blk * 512 + l];
+  int vid = vbase[0] + (own >> 3);
+  const int far[3] = {1, 3, 4};
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    if (!(own & (1 << a))) continue;
+    const float t = edge_t(s[0], s[far[a]]);
+    if (vid < vcap) {
+      float* V = verts + (size_t)vid * 3;
+      V[0] = (px + (a == 0 ? t : 0.0f)) * vs;
+      V[1] = (py + (a == 1 ? t : 0.0f)) * vs;
+      V[2] = (pz + (a == 2 ? t : 0.0f)) * vs;
+    }
+    ++vid;
+  }
+  for (int i = 0; i < nt; ++i, ++slot) {
+    if (slot >= tcap) return;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      const int e = (int)((word >> (4 + 12 * i + 4 * k)) & 0xF);
+      const int x = lx + kOwner[e][0], y = ly + kOwner[e][1], z = lz + kOwner[e][2];
+      const int nbk = (x >> 3) | ((y >> 3) << 1) | ((z >> 3) << 2);
+      const int ob = nb[nbk];  // present: every corner of an emitting cube is observed
+      const int oi = vinfo[(size_t)ob * 512 + ((z & 7) * 8 + (y & 7)) * 8 + (x & 7)];
+      tris[(size_t)slot * 3 + k] = vbase[nbk] + (oi >> 3) + __popc((oi & 7) & ((1 << kOwner[e][3]) - 1));
+    }
+  }
+}
+
 }  // namespace
 
 struct yas_mesher {
   yas_ctx* ctx;
   int max_tri;
-  float* tris;
-  float* normals;
-  int* counts;
-  int* offsets;
-  int* total;
-  int16_t* cases;
+  float* tris;      // soup: 9 floats per triangle; indexed: 3 ints per triangle
+  float* normals;   // soup: 9 floats per triangle; indexed: 3 floats per vertex
+  int2* counts;     // per block {triangles, vertices}
+  int2* offsets;    // exclusive prefix of counts
+  int2* chunk;      // scan chunk totals
+  int2* total;      // {triangles, vertices}
+  int16_t* cases;   // per voxel cube index
+  uint16_t* vinfo;  // per voxel (local vertex prefix << 3 | owned edges)
   int block_cap;
   int num_blocks;
   int num_tri;
+  int num_vert;
+  int indexed;
   yas_map* map;
 };
 
@@ -225,14 +355,66 @@ static int ensure_block_cap(yas_mesher* me, int n) {
   if (n <= me->block_cap) return YAS_OK;
   hipFree(me->counts);
   hipFree(me->offsets);
+  hipFree(me->chunk);
   hipFree(me->cases);
-  me->counts = me->offsets = nullptr;
+  hipFree(me->vinfo);
+  me->counts = me->offsets = me->chunk = nullptr;
   me->cases = nullptr;
+  me->vinfo = nullptr;
   me->block_cap = 0;
-  YAS_HIP(hipMalloc((void**)&me->counts, sizeof(int) * n));
-  YAS_HIP(hipMalloc((void**)&me->offsets, sizeof(int) * n));
+  YAS_HIP(hipMalloc((void**)&me->counts, sizeof(int2) * n));
+  YAS_HIP(hipMalloc((void**)&me->offsets, sizeof(int2) * n));
+  YAS_HIP(hipMalloc((void**)&me->chunk, sizeof(int2) * ((n + 1023) / 1024)));
   YAS_HIP(hipMalloc((void**)&me->cases, sizeof(int16_t) * 512 * (size_t)n));
+  YAS_HIP(hipMalloc((void**)&me->vinfo, sizeof(uint16_t) * 512 * (size_t)n));
   me->block_cap = n;
+  return YAS_OK;
+}
+
+// Count + scan over every allocated block of `map`; fills me->num_blocks and
+// leaves the totals on the device.
+static int count_and_scan(yas_mesher* me, yas_map* map, MapView& v) {
+  YAS_REQUIRE(me && map, "yas_mesh_extract: null argument");
+  YAS_REQUIRE(yas_map_ctx(map)->stream == me->ctx->stream, "yas_mesh_extract: map and mesher must share a stream");
+  YAS_HIP(hipSetDevice(me->ctx->device));
+  yas_map_stats st;
+  int rc = yas_map_get_stats(map, &st);
+  if (rc) return rc;
+  const int n = st.num_allocated;
+  me->num_blocks = n;
+  me->num_tri = me->num_vert = 0;
+  me->map = map;
+  if (n == 0) return YAS_OK;
+  if ((rc = ensure_block_cap(me, n))) return rc;
+  v = yas_map_view(map);
+  hipStream_t s = me->ctx->stream;
+  switch (v.format) {
+    case YAS_VOXEL_I16_U8:
+      hipLaunchKernelGGL(k_mc_count<VoxI16>, dim3(n), dim3(512), 0, s, v, n, me->counts, me->cases, me->vinfo);
+      break;
+    case YAS_VOXEL_F16_U16:
+      hipLaunchKernelGGL(k_mc_count<VoxF16>, dim3(n), dim3(512), 0, s, v, n, me->counts, me->cases, me->vinfo);
+      break;
+    default:
+      hipLaunchKernelGGL(k_mc_count<VoxF32>, dim3(n), dim3(512), 0, s, v, n, me->counts, me->cases, me->vinfo);
+  }
+  YAS_LAUNCH_CHECK();
+  const int nchunks = (n + 1023) / 1024;
+  hipLaunchKernelGGL(k_scan_local, dim3(nchunks), dim3(1024), 0, s, (const int2*)me->counts, me->offsets, n, me->chunk);
+  YAS_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_scan_chunks, dim3(1), dim3(1024), 0, s, me->chunk, nchunks, me->total);
+  YAS_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_scan_add, dim3(nchunks), dim3(1024), 0, s, me->offsets, (const int2*)me->chunk, n);
+  YAS_LAUNCH_CHECK();
+  return YAS_OK;
+}
+
+static int read_totals(yas_mesher* me) {
+  int2 t;
+  YAS_HIP(hipMemcpyAsync(&t, me->total, sizeof(int2), hipMemcpyDeviceToHost, me->ctx->stream));
+  YAS_HIP(hipStreamSynchronize(me->ctx->stream));
+  me->num_tri = t.x;
+  me->num_vert = t.y;
   return YAS_OK;
 }
 
@@ -247,7 +429,7 @@ int yas_mesher_create(yas_ctx* ctx, int max_triangles, yas_mesher** out) {
   me->max_tri = max_triangles;
   hipError_t e = hipMalloc((void**)&me->tris, sizeof(float) * 9 * (size_t)max_triangles);
   if (e == hipSuccess) e = hipMalloc((void**)&me->normals, sizeof(float) * 9 * (size_t)max_triangles);
-  if (e == hipSuccess) e = hipMalloc((void**)&me->total, sizeof(int));
+  if (e == hipSuccess) e = hipMalloc((void**)&me->total, sizeof(int2));
   if (e != hipSuccess) {
     yas_set_error("yas_mesher_create: %s", hipGetErrorString(e));
     yas_mesher_destroy(me);
@@ -265,62 +447,80 @@ int yas_mesher_destroy(yas_mesher* me) {
   hipFree(me->normals);
   hipFree(me->counts);
   hipFree(me->offsets);
+  hipFree(me->chunk);
   hipFree(me->total);
   hipFree(me->cases);
+  hipFree(me->vinfo);
   delete me;
   return YAS_OK;
 }
 
 int yas_mesh_extract(yas_mesher* me, yas_map* map, int* num_triangles) {
-  YAS_REQUIRE(me && map, "yas_mesh_extract: null argument");
-  YAS_REQUIRE(yas_map_ctx(map)->stream == me->ctx->stream, "yas_mesh_extract: map and mesher must share a stream");
-  YAS_HIP(hipSetDevice(me->ctx->device));
-  yas_map_stats st;
-  int rc = yas_map_get_stats(map, &st);
-  if (rc) return rc;
-  const int n = st.num_allocated;
-  me->num_blocks = n;
-  me->num_tri = 0;
-  me->map = map;
   if (num_triangles) *num_triangles = 0;
+  MapView v;
+  int rc = count_and_scan(me, map, v);
+  if (rc) return rc;
+  me->indexed = 0;
+  const int n = me->num_blocks;
   if (n == 0) return YAS_OK;
-  if ((rc = ensure_block_cap(me, n))) return rc;
-  MapView v = yas_map_view(map);
   hipStream_t s = me->ctx->stream;
   switch (v.format) {
     case YAS_VOXEL_I16_U8:
-      hipLaunchKernelGGL(k_mc_count<VoxI16>, dim3(n), dim3(512), 0, s, v, n, me->counts, me->cases);
+      hipLaunchKernelGGL(k_mc_emit<VoxI16>, dim3(n), dim3(512), 0, s, v, n, (const int2*)me->offsets, me->tris,
+                         me->normals, me->max_tri);
       break;
     case YAS_VOXEL_F16_U16:
-      hipLaunchKernelGGL(k_mc_count<VoxF16>, dim3(n), dim3(512), 0, s, v, n, me->counts, me->cases);
+      hipLaunchKernelGGL(k_mc_emit<VoxF16>, dim3(n), dim3(512), 0, s, v, n, (const int2*)me->offsets, me->tris,
+                         me->normals, me->max_tri);
       break;
     default:
-      hipLaunchKernelGGL(k_mc_count<VoxF32>, dim3(n), dim3(512), 0, s, v, n, me->counts, me->cases);
+      hipLaunchKernelGGL(k_mc_emit<VoxF32>, dim3(n), dim3(512), 0, s, v, n, (const int2*)me->offsets, me->tris,
+                         me->normals, me->max_tri);
   }
   YAS_LAUNCH_CHECK();
-  hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, s, (const int*)me->counts, me->offsets, n, me->total);
-  YAS_LAUNCH_CHECK();
+  if ((rc = read_totals(me))) return rc;
+  if (num_triangles) *num_triangles = me->num_tri;
+  if (me->num_tri > me->max_tri) {
+    yas_set_error("yas_mesh_extract: %d triangles exceed the capacity %d", me->num_tri, me->max_tri);
+    return YAS_E_CAPACITY;
+  }
+  return YAS_OK;
+}
+
+int yas_mesh_extract_indexed(yas_mesher* me, yas_map* map, int* num_vertices, int* num_triangles) {
+  if (num_vertices) *num_vertices = 0;
+  if (num_triangles) *num_triangles = 0;
+  MapView v;
+  int rc = count_and_scan(me, map, v);
+  if (rc) return rc;
+  me->indexed = 1;
+  const int n = me->num_blocks;
+  if (n == 0) return YAS_OK;
+  hipStream_t s = me->ctx->stream;
+  // triangle indices in the soup buffer (3 of its 9 floats per triangle),
+  // vertices in the normal buffer (capacity max_tri * 3 vertices)
+  int* tri_idx = (int*)me->tris;
+  const int vcap = me->max_tri * 3;
   switch (v.format) {
     case YAS_VOXEL_I16_U8:
-      hipLaunchKernelGGL(k_mc_emit<VoxI16>, dim3(n), dim3(512), 0, s, v, n, (const int*)me->offsets, me->tris,
-                         me->normals, me->max_tri);
+      hipLaunchKernelGGL(k_mc_emit_indexed<VoxI16>, dim3(n), dim3(512), 0, s, v, n, (const int2*)me->offsets,
+                         (const uint16_t*)me->vinfo, me->normals, vcap, tri_idx, me->max_tri);
       break;
     case YAS_VOXEL_F16_U16:
-      hipLaunchKernelGGL(k_mc_emit<VoxF16>, dim3(n), dim3(512), 0, s, v, n, (const int*)me->offsets, me->tris,
-                         me->normals, me->max_tri);
+      hipLaunchKernelGGL(k_mc_emit_indexed<VoxF16>, dim3(n), dim3(512), 0, s, v, n, (const int2*)me->offsets,
+                         (const uint16_t*)me->vinfo, me->normals, vcap, tri_idx, me->max_tri);
       break;
     default:
-      hipLaunchKernelGGL(k_mc_emit<VoxF32>, dim3(n), dim3(512), 0, s, v, n, (const int*)me->offsets, me->tris,
-                         me->normals, me->max_tri);
+      hipLaunchKernelGGL(k_mc_emit_indexed<VoxF32>, dim3(n), dim3(512), 0, s, v, n, (const int2*)me->offsets,
+                         (const uint16_t*)me->vinfo, me->normals, vcap, tri_idx, me->max_tri);
   }
   YAS_LAUNCH_CHECK();
-  int total = 0;
-  YAS_HIP(hipMemcpyAsync(&total, me->total, sizeof(int), hipMemcpyDeviceToHost, s));
-  YAS_HIP(hipStreamSynchronize(s));
-  me->num_tri = total;
-  if (num_triangles) *num_triangles = total;
-  if (total > me->max_tri) {
-    yas_set_error("yas_mesh_extract: %d triangles exceed the capacity %d", total, me->max_tri);
+  if ((rc = read_totals(me))) return rc;
+  if (num_vertices) *num_vertices = me->num_vert;
+  if (num_triangles) *num_triangles = me->num_tri;
+  if (me->num_tri > me->max_tri || me->num_vert > vcap) {
+    yas_set_error("yas_mesh_extract_indexed: %d triangles / %d vertices exceed the capacity %d / %d", me->num_tri,
+                  me->num_vert, me->max_tri, vcap);
     return YAS_E_CAPACITY;
   }
   return YAS_OK;
@@ -328,11 +528,32 @@ int yas_mesh_extract(yas_mesher* me, yas_map* map, int* num_triangles) {
 
 int yas_mesher_download(yas_mesher* me, float* vertices, float* normals, int capacity) {
   YAS_REQUIRE(me, "null mesher");
+  YAS_REQUIRE(!me->indexed, "yas_mesher_download: the last extraction was indexed");
   int n = me->num_tri < me->max_tri ? me->num_tri : me->max_tri;
   if (n > capacity) n = capacity;
   if (n <= 0) return YAS_OK;
   if (vertices) YAS_HIP(hipMemcpy(vertices, me->tris, sizeof(float) * 9 * (size_t)n, hipMemcpyDeviceToHost));
   if (normals) YAS_HIP(hipMemcpy(normals, me->normals, sizeof(float) * 9 * (size_t)n, hipMemcpyDeviceToHost));
+  return YAS_OK;
+}
+
+int yas_mesher_download_indexed(yas_mesher* me, float* vertices, int32_t* triangles, int vertex_capacity,
+                                int triangle_capacity) {
+  YAS_REQUIRE(me, "null mesher");
+  YAS_REQUIRE(me->indexed, "yas_mesher_download_indexed: the last extraction was not indexed");
+  int nv = me->num_vert < me->max_tri * 3 ? me->num_vert : me->max_tri * 3;
+  int nt = me->num_tri < me->max_tri ? me->num_tri : me->max_tri;
+  if (nv > vertex_capacity) nv = vertex_capacity;
+  if (nt > triangle_capacity) nt = triangle_capacity;
+  if (vertices && nv > 0) YAS_HIP(hipMemcpy(vertices, me->normals, sizeof(float) * 3 * (size_t)nv, hipMemcpyDeviceToHost));
+  if (triangles && nt > 0) YAS_HIP(hipMemcpy(triangles, me->tris, sizeof(int32_t) * 3 * (size_t)nt, hipMemcpyDeviceToHost));
+  return YAS_OK;
+}
+
+int yas_mesher_indexed_device(yas_mesher* me, const float** vertices, const int32_t** triangles) {
+  YAS_REQUIRE(me && me->indexed, "yas_mesher_indexed_device: no indexed extraction");
+  if (vertices) *vertices = me->normals;
+  if (triangles) *triangles = (const int32_t*)me->tris;
   return YAS_OK;
 }
 
@@ -349,8 +570,15 @@ int yas_mesher_blocks(yas_mesher* me, int32_t* keys, int32_t* first, int32_t* co
     YAS_HIP(hipMemcpy(k.data(), v.block_keys, sizeof(unsigned long long) * n, hipMemcpyDeviceToHost));
     for (int i = 0; i < n; ++i) unpack_key(k[i], keys[3 * i], keys[3 * i + 1], keys[3 * i + 2]);
   }
-  if (first) YAS_HIP(hipMemcpy(first, me->offsets, sizeof(int) * n, hipMemcpyDeviceToHost));
-  if (count) YAS_HIP(hipMemcpy(count, me->counts, sizeof(int) * n, hipMemcpyDeviceToHost));
+  if (first || count) {
+    std::vector<int2> o(n), c(n);
+    YAS_HIP(hipMemcpy(o.data(), me->offsets, sizeof(int2) * n, hipMemcpyDeviceToHost));
+    YAS_HIP(hipMemcpy(c.data(), me->counts, sizeof(int2) * n, hipMemcpyDeviceToHost));
+    for (int i = 0; i < n; ++i) {
+      if (first) first[i] = o[i].x;
+      if (count) count[i] = c[i].x;
+    }
+  }
   if (cases) YAS_HIP(hipMemcpy(cases, me->cases, sizeof(int16_t) * 512 * (size_t)n, hipMemcpyDeviceToHost));
   return YAS_OK;
 }

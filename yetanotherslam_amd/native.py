@@ -65,6 +65,9 @@ SIGNATURES = {
     "yas_mesher_destroy": (I, [P]),
     "yas_mesh_extract": (I, [P, P, PI]),
     "yas_mesher_download": (I, [P, P, P, I]),
+    "yas_mesh_extract_indexed": (I, [P, P, PI, PI]),
+    "yas_mesher_download_indexed": (I, [P, P, P, I, I]),
+    "yas_mesher_indexed_device": (I, [P, P, P]),
     "yas_mesher_blocks": (I, [P, P, P, P, P, I, PI]),
     "yas_tracker_create": (I, [P, I, I, P, ctypes.POINTER(TrackerParams), PP]),
     "yas_tracker_destroy": (I, [P]),

@@ -284,6 +284,18 @@ class MeshEngine:
         return out
 
 
+    def MeshifyIndexed(self, map_struct):
+        """Indexed marching cubes: float32 (V, 3) vertices, int32 (T, 3)
+        triangles, in the same triangle order as Meshify."""
+        nv, nt = ctypes.c_int(), ctypes.c_int()
+        call("yas_mesh_extract_indexed", self.h, map_struct.h, ctypes.byref(nv), ctypes.byref(nt), allow=(6,))
+        kv, kt = min(nv.value, 3 * self.max_triangles), min(nt.value, self.max_triangles)
+        verts = np.zeros((max(kv, 1), 3), np.float32)
+        tris = np.zeros((max(kt, 1), 3), np.int32)
+        call("yas_mesher_download_indexed", self.h, _ptr(verts), _ptr(tris), kv, kt)
+        return dict(num_vertices=nv.value, num_triangles=nt.value, vertices=verts[:kv], triangles=tris[:kt])
+
+
 def canonical_mesh(m):
     """Reorder a mesh's blocks by key (x, y, z); triangles keep their
     within-block order.  Returns (keys, counts, cases|None, tris, normals)."""
