@@ -235,6 +235,13 @@ template <> struct Vox<VRef> {
     v.s = pack((old * (float)w + sdf) / (float)(w + 1));
     v.w = (uint8_t)std::min(255, w + 1);
   }
+  // VoxelMap.cu:373-375: (sdf*w + sdf'*w') / (w + w'), weight capped at 255.
+  static void merge(VRef& v, float sdf, int wo, int /*maxw*/) {
+    float old = (float)v.s / 32767.0f;
+    int w = v.w;
+    v.s = pack((old * (float)w + sdf * (float)wo) / (float)(w + wo));
+    v.w = (uint8_t)std::min(255, w + wo);
+  }
 };
 template <> struct Vox<VF32> {
   typedef float R;
@@ -249,6 +256,11 @@ template <> struct Vox<VF32> {
     }
     v.s = (v.s * w + sdf) / (w + 1.0f);
     v.w = std::min((float)maxw, w + 1.0f);
+  }
+  static void merge(VF32& v, float sdf, int wo, int maxw) {
+    float w = v.w;
+    v.s = (v.s * w + sdf * (float)wo) / (w + (float)wo);
+    v.w = std::min((float)maxw, w + (float)wo);
   }
 };
 template <> struct Vox<VF16> {
@@ -266,6 +278,12 @@ template <> struct Vox<VF16> {
     v.s = f32_to_f16((old * (float)w + sdf) / (float)(w + 1));
     v.w = (uint16_t)std::min(maxw, w + 1);
   }
+  static void merge(VF16& v, float sdf, int wo, int maxw) {
+    int w = v.w;
+    float old = f16_to_f32(v.s);
+    v.s = f32_to_f16((old * (float)w + sdf * (float)wo) / (float)(w + wo));
+    v.w = (uint16_t)std::min(maxw, w + wo);
+  }
 };
 template <> struct Vox<VF64> {
   typedef double R;
@@ -280,6 +298,11 @@ template <> struct Vox<VF64> {
     }
     v.s = (v.s * w + sdf) / (w + 1.0);
     v.w = std::min((double)maxw, w + 1.0);
+  }
+  static void merge(VF64& v, double sdf, int wo, int maxw) {
+    double w = v.w;
+    v.s = (v.s * w + sdf * (double)wo) / (w + (double)wo);
+    v.w = std::min((double)maxw, w + (double)wo);
   }
 };
 
@@ -375,6 +398,7 @@ struct MapBase {
   virtual long mesh(int32_t* block_keys, int32_t* tri_counts, int16_t* cases, double* tris, double* normals,
                     long tri_cap, int block_cap) = 0;
   virtual int precision() const = 0;
+  virtual int fuse_map(MapBase* src, const double Tdst[16], const double Tsrc[16]) = 0;
 };
 
 template <class V> struct Map : MapBase {
@@ -390,6 +414,76 @@ template <class V> struct Map : MapBase {
       : vs((R)voxel), mu((R)trunc), dmin((R)dmin_), dmax((R)dmax_), maxw(maxw_) {}
 
   int precision() const override { return sizeof(R) == 4 ? 32 : 64; }
+
+  // ------------------------------------------------------------ map merge
+  // MapStruct::Fuse(MapStruct*) (VoxelMap.cu:400-449) with the defect of
+  // CreateBlockFunctor (VoxelMap.cu:233: a 1-D index over an 8x8 launch,
+  // which visits one voxel column in eight) fixed: every observed source
+  // voxel is visited.  Step 1 (VoxelMap.cu:228-259): each observed voxel of
+  // src, moved into this map's frame, allocates its block here.  Step 2
+  // (VoxelMap.cu:360-378): every voxel here samples src trilinearly at its
+  // position moved into src's frame and, when the 8 samples are observed,
+  // merges it with the weight of the first sample.
+  int fuse_map(MapBase* base, const double Tdst[16], const double Tsrc[16]) override {
+    Map<V>* src = dynamic_cast<Map<V>*>(base);
+    if (!src) return -1;
+    double inv[16], Tsd[16], Tds[16];
+    inverse_d(Tdst, inv);
+    mul_d(inv, Tsrc, Tsd);  // src frame -> this frame
+    inverse_d(Tsrc, inv);
+    mul_d(inv, Tdst, Tds);  // this frame -> src frame
+    const Pose<R> Psd = pose_from<R>(Tsd), Pds = pose_from<R>(Tds);
+    int created = 0;
+    for (const auto& kv : src->blocks) {
+      const I3 B = unpack_key(kv.first);
+      for (int l = 0; l < 512; ++l) {
+        if (Vox<V>::wt(kv.second[l]) == 0) continue;
+        const R p[3] = {(R)(B.x * 8 + (l & 7)) * vs, (R)(B.y * 8 + ((l >> 3) & 7)) * vs, (R)(B.z * 8 + (l >> 6)) * vs};
+        R q[3];
+        xform(Psd, p, q);
+        insert(vox_to_block(floor_sat(q[0] / vs)), vox_to_block(floor_sat(q[1] / vs)), vox_to_block(floor_sat(q[2] / vs)),
+               created);
+      }
+    }
+    for (auto& kv : blocks) {
+      const I3 B = unpack_key(kv.first);
+      for (int l = 0; l < 512; ++l) {
+        const R p[3] = {(R)(B.x * 8 + (l & 7)) * vs, (R)(B.y * 8 + ((l >> 3) & 7)) * vs, (R)(B.z * 8 + (l >> 6)) * vs};
+        R q[3];
+        xform(Pds, p, q);
+        int wo = 0;
+        R sdf;
+        if (!src->interp_merge(q, sdf, wo)) continue;
+        Vox<V>::merge(kv.second[l], sdf, wo, maxw);
+      }
+    }
+    return created;
+  }
+
+  // MapStructFusionFunctor::InterpolateSDF / ReadSDF (VoxelMap.cu:290-358):
+  // trilinear in the reference's evaluation order; *wt gets the weight of
+  // the first corner.  False (and no value) unless all 8 corners are observed.
+  bool interp_merge(const R q[3], R& out, int& wt) const {
+    R vp[3];
+    int v0[3];
+    for (int i = 0; i < 3; ++i) {
+      vp[i] = q[i] / vs;
+      v0[i] = floor_sat(vp[i]);
+    }
+    const R x = vp[0] - (R)v0[0], y = vp[1] - (R)v0[1], z = vp[2] - (R)v0[2];
+    R s[8];
+    for (int c = 0; c < 8; ++c) {
+      const V* v = find_voxel(v0[0] + (c & 1), v0[1] + ((c >> 1) & 1), v0[2] + ((c >> 2) & 1));
+      if (!v || Vox<V>::wt(*v) == 0) return false;
+      if (c == 0) wt = (int)Vox<V>::wt(*v);
+      s[c] = Vox<V>::sdf(*v);
+    }
+    R s2 = s[0] * (1 - x) + s[1] * x;
+    R s3 = s2 * (1 - y) + (s[2] * (1 - x) + s[3] * x) * y;
+    s2 = s[4] * (1 - x) + s[5] * x;
+    out = s3 * (1 - z) + (s2 * (1 - y) + (s[6] * (1 - x) + s[7] * x) * y) * z;
+    return true;
+  }
 
   const V* find_voxel(int vx, int vy, int vz) const {
     auto it = blocks.find(pack_key(vox_to_block(vx), vox_to_block(vy), vox_to_block(vz)));
@@ -1417,6 +1511,9 @@ int orc_map_visible(void* m, int W, int H, const double K[4], const double T[16]
   return ((MapBase*)m)->visible(W, H, K, T, keys, cap);
 }
 int orc_map_num_blocks(void* m) { return ((MapBase*)m)->num_blocks(); }
+int orc_map_fuse_map(void* dst, void* src, const double Tdst[16], const double Tsrc[16]) {
+  return ((MapBase*)dst)->fuse_map((MapBase*)src, Tdst, Tsrc);
+}
 int orc_map_export(void* m, int32_t* keys, double* sdf, double* wt, int cap) {
   return ((MapBase*)m)->export_blocks(keys, sdf, wt, cap);
 }

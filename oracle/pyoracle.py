@@ -34,6 +34,7 @@ def lib():
         "orc_map_allocate": (ctypes.c_int, [P, P, ctypes.c_int, ctypes.c_int, P, P]),
         "orc_map_visible": (ctypes.c_int, [P, ctypes.c_int, ctypes.c_int, P, P, P, ctypes.c_int]),
         "orc_map_num_blocks": (ctypes.c_int, [P]),
+        "orc_map_fuse_map": (ctypes.c_int, [P, P, P, P]),
         "orc_map_export": (ctypes.c_int, [P, P, P, P, ctypes.c_int]),
         "orc_map_raycast": (ctypes.c_int, [P, ctypes.c_int, ctypes.c_int, P, P, P, P, P]),
         "orc_map_mesh": (ctypes.c_long, [P, P, P, P, P, P, ctypes.c_long, ctypes.c_int]),
@@ -113,6 +114,15 @@ class OracleMap:
 
     def num_blocks(self):
         return lib().orc_map_num_blocks(self.h)
+
+    def fuse_map(self, src, T_self, T_src):
+        """Merge `src` (same voxel size and storage) into this map; T_self and
+        T_src are the two maps' poses (map -> world).  Returns the number of
+        blocks created here."""
+        r = lib().orc_map_fuse_map(self.h, src.h, _p(_f64(T_self)), _p(_f64(T_src)))
+        if r < 0:
+            raise ValueError("maps differ in storage or precision")
+        return r
 
     def export(self, with_voxels=True):
         n = self.num_blocks()

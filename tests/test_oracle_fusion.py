@@ -202,3 +202,76 @@ def test_storage_variants_against_fp64_tsdf():
         assert ((w > 0) == (w2 > 0)).mean() > 0.999
         rms_vox = np.sqrt(np.mean((s[both] - s2[both]) ** 2)) * mu / vs
         assert rms_vox < 0.1, (fmt, rms_vox)
+
+
+def voxel_dict(keys, sdf, wt):
+    out = {}
+    for b, s_blk, w_blk in zip(keys, sdf, wt):
+        for l in range(512):
+            if w_blk[l] > 0:
+                out[(b[0] * 8 + (l & 7), b[1] * 8 + ((l >> 3) & 7), b[2] * 8 + (l >> 6))] = (s_blk[l], w_blk[l])
+    return out
+
+
+def test_map_merge_by_a_whole_voxel_shift_is_a_copy():
+    """Map-to-map fusion (VoxelMap.cu:400-449) into an empty map whose pose is
+    the source's moved by 3 voxels along x.  With a power-of-two voxel size
+    every coordinate is exact, so each destination voxel u samples the source
+    exactly at u + 3 (trilinear weights 1, 0): it holds that source voxel when
+    all 8 trilinear corners are observed and is unobserved otherwise, and the
+    destination blocks are those of the shifted observed source voxels."""
+    vs = 2.0 ** -7
+    W, H = 32, 24
+    K = np.array([30.0, 30.0, 15.5, 11.5])
+    rng = np.random.default_rng(3)
+    depth = (0.5 + 0.05 * rng.random((H, W))).astype(np.float32)
+    src = po.OracleMap(vs, 3 * vs, voxfmt=po.VOX_F32)
+    src.fuse(depth, K, np.eye(4))
+    dst = po.OracleMap(vs, 3 * vs, voxfmt=po.VOX_F32)
+    Tdst = np.eye(4)
+    Tdst[0, 3] = 3 * vs
+    dst.fuse_map(src, Tdst, np.eye(4))
+    S = voxel_dict(*src.export())
+    # blocks of the observed source voxels moved by -3 voxels
+    want_blocks = {(po.lib().orc_vox_to_block(v[0] - 3), po.lib().orc_vox_to_block(v[1]),
+                    po.lib().orc_vox_to_block(v[2])) for v in S}
+    keys, sdf, wt = dst.export()
+    assert {tuple(k) for k in keys} == want_blocks and len(want_blocks) > 20
+    D = voxel_dict(keys, sdf, wt)
+    corners = [(dx, dy, dz) for dz in (0, 1) for dy in (0, 1) for dx in (0, 1)]
+    n_full = 0
+    for b in keys:
+        for l in range(512):
+            u = (b[0] * 8 + (l & 7), b[1] * 8 + ((l >> 3) & 7), b[2] * 8 + (l >> 6))
+            v = (u[0] + 3, u[1], u[2])
+            full = all((v[0] + c[0], v[1] + c[1], v[2] + c[2]) in S for c in corners)
+            if full:
+                assert D[u] == S[v], (u, D.get(u), S[v])
+                n_full += 1
+            else:
+                assert u not in D
+    assert n_full > 1000
+
+
+def test_map_merge_weights_average_two_maps():
+    """Merging a map into a copy of itself at the same pose averages equal
+    values (the sdf stays, the weights add); a second source seen from a
+    moved camera changes sdf only where it overlaps."""
+    W, H = 32, 24
+    K = np.array([30.0, 30.0, 15.5, 11.5])
+    depth = np.full((H, W), 0.8, np.float32)
+    a = po.OracleMap(0.01, 0.03, voxfmt=po.VOX_F32)
+    b = po.OracleMap(0.01, 0.03, voxfmt=po.VOX_F32)
+    for m in (a, b):
+        m.fuse(depth, K, np.eye(4))
+        m.fuse(depth, K, np.eye(4))
+    ka, sa, wa = a.export()
+    a.fuse_map(b, np.eye(4), np.eye(4))
+    ka2, sa2, wa2 = a.export()
+    np.testing.assert_array_equal(ka, ka2)
+    # interior voxels (8 corners observed in b) get weight 2 + 2 and keep their value
+    merged = wa2 > wa
+    assert merged.sum() > 1000
+    np.testing.assert_array_equal(wa2[merged], 4.0)
+    np.testing.assert_allclose(sa2[merged], sa[merged], rtol=0, atol=1e-6)
+    assert np.all(wa2[~merged] == wa[~merged]) and np.all(sa2[~merged] == sa[~merged])

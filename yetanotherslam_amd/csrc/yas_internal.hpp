@@ -13,6 +13,7 @@
 
 #include <cstdint>
 #include <cstdio>
+#include <mutex>
 #include <string>
 
 #include "../../include/yas.h"
@@ -324,4 +325,87 @@ __device__ inline float wave_sum(float v) {
   return v;
 }
 
+// Insert the keys held by the lanes with `has`, one CAS per distinct key of
+// the wavefront; new blocks get consecutive pool indices from one atomic.
+__device__ inline void wave_insert(const MapView& m, uint64_t key, bool has, int home) {
+  bool pending = has;
+  bool leader = false;
+  for (;;) {
+    unsigned long long mask = __ballot(pending);
+    if (mask == 0ull) break;
+    int l = __ffsll((unsigned long long)mask) - 1;
+    unsigned long long lk = __shfl((unsigned long long)key, l);
+    if (lane_id() == l) leader = true;
+    if (pending && key == lk) pending = false;
+  }
+  bool won = false;
+  int slot = home;
+  if (leader) {
+    const int lim = m.num_buckets < kMaxProbe ? m.num_buckets : kMaxProbe;
+    int p = 0;
+    for (; p < lim; ++p) {
+      unsigned long long k = __hip_atomic_load(&m.slots[slot].key, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      if (k == key) break;
+      if (k == kEmptyKey) {
+        unsigned long long old = atomicCAS(&m.slots[slot].key, kEmptyKey, (unsigned long long)key);
+        if (old == kEmptyKey) {
+          won = true;
+          break;
+        }
+        if (old == key) break;
+      }
+      slot = slot + 1 == m.num_buckets ? 0 : slot + 1;
+    }
+    if (p == lim) atomicOr(&m.ctr[kCtrFlags], kFlagProbeLimit);
+  }
+  unsigned long long wm = __ballot(won);
+  if (wm) {
+    int first = __ffsll(wm) - 1;
+    int base = 0;
+    if (lane_id() == first) base = atomicAdd(&m.ctr[kCtrAlloc], __popcll(wm));
+    base = __shfl(base, first);
+    if (won) {
+      int idx = base + __popcll(wm & lane_mask_lt());
+      if (idx < m.num_blocks) {
+        m.block_keys[idx] = key;
+        m.block_slot[idx] = slot;
+        m.slots[slot].flags = kBlockUnobserved;
+        m.slots[slot].val = idx;
+      } else {
+        m.slots[slot].val = -1;
+        atomicOr(&m.ctr[kCtrFlags], kFlagPoolOverflow);
+      }
+    }
+  }
+}
+
 }  // namespace yas
+
+// One volume (MapStruct, VoxelMap.h:32-144): parameters, pose, device view.
+// While hibernating, the hash table, block list and pool live in pinned host
+// memory and the device view's big arrays are released.
+struct yas_map {
+  yas_ctx* ctx;
+  yas_map_params P;
+  yas::Intr K;
+  double pose[16];
+  std::mutex pose_mutex;
+  yas::MapView v;
+  yas::PosePair* pose_dev;  // pose of the current call
+  float* depth_f32;         // scratch for uint16 depth (lazily sized)
+  size_t depth_f32_bytes;
+  int* host_ctr;            // pinned mirror of the counters
+  bool hibernating;
+  int hib_blocks;                 // allocated blocks at hibernation
+  unsigned long long* hib_keys;   // pinned [hib_blocks]
+  int* hib_flags;                 // pinned [hib_blocks]
+  void* hib_voxels;               // pinned [hib_blocks * 512] voxels
+};
+
+#define YAS_REQUIRE_ACTIVE(map)                                                                    \
+  do {                                                                                             \
+    if ((map) && (map)->hibernating) {                                                             \
+      yas_set_error("%s: the map is hibernating (call yas_map_reactivate)", __func__);             \
+      return YAS_E_INVALID;                                                                        \
+    }                                                                                              \
+  } while (0)

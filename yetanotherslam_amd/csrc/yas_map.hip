@@ -39,19 +39,6 @@ void yas_set_error(const char* fmt, ...) {
   g_last_error = buf;
 }
 
-struct yas_map {
-  yas_ctx* ctx;
-  yas_map_params P;
-  Intr K;
-  double pose[16];
-  std::mutex pose_mutex;
-  MapView v;
-  PosePair* pose_dev;  // pose of the current call
-  float* depth_f32;    // scratch for uint16 depth (lazily sized)
-  size_t depth_f32_bytes;
-  int* host_ctr;       // pinned mirror of the counters
-};
-
 extern "C" {
 
 const char* yas_version(void) { return "yas-mi355x 0.1.0 (gfx950)"; }
@@ -145,60 +132,6 @@ __global__ void __launch_bounds__(256) k_depth_u16(const uint16_t* src, size_t s
   if (x >= W || y >= H) return;
   uint16_t d = ((const uint16_t*)((const char*)src + y * sp))[x];
   ((float*)((char*)dst + y * dp))[x] = (float)d * scale;
-}
-
-// Insert the keys held by the lanes with `has`, one CAS per distinct key of
-// the wavefront; new blocks get consecutive pool indices from one atomic.
-__device__ inline void wave_insert(const MapView& m, uint64_t key, bool has, int home) {
-  bool pending = has;
-  bool leader = false;
-  for (;;) {
-    unsigned long long mask = __ballot(pending);
-    if (mask == 0ull) break;
-    int l = __ffsll((unsigned long long)mask) - 1;
-    unsigned long long lk = __shfl((unsigned long long)key, l);
-    if (lane_id() == l) leader = true;
-    if (pending && key == lk) pending = false;
-  }
-  bool won = false;
-  int slot = home;
-  if (leader) {
-    const int lim = m.num_buckets < kMaxProbe ? m.num_buckets : kMaxProbe;
-    int p = 0;
-    for (; p < lim; ++p) {
-      unsigned long long k = __hip_atomic_load(&m.slots[slot].key, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      if (k == key) break;
-      if (k == kEmptyKey) {
-        unsigned long long old = atomicCAS(&m.slots[slot].key, kEmptyKey, (unsigned long long)key);
-        if (old == kEmptyKey) {
-          won = true;
-          break;
-        }
-        if (old == key) break;
-      }
-      slot = slot + 1 == m.num_buckets ? 0 : slot + 1;
-    }
-    if (p == lim) atomicOr(&m.ctr[kCtrFlags], kFlagProbeLimit);
-  }
-  unsigned long long wm = __ballot(won);
-  if (wm) {
-    int first = __ffsll(wm) - 1;
-    int base = 0;
-    if (lane_id() == first) base = atomicAdd(&m.ctr[kCtrAlloc], __popcll(wm));
-    base = __shfl(base, first);
-    if (won) {
-      int idx = base + __popcll(wm & lane_mask_lt());
-      if (idx < m.num_blocks) {
-        m.block_keys[idx] = key;
-        m.block_slot[idx] = slot;
-        m.slots[slot].flags = kBlockUnobserved;
-        m.slots[slot].val = idx;
-      } else {
-        m.slots[slot].val = -1;
-        atomicOr(&m.ctr[kCtrFlags], kFlagPoolOverflow);
-      }
-    }
-  }
 }
 
 // Pixel ray -> block (VoxelStructUtils.h:97-110, 28-49).
@@ -545,6 +478,7 @@ int yas_map_create(yas_ctx* ctx, const yas_map_params* P, const float K[4], yas_
 
 int yas_map_reset(yas_map* m) {
   YAS_REQUIRE(m, "null map");
+  YAS_REQUIRE_ACTIVE(m);
   hipStream_t s = m->ctx->stream;
   hipLaunchKernelGGL(k_reset_hash, dim3(grid_for(m->P.num_buckets, 256, 8192)), dim3(256), 0, s, m->v.slots,
                      m->P.num_buckets);
@@ -570,6 +504,9 @@ int yas_map_destroy(yas_map* m) {
   hipFree(m->pose_dev);
   hipFree(m->depth_f32);
   if (m->host_ctr) hipHostFree(m->host_ctr);
+  if (m->hib_keys) hipHostFree(m->hib_keys);
+  if (m->hib_flags) hipHostFree(m->hib_flags);
+  if (m->hib_voxels) hipHostFree(m->hib_voxels);
   delete m;
   return YAS_OK;
 }
@@ -590,6 +527,7 @@ int yas_map_get_pose(yas_map* m, double T[16]) {
 
 int yas_map_fuse(yas_map* m, const float* depth, size_t pitch, int W, int H, const double T[16]) {
   YAS_REQUIRE(m && depth && T, "yas_map_fuse: null argument");
+  YAS_REQUIRE_ACTIVE(m);
   YAS_REQUIRE(W > 0 && H > 0 && pitch >= (size_t)W * sizeof(float), "yas_map_fuse: bad image size");
   YAS_HIP(hipSetDevice(m->ctx->device));
   int rc = prepare_pose(m, T, true);
@@ -600,6 +538,7 @@ int yas_map_fuse(yas_map* m, const float* depth, size_t pitch, int W, int H, con
 int yas_map_fuse_u16(yas_map* m, const uint16_t* depth, size_t pitch, int W, int H, float depth_scale,
                      const double T[16]) {
   YAS_REQUIRE(m && depth && T, "yas_map_fuse_u16: null argument");
+  YAS_REQUIRE_ACTIVE(m);
   YAS_REQUIRE(W > 0 && H > 0 && pitch >= (size_t)W * sizeof(uint16_t), "yas_map_fuse_u16: bad image size");
   YAS_REQUIRE(depth_scale > 0.f, "yas_map_fuse_u16: depth_scale must be > 0");
   YAS_HIP(hipSetDevice(m->ctx->device));
@@ -619,6 +558,7 @@ int yas_map_fuse_u16(yas_map* m, const uint16_t* depth, size_t pitch, int W, int
 
 int yas_map_check_visible_blocks(yas_map* m, int W, int H, const double T[16], int* count) {
   YAS_REQUIRE(m && T && count, "null argument");
+  YAS_REQUIRE_ACTIVE(m);
   YAS_HIP(hipSetDevice(m->ctx->device));
   int rc = yas_map_visible_async(m, T, W, H);
   if (rc) return rc;
@@ -645,6 +585,7 @@ int yas_map_voxel_bytes(yas_map* m) { return m ? voxel_bytes(m->P.voxel_format) 
 
 int yas_map_export(yas_map* m, int32_t* keys, void* voxels, int capacity, int* count) {
   YAS_REQUIRE(m && count, "null argument");
+  YAS_REQUIRE_ACTIVE(m);
   yas_map_stats st;
   int rc = yas_map_get_stats(m, &st);
   if (rc) return rc;

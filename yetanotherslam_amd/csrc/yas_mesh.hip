@@ -375,6 +375,7 @@ static int ensure_block_cap(yas_mesher* me, int n) {
 // leaves the totals on the device.
 static int count_and_scan(yas_mesher* me, yas_map* map, MapView& v) {
   YAS_REQUIRE(me && map, "yas_mesh_extract: null argument");
+  YAS_REQUIRE_ACTIVE(map);
   YAS_REQUIRE(yas_map_ctx(map)->stream == me->ctx->stream, "yas_mesh_extract: map and mesher must share a stream");
   YAS_HIP(hipSetDevice(me->ctx->device));
   yas_map_stats st;

@@ -297,6 +297,7 @@ int yas_raycaster_destroy(yas_raycaster* rc) {
 
 int yas_raycast(yas_raycaster* rc, yas_map* map, const double T[16]) {
   YAS_REQUIRE(rc && map && T, "yas_raycast: null argument");
+  YAS_REQUIRE_ACTIVE(map);
   YAS_HIP(hipSetDevice(rc->ctx->device));
   YAS_REQUIRE(yas_map_ctx(map)->stream == rc->ctx->stream, "yas_raycast: map and ray caster must share a stream");
   int r = yas_map_visible_async(map, T, rc->W, rc->H);
