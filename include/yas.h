@@ -137,8 +137,40 @@ int yas_map_get_stats(yas_map* map, yas_map_stats* stats);
  * blocks; at most `capacity` are copied.  Synchronous.  No reference
  * counterpart beyond Hibernate's device-to-host copy (VoxelMap.cu:894-923). */
 int yas_map_export(yas_map* map, int32_t* keys, void* voxels, int capacity, int* count);
+/* Parameters and intrinsics {fx, fy, cx, cy} the map was created with (K may
+ * be NULL); the sizes include any growth by yas_map_reserve. */
+int yas_map_get_params(yas_map* map, yas_map_params* params, float K[4]);
 /* Bytes per voxel of the map's storage format. */
 int yas_map_voxel_bytes(yas_map* map);
+
+/* ------------------------------------------------------------------ map management
+ * Submaps and paging around the per-frame path (VoxelMap.h:49-70). */
+/* Fuse(MapStruct*) (VoxelMap.cu:400-449): re-sample src's TSDF into dst at
+ * the relative pose of the two maps' poses (yas_map_set_pose).  dst first
+ * grows by src's capacities (Reserve, VoxelMap.cu:412-417); then every
+ * observed src voxel allocates its block in dst, and every dst voxel whose 8
+ * trilinear src samples are observed takes the weighted mean with the first
+ * sample's weight.  Same voxel size and storage required; src is not
+ * modified (the reference releases it: the facade does).  Synchronous. */
+int yas_map_fuse_map(yas_map* dst, yas_map* src);
+/* Reserve(h, b, v) (VoxelMap.cu:485-515): grow the hash table to num_buckets
+ * slots and the block pool to num_blocks blocks (no-op when both already
+ * fit); allocated blocks keep their pool index and contents. Synchronous. */
+int yas_map_reserve(yas_map* map, int num_buckets, int num_blocks);
+/* Hibernate() / ReActivate() (VoxelMap.cu:894-954): move the allocated
+ * blocks (keys, summary flags, voxels) to pinned host memory and release the
+ * table, pool and lists on the device / bring them back and rebuild the
+ * table.  While hibernating, calls that need the volume return
+ * YAS_E_INVALID; pose and counters stay available. Synchronous. */
+int yas_map_hibernate(yas_map* map);
+int yas_map_reactivate(yas_map* map);
+int yas_map_is_hibernating(yas_map* map, int* hibernating);
+/* SaveToFile / ReadFromFile (VoxelMap.cu:886-892, empty in the reference):
+ * a versioned binary file with the map's parameters, intrinsics, pose and
+ * its allocated blocks (key, summary flags, voxels).  yas_map_load creates a
+ * new map on ctx. Synchronous. */
+int yas_map_save(yas_map* map, const char* path);
+int yas_map_load(yas_ctx* ctx, const char* path, yas_map** out);
 
 /* ------------------------------------------------------------------ raycast
  * RayTraceEngine (RGBDFusion/include/RayTraceEngine.h:11-48). */
@@ -184,6 +216,9 @@ int yas_mesh_extract_indexed(yas_mesher* me, yas_map* map, int* num_vertices, in
  * triangle); either pointer may be NULL. Synchronous. */
 int yas_mesher_download_indexed(yas_mesher* me, float* vertices, int32_t* triangles, int vertex_capacity,
                                 int triangle_capacity);
+/* Write the last indexed extraction as a binary little-endian PLY (float
+ * x, y, z vertices; uchar-counted int32 triangle lists). Synchronous. */
+int yas_mesher_write_ply(yas_mesher* me, const char* path);
 /* Device pointers of the last indexed extraction (valid until the next
  * extraction or yas_mesher_destroy). */
 int yas_mesher_indexed_device(yas_mesher* me, const float** vertices, const int32_t** triangles);

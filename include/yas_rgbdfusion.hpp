@@ -12,6 +12,7 @@
 #include <cstring>
 #include <stdexcept>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "yas.h"
@@ -118,6 +119,54 @@ class MapStruct {
     normal_.clear();
     N = 0;
   }
+  // Fuse(MapStruct*) (VoxelMap.cu:400-449): merge `other` at the relative
+  // pose of the two maps, then release it; an empty map takes other's volume.
+  void Fuse(MapStruct* other) {
+    if (!other || other->Empty()) return;
+    if (Empty()) {
+      Swap(other);
+      return;
+    }
+    DeleteMesh();
+    check(yas_map_fuse_map(h_, other->h_), "yas_map_fuse_map");
+    other->Release();
+  }
+  // Swap(MapStruct*) (VoxelMap.cu:83-102).
+  void Swap(MapStruct* other) {
+    std::swap(h_, other->h_);
+    std::swap(K_, other->K_);
+    point_.swap(other->point_);
+    normal_.swap(other->normal_);
+    std::swap(N, other->N);
+  }
+  // Reserve(hashTableSize, bucketSize, voxelBlockSize) (VoxelMap.cu:485-515).
+  void Reserve(int hashTableSize, int /*bucketSize*/, int voxelBlockSize) {
+    check(yas_map_reserve(h_, hashTableSize, voxelBlockSize), "yas_map_reserve");
+  }
+  // Hibernate() / ReActivate() (VoxelMap.cu:894-954).
+  void Hibernate() {
+    check(yas_map_hibernate(h_), "yas_map_hibernate");
+    mbInHibernation = true;
+  }
+  void ReActivate() {
+    check(yas_map_reactivate(h_), "yas_map_reactivate");
+    mbInHibernation = false;
+  }
+  // SaveToFile / ReadFromFile (VoxelMap.cu:886-892).
+  void SaveToFile(const std::string& path) { check(yas_map_save(h_, path.c_str()), "yas_map_save"); }
+  void ReadFromFile(const std::string& path) {
+    yas_map* m = nullptr;
+    check(yas_map_load(ctx_->get(), path.c_str(), &m), "yas_map_load");
+    Release();
+    h_ = m;
+    yas_map_params p;
+    check(yas_map_get_params(h_, &p, K_), "yas_map_get_params");
+  }
+  void SetActiveFlag(bool flag) { mbActive = flag; }
+  bool isActive() const { return mbActive; }
+  bool mbActive = true;
+  bool mbInHibernation = false;
+
   const float* mplPoint() const { return point_.data(); }
   const float* mplNormal() const { return normal_.data(); }
   int N = 0;  // floats in mplPoint (9 per triangle), as the reference's N

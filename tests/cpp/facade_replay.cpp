@@ -38,6 +38,12 @@ int main() {
   trk.SetTrackingDepth(img);
   yas::Pose DT = trk.GetTransform(yas::Pose(), true);
   map.Fuse(img, DT);
+  // a second submap of the same view, merged in (System::FuseAllMapStruct)
+  yas::MapStruct sub(ctx, K);
+  sub.create(1 << 14, 0, 1 << 12, 0.01f, 0.03f);
+  sub.Fuse(img, DT);
+  map.Fuse(&sub);
+  if (!sub.Empty()) return 3;
   map.GenerateMesh();
   std::printf("tracked=%d t=(%.2e %.2e %.2e) visible=%u triangles=%d\n", (int)trk.IsTrackingGood(), DT.m[3], DT.m[7],
               DT.m[11], map.CheckNumVisibleBlocks(W, H, I), map.N / 9);

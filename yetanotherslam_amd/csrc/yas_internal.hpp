@@ -215,6 +215,15 @@ struct VoxI16 {  // int16 sdf (x32767, truncation) | uint8 weight << 16
     int nw = w + 1 < 255 ? w + 1 : 255;
     return ns | ((uint32_t)nw << 16);
   }
+  __device__ static inline int weight(T v) { return (int)((v >> 16) & 0xFFu); }
+  // Map-to-map merge (VoxelMap.cu:373-375): weighted mean, weight capped at 255.
+  __device__ static inline T merge(T v, float s, int wo, int) {
+    const int w = weight(v);
+    const float old = sdf(v);
+    const uint32_t ns = pack((old * (float)w + s * (float)wo) / (float)(w + wo));
+    const int nw = w + wo < 255 ? w + wo : 255;
+    return ns | ((uint32_t)nw << 16);
+  }
 };
 struct VoxF32 {  // float sdf, float weight
   typedef float2 T;
@@ -226,6 +235,12 @@ struct VoxF32 {  // float sdf, float weight
     if (w == 0.0f) return make_float2(s, 1.0f);
     float ns = (v.x * w + s) / (w + 1.0f);
     return make_float2(ns, fminf((float)maxw, w + 1.0f));
+  }
+  __device__ static inline int weight(T v) { return (int)v.y; }
+  __device__ static inline T merge(T v, float s, int wo, int maxw) {
+    const float w = v.y;
+    const float ns = (v.x * w + s * (float)wo) / (w + (float)wo);
+    return make_float2(ns, fminf((float)maxw, w + (float)wo));
   }
 };
 struct VoxF16 {  // half sdf | uint16 weight << 16
@@ -244,6 +259,14 @@ struct VoxF16 {  // half sdf | uint16 weight << 16
     float old = h2f(v);
     uint32_t ns = f2h((old * (float)w + s) / (float)(w + 1));
     int nw = w + 1 < maxw ? w + 1 : maxw;
+    return ns | ((uint32_t)nw << 16);
+  }
+  __device__ static inline int weight(T v) { return (int)(v >> 16); }
+  __device__ static inline T merge(T v, float s, int wo, int maxw) {
+    const int w = weight(v);
+    const float old = h2f(v);
+    const uint32_t ns = f2h((old * (float)w + s * (float)wo) / (float)(w + wo));
+    const int nw = w + wo < maxw ? w + wo : maxw;
     return ns | ((uint32_t)nw << 16);
   }
 };

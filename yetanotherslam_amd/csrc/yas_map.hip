@@ -583,6 +583,18 @@ int yas_map_get_stats(yas_map* m, yas_map_stats* st) {
 
 int yas_map_voxel_bytes(yas_map* m) { return m ? voxel_bytes(m->P.voxel_format) : 0; }
 
+int yas_map_get_params(yas_map* m, yas_map_params* params, float K[4]) {
+  YAS_REQUIRE(m && params, "null argument");
+  *params = m->P;
+  if (K) {
+    K[0] = m->K.fx;
+    K[1] = m->K.fy;
+    K[2] = m->K.cx;
+    K[3] = m->K.cy;
+  }
+  return YAS_OK;
+}
+
 int yas_map_export(yas_map* m, int32_t* keys, void* voxels, int capacity, int* count) {
   YAS_REQUIRE(m && count, "null argument");
   YAS_REQUIRE_ACTIVE(m);

@@ -419,6 +419,18 @@ static int read_totals(yas_mesher* me) {
   return YAS_OK;
 }
 
+// Host copies of the last indexed extraction (for file writers).
+int yas_mesher_indexed_host(yas_mesher* me, std::vector<float>& verts, std::vector<int32_t>& tris) {
+  YAS_REQUIRE(me && me->indexed, "no indexed extraction to write");
+  const int nv = me->num_vert < me->max_tri * 3 ? me->num_vert : me->max_tri * 3;
+  const int nt = me->num_tri < me->max_tri ? me->num_tri : me->max_tri;
+  verts.resize((size_t)nv * 3);
+  tris.resize((size_t)nt * 3);
+  if (nv > 0) YAS_HIP(hipMemcpy(verts.data(), me->normals, sizeof(float) * 3 * (size_t)nv, hipMemcpyDeviceToHost));
+  if (nt > 0) YAS_HIP(hipMemcpy(tris.data(), me->tris, sizeof(int32_t) * 3 * (size_t)nt, hipMemcpyDeviceToHost));
+  return YAS_OK;
+}
+
 extern "C" {
 
 int yas_mesher_create(yas_ctx* ctx, int max_triangles, yas_mesher** out) {

@@ -66,6 +66,15 @@ SIGNATURES = {
     "yas_mesh_extract": (I, [P, P, PI]),
     "yas_mesher_download": (I, [P, P, P, I]),
     "yas_mesh_extract_indexed": (I, [P, P, PI, PI]),
+    "yas_mesher_write_ply": (I, [P, ctypes.c_char_p]),
+    "yas_map_fuse_map": (I, [P, P]),
+    "yas_map_get_params": (I, [P, P, P]),
+    "yas_map_reserve": (I, [P, I, I]),
+    "yas_map_hibernate": (I, [P]),
+    "yas_map_reactivate": (I, [P]),
+    "yas_map_is_hibernating": (I, [P, PI]),
+    "yas_map_save": (I, [P, ctypes.c_char_p]),
+    "yas_map_load": (I, [P, ctypes.c_char_p, P]),
     "yas_mesher_download_indexed": (I, [P, P, P, I, I]),
     "yas_mesher_indexed_device": (I, [P, P, P]),
     "yas_mesher_blocks": (I, [P, P, P, P, P, I, PI]),

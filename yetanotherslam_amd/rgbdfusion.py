@@ -186,6 +186,40 @@ class MapStruct:
         sdf, wt = decode_voxels(raw[order], self.voxel_format)
         return keys[order], sdf, wt
 
+    def FuseMap(self, src):
+        """Fuse(MapStruct*) (VoxelMap.cu:400-449) without releasing src."""
+        call("yas_map_fuse_map", self.h, src.h)
+
+    def Reserve(self, num_buckets, num_blocks):
+        call("yas_map_reserve", self.h, num_buckets, num_blocks)
+
+    def Hibernate(self):
+        call("yas_map_hibernate", self.h)
+
+    def ReActivate(self):
+        call("yas_map_reactivate", self.h)
+
+    def is_hibernating(self):
+        v = ctypes.c_int()
+        call("yas_map_is_hibernating", self.h, ctypes.byref(v))
+        return bool(v.value)
+
+    def SaveToFile(self, path):
+        call("yas_map_save", self.h, str(path).encode())
+
+    @classmethod
+    def ReadFromFile(cls, path, ctx=None):
+        ctx = ctx or default_context()
+        h = ctypes.c_void_p()
+        call("yas_map_load", ctx.h, str(path).encode(), ctypes.byref(h))
+        m = cls.__new__(cls)
+        m.ctx, m.h = ctx, h
+        m.params = MapParams()
+        m.K = np.zeros(4, np.float32)
+        call("yas_map_get_params", h, ctypes.byref(m.params), _ptr(m.K))
+        m.voxel_format = m.params.voxel_format
+        return m
+
 
 class RayTraceEngine:
     """RayTraceEngine (RGBDFusion/include/RayTraceEngine.h:11-48)."""
@@ -283,6 +317,10 @@ class MeshEngine:
             out["cases"] = cases[:nbk]
         return out
 
+
+    def WritePLY(self, path):
+        """Write the last indexed extraction as binary PLY."""
+        call("yas_mesher_write_ply", self.h, str(path).encode())
 
     def MeshifyIndexed(self, map_struct):
         """Indexed marching cubes: float32 (V, 3) vertices, int32 (T, 3)
