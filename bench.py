@@ -10,7 +10,7 @@ the synthetic room sequence at 640x480, voxel 5 mm, truncation 4 voxels,
 pyramid levels with 10/5/3 Gauss-Newton iterations.  All frames are uploaded
 to HBM before the timed region.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline] [--no-mesh]
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline] [--no-mesh] [--no-batch]
 Prints one JSON line on stdout; progress goes to stderr.
 """
 import argparse
@@ -147,6 +147,73 @@ def mesh_bench(pl, dev_depth, K, nframes, reps=3):
     return out
 
 
+def batch_bench(dev_depth, nseq, n_maps=64, seed=1):
+    """Batching form: 64 per-keyframe submaps of 5-60 frames each (seeded
+    uniform), resolution 320x240 or 640x480, voxel 5 mm, 2^18 buckets and a
+    2^17- (640x480) or 2^16-block (320x240) pool each, f32 storage.  Submap i fuses frames
+    start_i .. start_i + n_i - 1 of the room sequence (at its resolution) at
+    the true poses relative to its first frame.  Timed twice over the same
+    schedule: one yas_map_fuse_u16 call per (submap, frame), and the batched
+    graph (one launch per stage per step, shapes from the autotuner)."""
+    import torch
+    from yetanotherslam_amd import rgbdfusion as rf
+    from yetanotherslam_amd import synth
+
+    rng = np.random.default_rng(seed)
+    nfr = rng.integers(5, 61, n_maps)
+    hi = rng.integers(0, 2, n_maps).astype(bool)
+    start = rng.integers(0, nseq - nfr)
+    p640 = synth.params(width=W, height=H, theta0=THETA0)
+    p320 = synth.params(width=W // 2, height=H // 2, theta0=THETA0)
+    d320 = torch.from_numpy(np.stack([synth.frame(p320, t)[0] for t in range(nseq)]).view(np.int16)).cuda()
+    K = {True: synth.intrinsics(p640).astype(np.float32), False: synth.intrinsics(p320).astype(np.float32)}
+    steps = int(nfr.max())
+    depth, poses = [], []
+    for s in range(steps):
+        drow, prow = [], []
+        for i in range(n_maps):
+            if s >= nfr[i]:
+                drow.append(None)
+                prow.append(np.eye(4))
+                continue
+            t = int(start[i] + s)
+            drow.append(dev_depth[t] if hi[i] else d320[t])
+            prow.append(np.linalg.inv(synth.pose(p640, int(start[i]))) @ synth.pose(p640, t))
+        depth.append(drow)
+        poses.append(prow)
+    maps = [rf.MapStruct(K[bool(hi[i])], 1 << 18, 1 << 17 if hi[i] else 1 << 16, VOXEL, TRUNC) for i in range(n_maps)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for s in range(steps):
+        for i in range(n_maps):
+            if depth[s][i] is not None:
+                maps[i].FuseU16(depth[s][i], poses[s][i])
+    torch.cuda.synchronize()
+    t_plain = time.perf_counter() - t0
+    blocks_plain = [m.stats().num_allocated for m in maps]
+    for m in maps:
+        m.Reset()
+    b = rf.FusionBatch(maps)
+    b.plan(depth, poses)
+    cfg = b.autotune(reset_maps=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    b.run()
+    torch.cuda.synchronize()
+    t_batch = time.perf_counter() - t0
+    blocks_batch = [m.stats().num_allocated for m in maps]
+    total = int(nfr.sum())
+    out = {"submaps": n_maps, "frames": total, "steps": steps, "frames_640x480": int(nfr[hi].sum()),
+           "frames_320x240": int(nfr[~hi].sum()), "blocks": int(sum(blocks_batch)),
+           "plain_frames_per_s": round(total / t_plain, 1), "batched_frames_per_s": round(total / t_batch, 1),
+           "speedup": round(t_plain / t_batch, 2), "same_block_counts": blocks_plain == blocks_batch,
+           "autotuned": cfg}
+    b.close()
+    for m in maps:
+        m.close()
+    return out
+
+
 def copy_probe():
     """A plain device copy of known size (512 MiB read + 512 MiB written),
     used to calibrate rocprofv3's FETCH_SIZE / WRITE_SIZE counters."""
@@ -169,6 +236,7 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--copy-probe", action="store_true", help="run only the counter-calibration copy")
     ap.add_argument("--no-mesh", action="store_true", help="skip the marching-cubes measurement")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched-submap measurement")
     args = ap.parse_args()
     if args.copy_probe:
         copy_probe()
@@ -255,6 +323,10 @@ def main():
         "roofline": dict(rl[dominant], kernel=dominant),
         "roofline_by_kernel": rl,
     }
+    if not args.no_batch:
+        log("batching form: 64 submaps")
+        result["batching"] = batch_bench(dev_depth, n)
+        log("batching:", result["batching"])
     if not args.no_mesh:
         log("marching cubes over the fused volume")
         result["mesh"] = mesh_bench(pl, dev_depth, K, n)

@@ -172,6 +172,43 @@ int yas_map_is_hibernating(yas_map* map, int* hibernating);
 int yas_map_save(yas_map* map, const char* path);
 int yas_map_load(yas_ctx* ctx, const char* path, yas_map** out);
 
+/* ------------------------------------------------------------------ batched fusion
+ * Many maps (per-keyframe submaps, Tracking.cpp:703-716) fused together: the
+ * per-frame fusion of MapStruct::Fuse (VoxelMap.cu:825-884) for every map of
+ * the batch in one launch per stage, captured once in a hipGraph and replayed
+ * step by step.  No counterpart class in the reference. */
+typedef struct yas_batch yas_batch;
+typedef struct {
+  int alloc_tile_w;       /* allocation tile width: 16, 32 or 64 (x 256/w rows) */
+  int visible_groups;     /* visibility workgroups (256 threads) per map */
+  int integrate_threads;  /* integration workgroup size: 64, 128 or 256 */
+  int integrate_groups;   /* integration workgroups per map */
+  float step_ms;          /* autotuner: time of one step with this shape */
+} yas_batch_config;
+/* A batch over num_maps distinct maps of one voxel format and one context. */
+int yas_batch_create(yas_ctx* ctx, int num_maps, yas_map* const* maps, yas_batch** out);
+int yas_batch_destroy(yas_batch* b);
+/* Schedule of `steps` steps.  Entry k = s * num_maps + i: map i fuses the
+ * uint16 depth image depth[k] (device pointer, pitch[k] bytes, W[k] x H[k],
+ * metres = raw / depth_scale) at pose T + 16 k (row-major camera -> map), or
+ * idles at step s when depth[k] is NULL.  Depth images must stay valid while
+ * the plan runs.  Captures the step graph (with a shape tuned earlier for the
+ * same kind of work, if any).  Synchronous. */
+int yas_batch_plan(yas_batch* b, int steps, const uint16_t* const* depth, const size_t* pitch, const int* W,
+                   const int* H, float depth_scale, const double* T);
+/* Enqueue steps [first_step, first_step + count) on the context's stream
+ * (one graph launch per step).  A map reallocated after the plan (reserve,
+ * merge, re-activation) makes this fail: plan again.  Asynchronous. */
+int yas_batch_run(yas_batch* b, int first_step, int count);
+/* Time the step graph on step 0 for candidate launch shapes (coordinate
+ * descent: allocation tile, then visibility, then integration) and keep the
+ * fastest, also for later plans of the same kind of work in this process.
+ * The trial runs fuse step 0 several times: with reset_maps != 0 the maps
+ * are reset afterwards.  Synchronous. */
+int yas_batch_autotune(yas_batch* b, int reset_maps, yas_batch_config* best);
+int yas_batch_set_config(yas_batch* b, const yas_batch_config* config);
+int yas_batch_get_config(yas_batch* b, yas_batch_config* config);
+
 /* ------------------------------------------------------------------ raycast
  * RayTraceEngine (RGBDFusion/include/RayTraceEngine.h:11-48). */
 int yas_raycaster_create(yas_ctx* ctx, int width, int height, const float K[4], yas_raycaster** out);

@@ -146,3 +146,88 @@ def test_ply_file_holds_the_indexed_mesh(tmp_path):
     faces = np.frombuffer(data, np.uint8, nt * 13, head_end + nv * 12).reshape(nt, 13)
     assert np.all(faces[:, 0] == 3)
     np.testing.assert_array_equal(faces[:, 1:].copy().view(np.int32), ix["triangles"])
+
+
+def batch_workload(n_maps=6, steps=4, seed=5):
+    """Maps with uneven frame counts and two resolutions; map i idles at the
+    steps past its frame count and at one step in the middle."""
+    rng = np.random.default_rng(seed)
+    seqs = {}
+    for (W, H) in ((320, 240), (160, 120)):
+        p, K, fr = frames(steps + 3, width=W, height=H)
+        seqs[(W, H)] = (p, K, fr, synth.relative_poses(p, steps + 3))
+    res = [(320, 240) if i % 2 == 0 else (160, 120) for i in range(n_maps)]
+    nfr = rng.integers(2, steps + 1, n_maps)
+    depth, poses = [], []
+    for s in range(steps):
+        drow, prow = [], []
+        for i in range(n_maps):
+            p, K, fr, gt = seqs[res[i]]
+            idle = s >= nfr[i] or (i == 1 and s == 1)
+            drow.append(None if idle else to_dev(fr[s + (i % 3)][0]))
+            prow.append(np.linalg.inv(gt[i % 3]) @ gt[s + (i % 3)])
+        depth.append(drow)
+        poses.append(prow)
+    return seqs, res, depth, poses
+
+
+def new_maps(seqs, res, fmt=rf.VOXEL_F32_F32):
+    return [rf.MapStruct(seqs[r][1].astype(np.float32), 1 << 14, 1 << 12, VOX, MU, voxel_format=fmt) for r in res]
+
+
+@pytest.mark.parametrize("fmt", [rf.VOXEL_F32_F32, rf.VOXEL_F16_U16], ids=["f32", "f16u16"])
+def test_batched_fusion_equals_one_map_at_a_time(fmt):
+    """The graph-replayed batch (maps of two resolutions, uneven frame counts,
+    idle steps) leaves every map bit-identical to fusing its frames one call
+    at a time; so do a tuned shape after the autotuner's reset, and a plan run
+    in two parts."""
+    seqs, res, depth, poses = batch_workload()
+    ref = new_maps(seqs, res, fmt)
+    for s in range(len(depth)):
+        for i, m in enumerate(ref):
+            if depth[s][i] is not None:
+                m.FuseU16(depth[s][i], poses[s][i])
+    want = [m.export_sorted() for m in ref]
+
+    def check(maps, what):
+        for i, (m, w) in enumerate(zip(maps, want)):
+            k, sd, wt = m.export_sorted()
+            np.testing.assert_array_equal(k, w[0], err_msg="%s map %d" % (what, i))
+            assert_bits_equal(sd, w[1], "%s map %d sdf" % (what, i))
+            assert_bits_equal(wt, w[2], "%s map %d weight" % (what, i))
+
+    maps = new_maps(seqs, res, fmt)
+    b = rf.FusionBatch(maps)
+    b.plan(depth, poses)
+    b.run()
+    check(maps, "default shape")
+
+    maps2 = new_maps(seqs, res, fmt)
+    b2 = rf.FusionBatch(maps2)
+    b2.plan(depth, poses)
+    cfg = b2.autotune(reset_maps=True)
+    assert cfg["step_ms"] > 0
+    assert all(m.stats().num_allocated == 0 for m in maps2)
+    b2.run(0, 2)
+    b2.run(2, len(depth) - 2)
+    check(maps2, "tuned shape %s" % cfg)
+
+    for shape in ({"alloc_tile_w": 64, "visible_groups": 3, "integrate_threads": 64, "integrate_groups": 7},):
+        maps3 = new_maps(seqs, res, fmt)
+        b3 = rf.FusionBatch(maps3)
+        b3.plan(depth, poses)
+        b3.set_config(**shape)
+        b3.run()
+        check(maps3, "shape %s" % shape)
+
+
+def test_batch_refuses_a_map_reallocated_after_planning():
+    seqs, res, depth, poses = batch_workload(n_maps=2, steps=2)
+    maps = new_maps(seqs, res)
+    b = rf.FusionBatch(maps)
+    b.plan(depth, poses)
+    maps[0].Reserve(1 << 15, 1 << 13)
+    with pytest.raises(native.YasError):
+        b.run()
+    b.plan(depth, poses)
+    b.run()

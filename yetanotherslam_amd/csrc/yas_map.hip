@@ -24,6 +24,7 @@
 #include <vector>
 
 #include "yas_internal.hpp"
+#include "yas_fusion_dev.hpp"
 
 using namespace yas;
 
@@ -134,194 +135,33 @@ __global__ void __launch_bounds__(256) k_depth_u16(const uint16_t* src, size_t s
   ((float*)((char*)dst + y * dp))[x] = (float)d * scale;
 }
 
-// Pixel ray -> block (VoxelStructUtils.h:97-110, 28-49).
-__device__ inline void pixel_block(int x, int y, float z, const Intr& K, const PoseF& T, float vs, int& bx, int& by,
-                                   int& bz) {
-  float px = K.invfx * ((float)x - K.cx) * z;
-  float py = K.invfy * ((float)y - K.cy) * z;
-  float wx, wy, wz;
-  xform(T, px, py, z, wx, wy, wz);
-  bx = vox_to_block(floor_sat(wx / vs));
-  by = vox_to_block(floor_sat(wy / vs));
-  bz = vox_to_block(floor_sat(wz / vs));
-}
-
-// K3 (VoxelMap.cu:517-652): every block on the 3D Bresenham walk between the
-// blocks of depth-mu/2 and depth+mu/2 along each pixel ray.  16x16 pixels per
-// workgroup, so a wavefront covers a 16x4 patch whose rays mostly share
-// blocks.
+// K3 (VoxelMap.cu:517-652), 16x16 pixels per workgroup: a wavefront covers
+// a 16x4 patch whose rays mostly share blocks.
 __global__ void __launch_bounds__(256) k_alloc(MapView m, const float* depth, size_t pitch, int W, int H, Intr K,
                                                const PosePair* pose, const int* gate) {
   if (gate && *gate == 0) return;
   __shared__ int walked;
   if (threadIdx.x == 0) walked = 0;
   __syncthreads();
-  const int x = blockIdx.x * 16 + (threadIdx.x & 15);
-  const int y = blockIdx.y * 16 + (threadIdx.x >> 4);
-  const PoseF T = pose->T;
-  const float vs = m.voxel_size;
-  int px = 0, py = 0, pz = 0;          // current block
-  int dax = 0, day = 0, daz = 0;       // unit step along the dominant axis
-  int d1x = 0, d1y = 0, d1z = 0;       // unit step along the first other axis
-  int d2x = 0, d2y = 0, d2z = 0;       // unit step along the second other axis
-  int Ea = 0, E1 = 0, E2 = 0, err1 = 0, err2 = 0;
-  int remaining = 0;
-  if (x < W && y < H) {
-    float d = ((const float*)((const char*)depth + (size_t)y * pitch))[x];
-    const float half = m.trunc * 0.5f;
-    if (!(d != d || d < m.depth_min || d > m.depth_max)) {
-      float dn = fmaxf(m.depth_min, d - half);
-      float df = fminf(m.depth_max, d + half);
-      if (dn < df) {
-        int ex, ey, ez;
-        pixel_block(x, y, dn, K, T, vs, px, py, pz);
-        pixel_block(x, y, df, K, T, vs, ex, ey, ez);
-        int dx = ex - px, dy = ey - py, dz = ez - pz;
-        int ix = dx < 0 ? -1 : 1, iy = dy < 0 ? -1 : 1, iz = dz < 0 ? -1 : 1;
-        int ax = abs(dx), ay = abs(dy), az = abs(dz);
-        if (ax >= ay && ax >= az) {
-          dax = ix; d1y = iy; d2z = iz;
-          Ea = ax << 1; E1 = ay << 1; E2 = az << 1;
-          remaining = ax + 1;
-        } else if (ay >= ax && ay >= az) {
-          day = iy; d1x = ix; d2z = iz;
-          Ea = ay << 1; E1 = ax << 1; E2 = az << 1;
-          remaining = ay + 1;
-        } else {
-          // the reference's z branch walks y first, then x (VoxelMap.cu:628-647)
-          daz = iz; d1y = iy; d2x = ix;
-          Ea = az << 1; E1 = ay << 1; E2 = ax << 1;
-          remaining = az + 1;
-        }
-        err1 = E1 - 1;
-        err2 = E2 - 1;
-      }
-    }
-  }
-  for (;;) {
-    bool has = remaining > 0;
-    unsigned long long act = __ballot(has);
-    if (act == 0ull) break;
-    bool ok = has && key_in_range(px, py, pz);
-    if (has && !ok) atomicOr(&m.ctr[kCtrFlags], kFlagKeyRange);
-    uint64_t key = ok ? pack_key(px, py, pz) : 0ull;
-    int home = ok ? ref_hash(px, py, pz, m.num_buckets) : 0;
-    if (lane_id() == __ffsll(act) - 1) atomicAdd(&walked, __popcll(act));
-    wave_insert(m, key, ok, home);
-    if (has) {
-      --remaining;
-      if (remaining > 0) {
-        if (err1 > 0) { px += d1x; py += d1y; pz += d1z; err1 -= Ea; }
-        if (err2 > 0) { px += d2x; py += d2y; pz += d2z; err2 -= Ea; }
-        err1 += E1;
-        err2 += E2;
-        px += dax; py += day; pz += daz;
-      }
-    }
-  }
+  alloc_walk(m, DepthF32{depth, pitch}, W, H, K, pose->T, blockIdx.x * 16 + (threadIdx.x & 15),
+             blockIdx.y * 16 + (threadIdx.x >> 4), &walked);
   __syncthreads();
   if (threadIdx.x == 0 && walked) atomicAdd(&m.ctr[kCtrWalk], walked);
 }
 
-// VoxelStructUtils.h:112-147.
-__device__ inline bool block_visible(int bx, int by, int bz, const PoseF& Ti, const Intr& K, int W, int H,
-                                     float vs, float dmin, float dmax) {
-  const float scale = vs * 8.0f;
-#pragma unroll
-  for (int c = 0; c < 8; ++c) {
-    float x = (float)(bx + (c & 1)) * scale;
-    float y = (float)(by + ((c >> 1) & 1)) * scale;
-    float z = (float)(bz + ((c >> 2) & 1)) * scale;
-    float qx, qy, qz;
-    xform(Ti, x, y, z, qx, qy, qz);
-    float u = K.fx * qx / qz + K.cx;
-    float v = K.fy * qy / qz + K.cy;
-    if (!(u < 0.0f || v < 0.0f || u > (float)(W - 1) || v > (float)(H - 1) || qz < dmin || qz > dmax)) return true;
-  }
-  return false;
-}
-
-// K4 (VoxelMap.cu:654-717): compact the visible blocks of the pool into
-// `visible` with one atomic per wavefront.
+// K4 (VoxelMap.cu:654-717).
 __global__ void __launch_bounds__(256) k_visible(MapView m, const PosePair* pose, Intr K, int W, int H,
                                                  const int* gate) {
   if (gate && *gate == 0) return;
-  const int n0 = m.ctr[kCtrAlloc];
-  const int n = n0 < m.num_blocks ? n0 : m.num_blocks;
-  const PoseF Ti = pose->Tinv;
-  for (int base = blockIdx.x * blockDim.x; base < n; base += gridDim.x * blockDim.x) {
-    int i = base + threadIdx.x;
-    bool vis = false;
-    if (i < n) {
-      int bx, by, bz;
-      unpack_key(m.block_keys[i], bx, by, bz);
-      vis = block_visible(bx, by, bz, Ti, K, W, H, m.voxel_size, m.depth_min, m.depth_max);
-    }
-    unsigned long long mask = __ballot(vis);
-    if (mask) {
-      int first = __ffsll(mask) - 1;
-      int b = 0;
-      if (lane_id() == first) b = atomicAdd(&m.ctr[kCtrVisible], __popcll(mask));
-      b = __shfl(b, first);
-      const int slot = b + __popcll(mask & lane_mask_lt());
-      if (vis && slot < m.num_blocks) m.visible[slot] = i;
-    }
-  }
+  visible_scan(m, pose->Tinv, K, W, H, blockIdx.x * blockDim.x, gridDim.x * blockDim.x);
 }
 
-// K5 (VoxelMap.cu:719-786): one wavefront per visible block, lane = (x, y),
-// looping over the 8 z-slices; each slice is 64 consecutive voxels.  After
-// the update the wavefront re-derives the block's summary flags from the
-// final voxel values and stores them in the block's hash slot.
+// K5 (VoxelMap.cu:719-786).
 template <class VT>
 __global__ void __launch_bounds__(256) k_integrate(MapView m, const PosePair* pose, Intr K, const float* depth,
                                                    size_t pitch, int W, int H) {
-  const int nv = m.ctr[kCtrVisible];
-  const int nvis = nv < m.num_blocks ? nv : m.num_blocks;
-  const int lane = threadIdx.x & 63;
-  const int lx = lane & 7, ly = lane >> 3;
-  const int nwaves = gridDim.x * (blockDim.x >> 6);
-  const PoseF Ti = pose->Tinv;
-  const float vs = m.voxel_size, mu = m.trunc;
-  typename VT::T* vox = (typename VT::T*)m.voxels;
-  for (int w = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); w < nvis; w += nwaves) {
-    const int blk = m.visible[w];
-    int bx, by, bz;
-    unpack_key(m.block_keys[blk], bx, by, bz);
-    typename VT::T* bv = vox + (size_t)blk * 512;
-    const float wx = (float)(bx * 8 + lx) * vs;
-    const float wy = (float)(by * 8 + ly) * vs;
-    bool unobserved = true, free_space = true;
-#pragma unroll 2
-    for (int lz = 0; lz < 8; ++lz) {
-      const int li = lz * 64 + lane;
-      typename VT::T cur = bv[li];
-      const float wz = (float)(bz * 8 + lz) * vs;
-      float px, py, pz;
-      xform(Ti, wx, wy, wz, px, py, pz);
-      int u = floor_sat(K.fx * px / pz + K.cx + 0.5f);
-      int v = floor_sat(K.fy * py / pz + K.cy + 0.5f);
-      if (!(u < 0 || v < 0 || u > W - 1 || v > H - 1)) {
-        float d = ((const float*)((const char*)depth + (size_t)v * pitch))[u];
-        if (!(d != d || d > m.depth_max || d < m.depth_min)) {
-          float sdf = d - pz;
-          if (!(sdf < -mu)) {
-            sdf = fminf(1.0f, sdf / mu);
-            cur = VT::update(cur, sdf, m.max_weight);
-            bv[li] = cur;
-          }
-        }
-      }
-      const bool obs = VT::observed(cur);
-      unobserved = unobserved && !obs;
-      free_space = free_space && obs && VT::sdf(cur) == 1.0f;
-    }
-    const unsigned long long un = __ballot(unobserved), fr = __ballot(free_space);
-    if (lane == 0) {
-      const int f = (un == ~0ull ? kBlockUnobserved : 0) | (fr == ~0ull ? kBlockFree : 0);
-      m.slots[m.block_slot[blk]].flags = f;
-    }
-  }
+  integrate_waves<VT>(m, pose->Tinv, K, DepthF32{depth, pitch}, W, H,
+                      blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6), gridDim.x * (blockDim.x >> 6));
 }
 
 // ------------------------------------------------------------------ host side

@@ -23,6 +23,14 @@ class MapParams(ctypes.Structure):
                 ("voxel_format", I), ("max_weight", I), ("depth_min", F), ("depth_max", F)]
 
 
+class BatchConfig(ctypes.Structure):
+    _fields_ = [("alloc_tile_w", I), ("visible_groups", I), ("integrate_threads", I), ("integrate_groups", I),
+                ("step_ms", F)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 class MapStats(ctypes.Structure):
     _fields_ = [("num_allocated", I), ("num_visible", I), ("flags", I), ("reserved", I)]
 
@@ -68,6 +76,13 @@ SIGNATURES = {
     "yas_mesh_extract_indexed": (I, [P, P, PI, PI]),
     "yas_mesher_write_ply": (I, [P, ctypes.c_char_p]),
     "yas_map_fuse_map": (I, [P, P]),
+    "yas_batch_create": (I, [P, I, P, P]),
+    "yas_batch_destroy": (I, [P]),
+    "yas_batch_plan": (I, [P, I, P, P, P, P, ctypes.c_float, P]),
+    "yas_batch_run": (I, [P, I, I]),
+    "yas_batch_autotune": (I, [P, I, P]),
+    "yas_batch_set_config": (I, [P, P]),
+    "yas_batch_get_config": (I, [P, P]),
     "yas_map_get_params": (I, [P, P, P]),
     "yas_map_reserve": (I, [P, I, I]),
     "yas_map_hibernate": (I, [P]),

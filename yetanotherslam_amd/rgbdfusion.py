@@ -16,7 +16,7 @@ import numpy as np
 import torch
 
 from . import native
-from .native import MapParams, MapStats, TrackerParams, TrackStats, call
+from .native import BatchConfig, MapParams, MapStats, TrackerParams, TrackStats, call
 
 VOXEL_I16_U8, VOXEL_F32_F32, VOXEL_F16_U16 = 0, 1, 2
 TRACK_RGB, TRACK_DEPTH, TRACK_RGBD = 0, 1, 2
@@ -219,6 +219,71 @@ class MapStruct:
         call("yas_map_get_params", h, ctypes.byref(m.params), _ptr(m.K))
         m.voxel_format = m.params.voxel_format
         return m
+
+
+class FusionBatch:
+    """Many maps fused together, one launch per stage (yas_batch_*)."""
+
+    def __init__(self, maps, ctx=None):
+        self.ctx = ctx or default_context()
+        self.maps = list(maps)
+        arr = (ctypes.c_void_p * len(self.maps))(*[m.h.value for m in self.maps])
+        h = ctypes.c_void_p()
+        call("yas_batch_create", self.ctx.h, len(self.maps), arr, ctypes.byref(h))
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            native.lib().yas_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def plan(self, depth, poses, depth_scale=1000.0):
+        """depth[s][i]: uint16/int16 device tensor (H, W) or None (map i idles
+        at step s); poses[s][i]: 4x4 camera -> map."""
+        S, n = len(depth), len(self.maps)
+        ptrs = (ctypes.c_void_p * (S * n))()
+        pitch = (ctypes.c_size_t * (S * n))()
+        W = (ctypes.c_int * (S * n))()
+        H = (ctypes.c_int * (S * n))()
+        T = np.zeros((S * n, 16))
+        for s in range(S):
+            for i in range(n):
+                k = s * n + i
+                d = depth[s][i]
+                if d is None:
+                    continue
+                assert d.dtype in (torch.int16, torch.uint16) and d.dim() == 2 and d.is_contiguous()
+                ptrs[k], pitch[k], W[k], H[k] = d.data_ptr(), d.stride(0) * 2, d.shape[1], d.shape[0]
+                T[k] = np.asarray(poses[s][i], np.float64).reshape(16)
+        self._keep = depth
+        call("yas_batch_plan", self.h, S, ptrs, pitch, W, H, depth_scale, _ptr(T))
+        self.steps = S
+
+    def autotune(self, reset_maps=True):
+        c = BatchConfig()
+        call("yas_batch_autotune", self.h, 1 if reset_maps else 0, ctypes.byref(c))
+        return c.as_dict()
+
+    def config(self):
+        c = BatchConfig()
+        call("yas_batch_get_config", self.h, ctypes.byref(c))
+        return c.as_dict()
+
+    def set_config(self, **kw):
+        c = BatchConfig()
+        call("yas_batch_get_config", self.h, ctypes.byref(c))
+        for k, v in kw.items():
+            setattr(c, k, v)
+        call("yas_batch_set_config", self.h, ctypes.byref(c))
+
+    def run(self, first=0, count=None):
+        call("yas_batch_run", self.h, first, self.steps - first if count is None else count)
 
 
 class RayTraceEngine:
