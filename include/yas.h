@@ -209,6 +209,56 @@ int yas_batch_autotune(yas_batch* b, int reset_maps, yas_batch_config* best);
 int yas_batch_set_config(yas_batch* b, const yas_batch_config* config);
 int yas_batch_get_config(yas_batch* b, yas_batch_config* config);
 
+/* ------------------------------------------------------------------ serving
+ * Many RGB-D streams tracked and fused concurrently (no counterpart in the
+ * reference, which runs one Tracking thread per System).  Every frame is the
+ * dense step of yas_pipeline_step for its stream.  Continuous batching: the
+ * oldest waiting frame of each stream joins the next batch; a batch is one
+ * replay of a captured graph over all its streams, and batches queue back to
+ * back on the server's own HIP stream.  A stream's results are identical
+ * alone and in any mix.  Streams are admitted while their device buffers fit
+ * the HBM budget. */
+typedef struct yas_server yas_server;
+typedef struct {
+  int stream;          /* stream id */
+  int tracked;         /* 1 if the frame was tracked (or was the stream's first) */
+  long long request;   /* id returned by yas_server_submit */
+  double pose[16];     /* row-major camera -> map pose of the frame */
+} yas_result;
+typedef struct {
+  int streams;
+  long long frames_done;
+  long long batches_done;
+  size_t bytes_in_use;
+  size_t bytes_budget;
+} yas_server_stats;
+/* All streams share the tracker parameters and voxel format; images are at
+ * most max_width x max_height. */
+int yas_server_create(yas_ctx* ctx, int max_width, int max_height, int max_streams, const yas_tracker_params* tp,
+                      int voxel_format, size_t hbm_budget_bytes, yas_server** out);
+int yas_server_destroy(yas_server* sv);
+/* Admit a stream (its volume, ray caster, tracker and pose chain are
+ * allocated now); YAS_E_CAPACITY when its buffers would exceed the budget or
+ * all stream slots are taken. Synchronous. */
+int yas_server_add_stream(yas_server* sv, int W, int H, const float K[4], const yas_map_params* mp, float depth_scale,
+                          int gray_mode, int* stream_id);
+/* Release a stream once its frames in flight are done (none may be waiting). */
+int yas_server_remove_stream(yas_server* sv, int stream_id);
+/* The stream's volume (for meshing or export between batches). */
+int yas_server_stream_map(yas_server* sv, int stream_id, yas_map** map);
+/* Queue a frame: uint16 depth and RGB8 colour in device memory, ready when
+ * submitted and valid until its result is polled. */
+int yas_server_submit(yas_server* sv, int stream_id, const uint16_t* depth, size_t dpitch, const uint8_t* rgb,
+                      size_t rpitch, long long* request_id);
+/* Launch batches from the waiting frames while fewer than three batches are
+ * queued; collect finished ones.  Never blocks on the device. */
+int yas_server_pump(yas_server* sv, int* batches_launched);
+/* Results of finished frames, oldest first. Never blocks. */
+int yas_server_poll(yas_server* sv, yas_result* out, int capacity, int* count);
+/* Pump until every submitted frame is finished. */
+int yas_server_drain(yas_server* sv);
+int yas_server_get_stats(yas_server* sv, yas_server_stats* stats);
+
 /* ------------------------------------------------------------------ raycast
  * RayTraceEngine (RGBDFusion/include/RayTraceEngine.h:11-48). */
 int yas_raycaster_create(yas_ctx* ctx, int width, int height, const float K[4], yas_raycaster** out);

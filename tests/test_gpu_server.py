@@ -1,0 +1,141 @@
+"""GPU tests of the serving engine: a stream's results are bit-identical
+alone, in a mix of streams of other resolutions, and to the single-stream
+pipeline; streams are admitted by the HBM budget; continuous batching keeps
+each stream's frames in order."""
+import math
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import pyoracle as po
+from yetanotherslam_amd import native
+from yetanotherslam_amd import rgbdfusion as rf
+from yetanotherslam_amd import synth
+
+from helpers import assert_bits_equal, to_dev
+
+pytestmark = pytest.mark.gpu
+
+VOX, MU = 0.008, 0.024
+TP = dict(levels=3, iters=(10, 5, 3), mode=rf.TRACK_RGBD, weights=(0.1, 0.1, 1.0, 1.0))
+
+
+def stream_frames(W, H, theta0, n):
+    p = synth.params(width=W, height=H, theta0=theta0)
+    fr = [synth.frame(p, t) for t in range(n)]
+    return p, synth.intrinsics(p), [(to_dev(d), to_dev(c)) for d, c, _ in fr]
+
+
+def map_params():
+    return rf.map_params(1 << 16, 1 << 14, VOX, MU, rf.VOXEL_F32_F32)
+
+
+def tracker_params():
+    return rf.tracker_params(TP["levels"], TP["iters"], TP["mode"], TP["weights"])
+
+
+def run_single(p, K, frames):
+    pl = rf.DensePipeline(p.width, p.height, K, map_params(), tracker_params())
+    for d, c in frames:
+        pl.step(d, c)
+    torch.cuda.synchronize()
+    poses = [pl.pose(t) for t in range(len(frames))]
+    return poses, pl.export_map_sorted(rf.VOXEL_F32_F32)
+
+
+def run_server(streams, n, interleave=True):
+    """streams: list of (p, K, frames).  Submits frame t of every stream
+    before frame t+1 of any, pumping as it goes."""
+    sv = rf.Server(320, 240, 8, tracker_params())
+    ids = [sv.add_stream(p.width, p.height, K, map_params()) for p, K, _ in streams]
+    torch.cuda.synchronize()
+    results = {i: [] for i in ids}
+    for t in range(n):
+        for sid, (_, _, fr) in zip(ids, streams):
+            sv.submit(sid, *fr[t])
+        if interleave:
+            sv.pump()
+            for r in sv.poll():
+                results[r["stream"]].append(r)
+    sv.drain()
+    for r in sv.poll():
+        results[r["stream"]].append(r)
+    maps = [sv.stream_map(sid).export_sorted() for sid in ids]
+    return sv, ids, results, maps
+
+
+def test_stream_is_identical_alone_in_a_mix_and_in_the_pipeline():
+    n = 5
+    a = stream_frames(320, 240, math.pi / 4, n)
+    b = stream_frames(160, 120, math.pi / 3, n)
+    c = stream_frames(320, 240, 1.2, n)
+    single_poses, single_map = run_single(*a)
+    _, _, alone, alone_maps = run_server([a], n)
+    sv, ids, mix, mix_maps = run_server([b, a, c], n)
+    ra_alone = alone[0]
+    ra_mix = mix[ids[1]]
+    assert len(ra_alone) == len(ra_mix) == n
+    # requests of a stream come back in order
+    assert [r["request"] for r in ra_mix] == sorted(r["request"] for r in ra_mix)
+    for t in range(n):
+        Ts, ok = single_poses[t]
+        assert ra_alone[t]["tracked"] == ok and ra_mix[t]["tracked"] == ok
+        np.testing.assert_array_equal(ra_alone[t]["pose"], Ts)
+        np.testing.assert_array_equal(ra_mix[t]["pose"], Ts)
+    for arr, ref in zip(mix_maps[1], single_map):
+        np.testing.assert_array_equal(arr, ref)
+    for arr, ref in zip(alone_maps[0], single_map):
+        np.testing.assert_array_equal(arr, ref)
+    # the other streams of the mix match their own single pipelines too
+    for sid, s in ((ids[0], b), (ids[2], c)):
+        poses, m = run_single(s[0], s[1], s[2])
+        for t in range(n):
+            np.testing.assert_array_equal(mix[sid][t]["pose"], poses[t][0])
+    st = sv.stats()
+    assert st["frames_done"] == 3 * n and st["batches_done"] >= n
+
+
+def test_admission_by_hbm_budget():
+    p, K, _ = stream_frames(160, 120, 0.5, 1)
+    probe = rf.Server(160, 120, 4, tracker_params(), hbm_budget=1 << 40)
+    probe.add_stream(160, 120, K, map_params())
+    one = probe.stats()["bytes_in_use"]
+    probe.close()
+    sv = rf.Server(160, 120, 4, tracker_params(), hbm_budget=int(2.5 * one))
+    s0 = sv.add_stream(160, 120, K, map_params())
+    s1 = sv.add_stream(160, 120, K, map_params())
+    with pytest.raises(native.YasError) as e:
+        sv.add_stream(160, 120, K, map_params())
+    assert e.value.code == 6  # YAS_E_CAPACITY
+    sv.remove_stream(s0)
+    s2 = sv.add_stream(160, 120, K, map_params())
+    assert sv.stats()["streams"] == 2 and s2 != s1
+
+
+def test_frames_arriving_during_a_batch_join_later_batches():
+    """Continuous batching: a stream that submits several frames at once
+    gets them in consecutive batches, in order; a stream that joins later is
+    batched with the others' later frames; everything equals running each
+    stream alone."""
+    n = 4
+    a = stream_frames(160, 120, 0.7, n)
+    b = stream_frames(160, 120, 1.9, n)
+    sv = rf.Server(160, 120, 4, tracker_params())
+    ia = sv.add_stream(160, 120, a[1], map_params())
+    torch.cuda.synchronize()
+    for t in range(n):
+        sv.submit(ia, *a[2][t])
+    sv.pump()
+    ib = sv.add_stream(160, 120, b[1], map_params())
+    torch.cuda.synchronize()
+    for t in range(n):
+        sv.submit(ib, *b[2][t])
+    sv.drain()
+    res = sv.poll()
+    by = {ia: [r for r in res if r["stream"] == ia], ib: [r for r in res if r["stream"] == ib]}
+    for sid, s in ((ia, a), (ib, b)):
+        assert [r["request"] for r in by[sid]] == sorted(r["request"] for r in by[sid]) and len(by[sid]) == n
+        poses, _ = run_single(s[0], s[1], s[2])
+        for t in range(n):
+            np.testing.assert_array_equal(by[sid][t]["pose"], poses[t][0])

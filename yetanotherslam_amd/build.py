@@ -18,7 +18,7 @@ CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(REPO, "_build", "obj")
 
 HIP_SOURCES = ["yas_map.hip", "yas_mapops.hip", "yas_batch.hip", "yas_raycast.hip", "yas_tracker.hip",
-               "yas_mesh.hip", "yas_pipeline.hip"]
+               "yas_mesh.hip", "yas_pipeline.hip", "yas_server.hip"]
 
 # -ffp-contract=off: one rounding per written operation (bit-exact parity
 # with the scalar oracle); packed-fp32 instructions are disabled.

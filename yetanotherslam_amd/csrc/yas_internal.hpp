@@ -324,6 +324,29 @@ __device__ inline int find_block_xyz(const MapView& m, int bx, int by, int bz, i
   return find_block(m, pack_key(bx, by, bz), ref_hash(bx, by, bz, m.num_buckets), flags);
 }
 
+// ------------------------------------------------------------------ frame outcome
+// Tracking.cpp:178-187: reject |t| > 0.1 m (or a NaN update), otherwise
+// T_last <- T_last * DT^-1.  Records the frame's pose and outcome and sets
+// the gate that lets the fusion kernels run.  One thread.
+__device__ inline void post_track_at(const double* DT, const int* good, double* T_last, double* pose_out,
+                                     int* tracked, int* gate) {
+  double tn = sqrt(DT[3] * DT[3] + DT[7] * DT[7] + DT[11] * DT[11]);
+  int ok = (*good != 0) && !(tn > 0.1);
+  if (ok) {
+    double Di[16];
+    inv44(DT, Di);
+    mul44(T_last, Di, T_last);
+  }
+  for (int i = 0; i < 16; ++i) pose_out[i] = T_last[i];
+  *tracked = ok;
+  *gate = ok;
+}
+// StereoInitialization (Tracking.cpp:139-155): the first frame keeps T_last.
+__device__ inline void record_first_at(const double* T_last, double* pose_out, int* tracked) {
+  for (int i = 0; i < 16; ++i) pose_out[i] = T_last[i];
+  *tracked = 1;
+}
+
 // ------------------------------------------------------------------ XCD mapping
 // Workgroups are dealt to the 8 XCDs round-robin (workgroup b runs on XCD
 // b % 8).  This bijection from launch order to tile index gives XCD j one

@@ -4,10 +4,12 @@
 // pose, track against the ray-cast model, gate |t| > 0.1 m, compose
 // T_cur = T_last * DT^-1 and fuse at T_cur.  A frame enqueues kernels only;
 // the host never waits on the device between frames.
+#include <algorithm>
 #include <cstring>
 #include <vector>
 
 #include "yas_internal.hpp"
+#include "yas_server_dev.hpp"
 
 using namespace yas;
 
@@ -31,23 +33,11 @@ int yas_rgb_to_gray_async(yas_ctx* ctx, const uint8_t* rgb, size_t sp, float* ds
 
 namespace {
 
-// Tracking.cpp:178-187: reject |t| > 0.1 m (or a NaN update), otherwise
-// T_last <- T_last * DT^-1.  Records the frame's pose and outcome and sets
-// the gate that lets the fusion kernels run.
 __global__ void k_post_track(const double* DT, const int* good, double* T_last, double* pose_out, int* tracked,
                              int* gate, const int* ray_visible, int* ray_visible_out) {
   if (threadIdx.x != 0) return;
   *ray_visible_out = *ray_visible;
-  double tn = sqrt(DT[3] * DT[3] + DT[7] * DT[7] + DT[11] * DT[11]);
-  int ok = (*good != 0) && !(tn > 0.1);
-  if (ok) {
-    double Di[16];
-    inv44(DT, Di);
-    mul44(T_last, Di, T_last);
-  }
-  for (int i = 0; i < 16; ++i) pose_out[i] = T_last[i];
-  *tracked = ok;
-  *gate = ok;
+  post_track_at(DT, good, T_last, pose_out, tracked, gate);
 }
 
 // Per-frame counters of the map: allocated, visible, flags, walk steps.
@@ -57,8 +47,7 @@ __global__ void k_frame_stats(const int* ctr, int* out) {
 
 __global__ void k_record_first(const double* T_last, double* pose_out, int* tracked) {
   if (threadIdx.x != 0) return;
-  for (int i = 0; i < 16; ++i) pose_out[i] = T_last[i];
-  *tracked = 1;
+  record_first_at(T_last, pose_out, tracked);
 }
 
 __global__ void k_set_identity(double* T) {
@@ -121,6 +110,34 @@ static int grow(yas_pipeline* p) {
   p->cap = ncap;
   return YAS_OK;
 }
+
+int yas_tracker_fill(yas_tracker* t, StreamDev& d);
+int yas_raycaster_fill(yas_raycaster* rc, const MapView& v, StreamDev& d);
+const PosePair* yas_map_pose_dev(yas_map* map);
+
+// The device description of this pipeline's stream (serving engine).
+int yas_pipeline_fill(yas_pipeline* p, StreamDev& d) {
+  std::memset(&d, 0, sizeof(d));
+  d.W = p->W;
+  d.H = p->H;
+  d.depth_scale_inv = (float)(1.0 / (double)p->depth_scale);
+  d.gray_mode = p->gray_mode;
+  d.K = p->map->K;
+  d.m = yas_map_view(p->map);
+  d.pose = (PosePair*)yas_map_pose_dev(p->map);
+  long nb = p->map->P.num_blocks;
+  d.vis_groups = (int)std::min<long>(std::max<long>((nb + 255) / 256, 1), 4096);
+  d.int_groups = (int)std::min<long>(std::max<long>((nb + 3) / 4, 1), 2048);
+  d.depth = p->depth;
+  d.gray = p->gray;
+  d.T_last = p->T_last;
+  d.gate = p->gate;
+  int rc = yas_raycaster_fill(p->rc, d.m, d);
+  if (!rc) rc = yas_tracker_fill(p->trk, d);
+  return rc;
+}
+yas_map* yas_pipeline_map_of(yas_pipeline* p) { return p->map; }
+yas_tracker* yas_pipeline_tracker_of(yas_pipeline* p) { return p->trk; }
 
 extern "C" {
 

@@ -1,0 +1,648 @@
+// Serving engine: many RGB-D streams (sensors) tracked and fused
+// concurrently, each with its own volume, tracker and pose resident in HBM.
+// Each frame is the dense half of Tracking::TrackRGBD
+// (FullSystem/src/Tracking.cpp:139-197) for its stream, as in yas_pipeline.
+//
+// Continuous batching: submitted frames wait in per-stream queues; each pump
+// takes the oldest waiting frame of every stream (one per stream, since a
+// stream's frames depend on each other through its pose and reference
+// images) and launches them as one batch.  A batch is one replay of a
+// captured hipGraph of ~90 kernels whose grids have one z-slice per batch
+// slot; every kernel reads its slot's stream from a small table, and every
+// stream keeps the workgroup counts of the single-stream pipeline, so a
+// stream's results are bit-identical alone and in any mix.  Batches queue
+// back to back on one HIP stream (in-order execution carries each stream's
+// dependencies), so frames that arrive while a batch runs are admitted to
+// the next one.  Streams are admitted while the HBM budget holds their
+// buffers, and release them on removal.
+//
+// Per stream the reference and current image pyramids keep fixed buffers:
+// after tracking, the current frame's intensity and inverse-depth pyramids
+// are copied into the reference ones (the single pipeline swaps pointers;
+// the values are the same).
+#include <algorithm>
+#include <chrono>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <vector>
+
+#include "yas_fusion_dev.hpp"
+#include "yas_internal.hpp"
+#include "yas_raycast_dev.hpp"
+#include "yas_server_dev.hpp"
+#include "yas_tracker_dev.hpp"
+
+using namespace yas;
+
+int yas_pipeline_fill(yas_pipeline* p, StreamDev& d);
+yas_map* yas_pipeline_map_of(yas_pipeline* p);
+
+namespace {
+
+constexpr int kMaxSlots = 64;     // frames per batch
+constexpr int kInFlight = 3;      // batches queued on the device at once
+constexpr int kResultStride = 20; // doubles per result record: pose[16], tracked, pad
+
+struct Slot {
+  int stream;
+  int first;  // the stream's first frame: initialise, fuse at the start pose
+  const uint16_t* depth_in;
+  size_t dpitch;
+  const uint8_t* rgb_in;
+  size_t rpitch;
+  double* out;  // result record: pose[16], tracked
+};
+struct Table {
+  int count;
+  int pad;
+  Slot slot[kMaxSlots];
+};
+
+#define SLOT_OR_RETURN                                    \
+  if ((int)blockIdx.z >= tab->count) return;              \
+  const Slot& sl = tab->slot[blockIdx.z];                 \
+  const StreamDev& S = streams[sl.stream]
+
+// ------------------------------------------------------------------ kernels
+__global__ void __launch_bounds__(256) k_s_convert(const StreamDev* streams, const Table* tab) {
+  SLOT_OR_RETURN;
+  const int x = blockIdx.x * 64 + (threadIdx.x & 63), y = blockIdx.y * 4 + (threadIdx.x >> 6);
+  if (x >= S.W || y >= S.H) return;
+  const uint16_t d = ((const uint16_t*)((const char*)sl.depth_in + (size_t)y * sl.dpitch))[x];
+  S.depth[(size_t)y * S.W + x] = (float)d * S.depth_scale_inv;
+  rgb_to_gray_at(sl.rgb_in, sl.rpitch, S.gray, (size_t)S.W * 4, S.W, S.H, S.gray_mode, blockIdx.x, blockIdx.y);
+}
+
+// Pose pair of T_last (ray cast at the last pose, or fusion at the new one);
+// clears the visible counter.
+__global__ void k_s_pose(const StreamDev* streams, const Table* tab, int skip_first) {
+  SLOT_OR_RETURN;
+  if (threadIdx.x != 0 || (skip_first && sl.first)) return;
+  PosePair p;
+  pose_pair_from(S.T_last, p);
+  *S.pose = p;
+  S.m.ctr[kCtrVisible] = 0;
+}
+
+__global__ void __launch_bounds__(256) k_s_visible(const StreamDev* streams, const Table* tab, int fusion) {
+  SLOT_OR_RETURN;
+  if (!fusion && sl.first) return;
+  if (fusion && !sl.first && *S.gate == 0) return;
+  if ((int)blockIdx.x >= S.vis_groups) return;
+  visible_scan(S.m, S.pose->Tinv, S.K, S.W, S.H, blockIdx.x * blockDim.x, S.vis_groups * blockDim.x);
+}
+
+__global__ void __launch_bounds__(256) k_s_zrange_init(const StreamDev* streams, const Table* tab) {
+  SLOT_OR_RETURN;
+  if (sl.first) return;
+  const int n = S.cols * S.rows, g = (n + 255) / 256;
+  if ((int)blockIdx.x >= g) return;
+  zrange_init_at(S.zmin, S.zmax, n, blockIdx.x, g);
+}
+
+__global__ void __launch_bounds__(256) k_s_zrange(const StreamDev* streams, const Table* tab) {
+  SLOT_OR_RETURN;
+  if (sl.first || (int)blockIdx.x >= S.zr_groups) return;
+  zrange_at(S.m, S.pose, S.K, S.cols, S.rows, S.zmin, S.zmax, blockIdx.x, S.zr_groups);
+}
+
+template <class VT>
+__global__ void __launch_bounds__(256) k_s_march(const StreamDev* streams, const Table* tab) {
+  SLOT_OR_RETURN;
+  if (sl.first) return;
+  const int T = ((S.W + 15) >> 4) * ((S.H + 15) >> 4);
+  if ((int)blockIdx.x >= T) return;
+  march_at<VT>(S.m, S.pose, S.K, S.W, S.H, S.cols, (const float*)S.zmin, (const float*)S.zmax, S.vmap, S.vsinv,
+               S.step_scale, blockIdx.x);
+}
+
+// Tracker inputs.  `what`: 0 model copy, 1 model pyrDown, 2 model normals,
+// 3 intensity copy, 4 intensity pyrDown, 5 gradients, 6 inverse depth /
+// depth pyrDown, 7 vertices, 8 current normals, 9 keep (current -> reference).
+__global__ void __launch_bounds__(256) k_s_image(const StreamDev* streams, const Table* tab, int what, int l) {
+  SLOT_OR_RETURN;
+  if (l >= S.levels) return;
+  const Level& a = S.lv[l];
+  const TrackBufs& B = S.tb[l];
+  const int bx = blockIdx.x, by = blockIdx.y;
+  const size_t p4 = (size_t)a.w * 4, p16 = (size_t)a.w * 16;
+  switch (what) {
+    case 0:
+      if (!sl.first) copy_f4_at(S.vmap, (size_t)S.W * 16, (float4*)B.ref_v, S.W, S.H, bx, by);
+      break;
+    case 1:
+      if (!sl.first)
+        pyrdown_vmap_at(S.tb[l - 1].ref_v, (size_t)S.lv[l - 1].w * 16, (float4*)B.ref_v, p16, a.w, a.h, bx, by);
+      break;
+    case 2:
+      if (!sl.first) normal_at(B.ref_v, p16, (float4*)B.ref_n, p16, a.w, a.h, bx, by);
+      break;
+    case 3:
+      copy_f32_at(S.gray, (size_t)S.W * 4, (float*)B.cur_int, S.W, S.H, bx, by);
+      break;
+    case 4:
+      if (by * 16 < a.h && bx * 16 < a.w)
+        pyrdown_gauss_at(S.tb[l - 1].cur_int, (size_t)S.lv[l - 1].w * 4, S.lv[l - 1].w, S.lv[l - 1].h,
+                         (float*)B.cur_int, p4, a.w, a.h, bx, by);
+      break;
+    case 5:
+      gradient_at(B.cur_int, p4, (float*)B.gx, (float*)B.gy, p4, a.w, a.h, bx, by);
+      break;
+    case 6:
+      if (l == 0) invdepth_at(S.depth, (size_t)S.W * 4, S.cur_inv[0], p4, a.w, a.h, bx, by);
+      else pyrdown_depth_at(S.cur_inv[l - 1], (size_t)S.lv[l - 1].w * 4, S.cur_inv[l], p4, a.w, a.h, bx, by);
+      break;
+    case 7:
+      vertex_at(S.cur_inv[l], p4, (float4*)B.cur_v, p16, a.w, a.h, a.invfx, a.invfy, a.cx, a.cy, 3.0f, bx, by);
+      break;
+    case 8:
+      normal_at(B.cur_v, p16, (float4*)B.cur_n, p16, a.w, a.h, bx, by);
+      break;
+    default: {
+      const int x = bx * 64 + (threadIdx.x & 63), y = by * 4 + (threadIdx.x >> 6);
+      if (x < a.w && y < a.h) {
+        const size_t k = (size_t)y * a.w + x;
+        ((float*)B.ref_int)[k] = B.cur_int[k];
+        ((float*)B.ref_inv)[k] = S.cur_inv[l][k];
+      }
+    }
+  }
+}
+
+__global__ void k_s_track_init(const StreamDev* streams, const Table* tab) {
+  SLOT_OR_RETURN;
+  if (threadIdx.x != 0 || sl.first) return;
+  track_init_identity_at(S.st);
+}
+
+template <bool ICP, bool RGB>
+__global__ void __launch_bounds__(256) k_s_pass1(const StreamDev* streams, const Table* tab, int l) {
+  SLOT_OR_RETURN;
+  if (sl.first || (int)blockIdx.x >= S.groups[l]) return;
+  pass1_at<ICP, RGB>(S.lv[l], S.tb[l], S.st, S.part_icp, S.part_res, S.angle_th, S.dist_th, blockIdx.x, S.groups[l]);
+}
+
+__global__ void __launch_bounds__(256) k_s_pass2(const StreamDev* streams, const Table* tab, int l) {
+  SLOT_OR_RETURN;
+  if (sl.first || (int)blockIdx.x >= S.groups[l]) return;
+  pass2_at(S.lv[l], S.tb[l], S.st, S.part_res, S.groups[l], S.part_rgb, blockIdx.x, S.groups[l]);
+}
+
+__global__ void __launch_bounds__(256) k_s_update(const StreamDev* streams, const Table* tab, int l, int icp, int rgb,
+                                                  int mode, int first_of_level, float wrh, float wrb, float wih,
+                                                  float wib) {
+  SLOT_OR_RETURN;
+  if (sl.first) return;
+  update_at(S.st, S.part_icp, icp ? S.groups[l] : 0, S.part_rgb, rgb ? S.groups[l] : 0, mode, first_of_level, 1,
+            l == 0 ? 1 : 0, wrh, wrb, wih, wib);
+}
+
+__global__ void k_s_post(const StreamDev* streams, const Table* tab) {
+  SLOT_OR_RETURN;
+  if (threadIdx.x != 0) return;
+  int tracked = 0;
+  if (sl.first) {
+    record_first_at(S.T_last, sl.out, &tracked);
+    *S.gate = 1;
+  } else {
+    post_track_at(S.st->last_ok, &S.st->good, S.T_last, sl.out, &tracked, S.gate);
+  }
+  sl.out[16] = (double)tracked;
+}
+
+__global__ void __launch_bounds__(256) k_s_alloc(const StreamDev* streams, const Table* tab) {
+  SLOT_OR_RETURN;
+  if (!sl.first && *S.gate == 0) return;
+  if ((int)blockIdx.x * 16 >= S.W || (int)blockIdx.y * 16 >= S.H) return;
+  __shared__ int walked;
+  if (threadIdx.x == 0) walked = 0;
+  __syncthreads();
+  alloc_walk(S.m, DepthF32{S.depth, (size_t)S.W * 4}, S.W, S.H, S.K, S.pose->T, blockIdx.x * 16 + (threadIdx.x & 15),
+             blockIdx.y * 16 + (threadIdx.x >> 4), &walked);
+  __syncthreads();
+  if (threadIdx.x == 0 && walked) atomicAdd(&S.m.ctr[kCtrWalk], walked);
+}
+
+template <class VT>
+__global__ void __launch_bounds__(256) k_s_integrate(const StreamDev* streams, const Table* tab) {
+  SLOT_OR_RETURN;
+  if ((int)blockIdx.x >= S.int_groups) return;
+  integrate_waves<VT>(S.m, S.pose->Tinv, S.K, DepthF32{S.depth, (size_t)S.W * 4}, S.W, S.H,
+                      blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6), S.int_groups * (blockDim.x >> 6));
+}
+
+inline dim3 g64x4(int W, int H, int z) { return dim3((W + 63) / 64, (H + 3) / 4, z); }
+
+struct Request {
+  long long id;
+  const uint16_t* depth;
+  size_t dpitch;
+  const uint8_t* rgb;
+  size_t rpitch;
+};
+
+struct HostStream {
+  yas_pipeline* pipe = nullptr;
+  size_t bytes = 0;
+  long long frames = 0;  // frames launched so far
+  std::deque<Request> waiting;
+  int in_flight = 0;
+};
+
+struct Batch {
+  Table* staged = nullptr;   // pinned
+  double* results = nullptr; // pinned [kMaxSlots * kResultStride]
+  hipEvent_t done = nullptr;
+  bool busy = false;
+  int count = 0;
+  int stream_of[kMaxSlots];
+  long long req_of[kMaxSlots];
+};
+
+// Device bytes held by one stream (its map, ray caster, tracker and inputs).
+size_t stream_bytes(int W, int H, const yas_map_params& mp, int levels) {
+  const size_t vb = mp.voxel_format == YAS_VOXEL_F32_F32 ? 8 : 4;
+  size_t b = (size_t)mp.num_buckets * 16 + (size_t)mp.num_blocks * (vb * 512 + 8 + 4 + 4);
+  b += (size_t)W * H * 16 + (size_t)(W / 8) * (H / 8) * 8;  // vertex map, z-ranges
+  size_t px = 0;
+  for (int l = 0; l < levels; ++l) px += (size_t)(W >> l) * (H >> l);
+  b += px * (6 * 4 + 6 * 16);                                 // tracker pyramids
+  b += 256 * 68 * 4 + 1024;                                    // partials, state
+  b += (size_t)W * H * 8 + 4096 + 1024 * (16 * 8 + 4 + 32);    // inputs, pose records
+  return b;
+}
+
+}  // namespace
+
+struct yas_server {
+  yas_ctx* ctx;      // own stream (graphs cannot be captured on the null stream)
+  int device;
+  yas_tracker_params tp;
+  int voxel_format;
+  int maxW, maxH;
+  size_t budget, used;
+  int max_streams;
+  std::vector<HostStream> streams;  // slot per stream id (pipe == nullptr: free)
+  StreamDev* dev_streams;
+  Table* dev_table;
+  double* dev_results;  // [kInFlight][kMaxSlots * kResultStride]
+  Batch batches[kInFlight];
+  int next_batch;
+  std::map<int, hipGraphExec_t> graphs;  // batch-size class -> graph
+  std::vector<hipGraph_t> graph_defs;
+  std::deque<yas_result> done;
+  long long next_request;
+  long long frames_done, batches_done, slots_done;
+};
+
+static int launch_batch_kernels(yas_server* sv, hipStream_t s, int nz) {
+  const StreamDev* S = sv->dev_streams;
+  const Table* T = sv->dev_table;
+  const int W = sv->maxW, H = sv->maxH;
+  const int L = sv->tp.levels;
+  const bool icp = sv->tp.mode != YAS_TRACK_RGB, rgb = sv->tp.mode != YAS_TRACK_DEPTH;
+  auto level_grid = [&](int l) { return g64x4(W >> l, H >> l, nz); };
+  // inputs
+  hipLaunchKernelGGL(k_s_convert, g64x4(W, H, nz), dim3(256), 0, s, S, T);
+  // ray cast at the last pose
+  hipLaunchKernelGGL(k_s_pose, dim3(1, 1, nz), dim3(64), 0, s, S, T, 1);
+  hipLaunchKernelGGL(k_s_visible, dim3(4096, 1, nz), dim3(256), 0, s, S, T, 0);
+  hipLaunchKernelGGL(k_s_zrange_init, dim3(((W / 8) * (H / 8) + 255) / 256, 1, nz), dim3(256), 0, s, S, T);
+  hipLaunchKernelGGL(k_s_zrange, dim3(2048, 1, nz), dim3(256), 0, s, S, T);
+  const dim3 mg(((W + 15) / 16) * ((H + 15) / 16), 1, nz);
+  switch (sv->voxel_format) {
+    case YAS_VOXEL_I16_U8: hipLaunchKernelGGL(k_s_march<VoxI16>, mg, dim3(256), 0, s, S, T); break;
+    case YAS_VOXEL_F16_U16: hipLaunchKernelGGL(k_s_march<VoxF16>, mg, dim3(256), 0, s, S, T); break;
+    default: hipLaunchKernelGGL(k_s_march<VoxF32>, mg, dim3(256), 0, s, S, T);
+  }
+  // tracker inputs: reference model pyramid, current intensity and depth pyramids
+  hipLaunchKernelGGL(k_s_image, level_grid(0), dim3(256), 0, s, S, T, 0, 0);
+  for (int l = 0; l < L; ++l) {
+    if (l > 0) hipLaunchKernelGGL(k_s_image, level_grid(l), dim3(256), 0, s, S, T, 1, l);
+    hipLaunchKernelGGL(k_s_image, level_grid(l), dim3(256), 0, s, S, T, 2, l);
+  }
+  hipLaunchKernelGGL(k_s_image, level_grid(0), dim3(256), 0, s, S, T, 3, 0);
+  for (int l = 1; l < L; ++l)
+    hipLaunchKernelGGL(k_s_image, dim3(((W >> l) + 15) / 16, ((H >> l) + 15) / 16, nz), dim3(256), 0, s, S, T, 4, l);
+  for (int l = 0; l < L; ++l) hipLaunchKernelGGL(k_s_image, level_grid(l), dim3(256), 0, s, S, T, 5, l);
+  for (int l = 0; l < L; ++l) {
+    hipLaunchKernelGGL(k_s_image, level_grid(l), dim3(256), 0, s, S, T, 6, l);
+    hipLaunchKernelGGL(k_s_image, level_grid(l), dim3(256), 0, s, S, T, 7, l);
+    hipLaunchKernelGGL(k_s_image, level_grid(l), dim3(256), 0, s, S, T, 8, l);
+  }
+  // Gauss-Newton, coarse to fine
+  hipLaunchKernelGGL(k_s_track_init, dim3(1, 1, nz), dim3(64), 0, s, S, T);
+  for (int l = L - 1; l >= 0; --l) {
+    const int G = std::max(1, std::min(256, ((W >> l) * (H >> l) + 1023) / 1024));
+    for (int it = 0; it < sv->tp.iters[l]; ++it) {
+      if (icp && rgb) hipLaunchKernelGGL((k_s_pass1<true, true>), dim3(G, 1, nz), dim3(256), 0, s, S, T, l);
+      else if (icp) hipLaunchKernelGGL((k_s_pass1<true, false>), dim3(G, 1, nz), dim3(256), 0, s, S, T, l);
+      else hipLaunchKernelGGL((k_s_pass1<false, true>), dim3(G, 1, nz), dim3(256), 0, s, S, T, l);
+      if (rgb) hipLaunchKernelGGL(k_s_pass2, dim3(G, 1, nz), dim3(256), 0, s, S, T, l);
+      hipLaunchKernelGGL(k_s_update, dim3(1, 1, nz), dim3(256), 0, s, S, T, l, icp ? 1 : 0, rgb ? 1 : 0, sv->tp.mode,
+                         it == 0 ? 1 : 0, sv->tp.w_rgb_h, sv->tp.w_rgb_b, sv->tp.w_icp_h, sv->tp.w_icp_b);
+    }
+  }
+  // gate, pose chain, result record
+  hipLaunchKernelGGL(k_s_post, dim3(1, 1, nz), dim3(64), 0, s, S, T);
+  // the current pyramids become the reference of the next frame
+  for (int l = 0; l < L; ++l) hipLaunchKernelGGL(k_s_image, level_grid(l), dim3(256), 0, s, S, T, 9, l);
+  // fusion at the new pose
+  hipLaunchKernelGGL(k_s_pose, dim3(1, 1, nz), dim3(64), 0, s, S, T, 0);
+  hipLaunchKernelGGL(k_s_alloc, dim3((W + 15) / 16, (H + 15) / 16, nz), dim3(256), 0, s, S, T);
+  hipLaunchKernelGGL(k_s_visible, dim3(4096, 1, nz), dim3(256), 0, s, S, T, 1);
+  switch (sv->voxel_format) {
+    case YAS_VOXEL_I16_U8: hipLaunchKernelGGL(k_s_integrate<VoxI16>, dim3(2048, 1, nz), dim3(256), 0, s, S, T); break;
+    case YAS_VOXEL_F16_U16: hipLaunchKernelGGL(k_s_integrate<VoxF16>, dim3(2048, 1, nz), dim3(256), 0, s, S, T); break;
+    default: hipLaunchKernelGGL(k_s_integrate<VoxF32>, dim3(2048, 1, nz), dim3(256), 0, s, S, T);
+  }
+  YAS_LAUNCH_CHECK();
+  return YAS_OK;
+}
+
+static int graph_for(yas_server* sv, int count, hipGraphExec_t* out) {
+  int nz = 1;
+  while (nz < count) nz <<= 1;
+  auto f = sv->graphs.find(nz);
+  if (f != sv->graphs.end()) {
+    *out = f->second;
+    return YAS_OK;
+  }
+  hipStream_t s = sv->ctx->stream;
+  YAS_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+  int rc = launch_batch_kernels(sv, s, nz);
+  hipGraph_t g = nullptr;
+  hipError_t e = hipStreamEndCapture(s, &g);
+  if (rc) {
+    if (g) hipGraphDestroy(g);
+    return rc;
+  }
+  YAS_HIP(e);
+  hipGraphExec_t ex = nullptr;
+  YAS_HIP(hipGraphInstantiate(&ex, g, nullptr, nullptr, 0));
+  sv->graph_defs.push_back(g);
+  sv->graphs[nz] = ex;
+  *out = ex;
+  return YAS_OK;
+}
+
+// Completed batches -> results queue.
+static int collect(yas_server* sv, bool wait) {
+  for (int k = 0; k < kInFlight; ++k) {
+    Batch& b = sv->batches[k];
+    if (!b.busy) continue;
+    if (wait) YAS_HIP(hipEventSynchronize(b.done));
+    hipError_t q = hipEventQuery(b.done);
+    if (q == hipErrorNotReady) continue;
+    YAS_HIP(q);
+    for (int i = 0; i < b.count; ++i) {
+      yas_result r;
+      r.stream = b.stream_of[i];
+      r.request = b.req_of[i];
+      const double* rec = b.results + (size_t)i * kResultStride;
+      std::memcpy(r.pose, rec, sizeof(r.pose));
+      r.tracked = rec[16] != 0.0 ? 1 : 0;
+      sv->done.push_back(r);
+      sv->streams[r.stream].in_flight--;
+    }
+    sv->frames_done += b.count;
+    sv->batches_done += 1;
+    b.busy = false;
+  }
+  return YAS_OK;
+}
+
+extern "C" {
+
+int yas_server_create(yas_ctx* ctx, int max_width, int max_height, int max_streams, const yas_tracker_params* tp,
+                      int voxel_format, size_t hbm_budget_bytes, yas_server** out) {
+  YAS_REQUIRE(ctx && tp && out, "yas_server_create: null argument");
+  YAS_REQUIRE(max_width >= 8 && max_height >= 8 && max_streams >= 1, "yas_server_create: bad sizes");
+  YAS_REQUIRE(tp->levels >= 1 && tp->levels <= kTrackMaxLevels, "yas_server_create: levels must be 1..5");
+  YAS_REQUIRE(voxel_format >= 0 && voxel_format <= 2, "yas_server_create: unknown voxel format");
+  yas_server* sv = new yas_server();
+  sv->device = ctx->device;
+  sv->tp = *tp;
+  sv->voxel_format = voxel_format;
+  sv->maxW = max_width;
+  sv->maxH = max_height;
+  sv->budget = hbm_budget_bytes;
+  sv->used = 0;
+  sv->max_streams = max_streams;
+  sv->streams.resize(max_streams);
+  sv->next_batch = 0;
+  sv->next_request = 0;
+  sv->frames_done = sv->batches_done = sv->slots_done = 0;
+  int rc = yas_ctx_create(ctx->device, nullptr, &sv->ctx);
+  if (rc) {
+    delete sv;
+    return rc;
+  }
+  hipError_t e = hipMalloc((void**)&sv->dev_streams, sizeof(StreamDev) * max_streams);
+  if (e == hipSuccess) e = hipMalloc((void**)&sv->dev_table, sizeof(Table));
+  if (e == hipSuccess) e = hipMalloc((void**)&sv->dev_results, sizeof(double) * kInFlight * kMaxSlots * kResultStride);
+  for (int k = 0; e == hipSuccess && k < kInFlight; ++k) {
+    e = hipHostMalloc((void**)&sv->batches[k].staged, sizeof(Table), hipHostMallocDefault);
+    if (e == hipSuccess)
+      e = hipHostMalloc((void**)&sv->batches[k].results, sizeof(double) * kMaxSlots * kResultStride, hipHostMallocDefault);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&sv->batches[k].done, hipEventDisableTiming);
+  }
+  if (e != hipSuccess) {
+    yas_set_error("yas_server_create: %s", hipGetErrorString(e));
+    yas_server_destroy(sv);
+    return YAS_E_HIP;
+  }
+  *out = sv;
+  return YAS_OK;
+}
+
+int yas_server_destroy(yas_server* sv) {
+  if (!sv) return YAS_OK;
+  hipSetDevice(sv->device);
+  if (sv->ctx) hipStreamSynchronize(sv->ctx->stream);
+  for (auto& kv : sv->graphs) hipGraphExecDestroy(kv.second);
+  for (hipGraph_t g : sv->graph_defs) hipGraphDestroy(g);
+  for (auto& hs : sv->streams)
+    if (hs.pipe) yas_pipeline_destroy(hs.pipe);
+  for (auto& b : sv->batches) {
+    if (b.staged) hipHostFree(b.staged);
+    if (b.results) hipHostFree(b.results);
+    if (b.done) hipEventDestroy(b.done);
+  }
+  hipFree(sv->dev_streams);
+  hipFree(sv->dev_table);
+  hipFree(sv->dev_results);
+  if (sv->ctx) yas_ctx_destroy(sv->ctx);
+  delete sv;
+  return YAS_OK;
+}
+
+int yas_server_add_stream(yas_server* sv, int W, int H, const float K[4], const yas_map_params* mp, float depth_scale,
+                          int gray_mode, int* stream_id) {
+  YAS_REQUIRE(sv && K && mp && stream_id, "yas_server_add_stream: null argument");
+  YAS_REQUIRE(W <= sv->maxW && H <= sv->maxH, "yas_server_add_stream: image larger than the server's maximum");
+  YAS_REQUIRE(mp->voxel_format == sv->voxel_format, "yas_server_add_stream: voxel format differs from the server's");
+  int id = -1;
+  for (int i = 0; i < sv->max_streams; ++i)
+    if (!sv->streams[i].pipe) {
+      id = i;
+      break;
+    }
+  if (id < 0) {
+    yas_set_error("yas_server_add_stream: all %d stream slots are taken", sv->max_streams);
+    return YAS_E_CAPACITY;
+  }
+  const size_t need = stream_bytes(W, H, *mp, sv->tp.levels);
+  if (sv->used + need > sv->budget) {
+    yas_set_error("yas_server_add_stream: %zu bytes would exceed the HBM budget (%zu of %zu in use)", need, sv->used,
+                  sv->budget);
+    return YAS_E_CAPACITY;
+  }
+  YAS_HIP(hipSetDevice(sv->device));
+  yas_pipeline* p = nullptr;
+  int rc = yas_pipeline_create(sv->ctx, W, H, K, mp, &sv->tp, depth_scale, gray_mode, &p);
+  if (rc) return rc;
+  StreamDev d;
+  if ((rc = yas_pipeline_fill(p, d))) {
+    yas_pipeline_destroy(p);
+    return rc;
+  }
+  YAS_HIP(hipMemcpyAsync(sv->dev_streams + id, &d, sizeof(d), hipMemcpyHostToDevice, sv->ctx->stream));
+  YAS_HIP(hipStreamSynchronize(sv->ctx->stream));
+  HostStream& hs = sv->streams[id];
+  hs.pipe = p;
+  hs.bytes = need;
+  hs.frames = 0;
+  hs.waiting.clear();
+  hs.in_flight = 0;
+  sv->used += need;
+  *stream_id = id;
+  return YAS_OK;
+}
+
+int yas_server_remove_stream(yas_server* sv, int stream_id) {
+  YAS_REQUIRE(sv && stream_id >= 0 && stream_id < sv->max_streams && sv->streams[stream_id].pipe,
+              "yas_server_remove_stream: no such stream");
+  HostStream& hs = sv->streams[stream_id];
+  YAS_REQUIRE(hs.waiting.empty(), "yas_server_remove_stream: the stream still has frames waiting");
+  while (hs.in_flight > 0) {
+    int rc = collect(sv, true);
+    if (rc) return rc;
+  }
+  yas_pipeline_destroy(hs.pipe);
+  hs.pipe = nullptr;
+  sv->used -= hs.bytes;
+  hs.bytes = 0;
+  return YAS_OK;
+}
+
+int yas_server_stream_map(yas_server* sv, int stream_id, yas_map** map) {
+  YAS_REQUIRE(sv && map && stream_id >= 0 && stream_id < sv->max_streams && sv->streams[stream_id].pipe,
+              "yas_server_stream_map: no such stream");
+  *map = yas_pipeline_map_of(sv->streams[stream_id].pipe);
+  return YAS_OK;
+}
+
+int yas_server_submit(yas_server* sv, int stream_id, const uint16_t* depth, size_t dpitch, const uint8_t* rgb,
+                      size_t rpitch, long long* request_id) {
+  YAS_REQUIRE(sv && depth && rgb && stream_id >= 0 && stream_id < sv->max_streams && sv->streams[stream_id].pipe,
+              "yas_server_submit: bad argument");
+  Request r{sv->next_request++, depth, dpitch, rgb, rpitch};
+  sv->streams[stream_id].waiting.push_back(r);
+  if (request_id) *request_id = r.id;
+  return YAS_OK;
+}
+
+int yas_server_pump(yas_server* sv, int* launched) {
+  YAS_REQUIRE(sv, "null server");
+  YAS_HIP(hipSetDevice(sv->device));
+  int n_launched = 0;
+  int rc = collect(sv, false);
+  if (rc) return rc;
+  for (;;) {
+    Batch& b = sv->batches[sv->next_batch];
+    if (b.busy) break;  // kInFlight batches queued already
+    int count = 0;
+    for (int i = 0; i < sv->max_streams && count < kMaxSlots; ++i) {
+      HostStream& hs = sv->streams[i];
+      if (!hs.pipe || hs.waiting.empty()) continue;
+      const Request r = hs.waiting.front();
+      hs.waiting.pop_front();
+      Slot& s = b.staged->slot[count];
+      s.stream = i;
+      s.first = hs.frames == 0 ? 1 : 0;
+      s.depth_in = r.depth;
+      s.dpitch = r.dpitch;
+      s.rgb_in = r.rgb;
+      s.rpitch = r.rpitch;
+      s.out = sv->dev_results + ((size_t)sv->next_batch * kMaxSlots + count) * kResultStride;
+      b.stream_of[count] = i;
+      b.req_of[count] = r.id;
+      hs.frames++;
+      hs.in_flight++;
+      ++count;
+    }
+    if (count == 0) break;
+    b.staged->count = count;
+    b.count = count;
+    hipGraphExec_t ex = nullptr;
+    if ((rc = graph_for(sv, count, &ex))) return rc;
+    hipStream_t s = sv->ctx->stream;
+    YAS_HIP(hipMemcpyAsync(sv->dev_table, b.staged, sizeof(int) * 2 + sizeof(Slot) * count, hipMemcpyHostToDevice, s));
+    YAS_HIP(hipGraphLaunch(ex, s));
+    YAS_HIP(hipMemcpyAsync(b.results, sv->dev_results + (size_t)sv->next_batch * kMaxSlots * kResultStride,
+                           sizeof(double) * kResultStride * count, hipMemcpyDeviceToHost, s));
+    YAS_HIP(hipEventRecord(b.done, s));
+    b.busy = true;
+    sv->slots_done += count;
+    sv->next_batch = (sv->next_batch + 1) % kInFlight;
+    ++n_launched;
+  }
+  if (launched) *launched = n_launched;
+  return YAS_OK;
+}
+
+int yas_server_poll(yas_server* sv, yas_result* out, int capacity, int* count) {
+  YAS_REQUIRE(sv && count && (out || capacity == 0), "yas_server_poll: bad argument");
+  int rc = collect(sv, false);
+  if (rc) return rc;
+  int n = 0;
+  while (n < capacity && !sv->done.empty()) {
+    out[n++] = sv->done.front();
+    sv->done.pop_front();
+  }
+  *count = n;
+  return YAS_OK;
+}
+
+int yas_server_drain(yas_server* sv) {
+  YAS_REQUIRE(sv, "null server");
+  for (;;) {
+    int launched = 0;
+    int rc = yas_server_pump(sv, &launched);
+    if (rc) return rc;
+    bool waiting = false;
+    for (auto& hs : sv->streams) waiting = waiting || !hs.waiting.empty();
+    bool busy = false;
+    for (auto& b : sv->batches) busy = busy || b.busy;
+    if (!waiting && !busy) return YAS_OK;
+    if (!launched) {
+      if ((rc = collect(sv, true))) return rc;
+    }
+  }
+}
+
+int yas_server_get_stats(yas_server* sv, yas_server_stats* st) {
+  YAS_REQUIRE(sv && st, "null argument");
+  int n = 0;
+  for (auto& hs : sv->streams) n += hs.pipe ? 1 : 0;
+  st->streams = n;
+  st->bytes_in_use = sv->used;
+  st->bytes_budget = sv->budget;
+  st->frames_done = sv->frames_done;
+  st->batches_done = sv->batches_done;
+  return YAS_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,670 @@
+// Device code of the dense tracker (RGBDFusion/src/RGBDTracking.cu,
+// RGBDFusion/include/TrackingUtils.h) and its image operations
+// (RGBDFusion/src/ImageProc.cu), shared by the single-tracker kernels and
+// the kernels that run many trackers in one launch.  Every body takes its
+// block coordinates (bx, by) and, for the reductions, the number of
+// workgroups nb of its own grid, so a tracker's arithmetic and summation
+// order are the same whichever launch runs it.
+#pragma once
+
+#include <cfloat>
+#include <cmath>
+
+#include "yas_internal.hpp"
+
+namespace yas {
+
+constexpr int kRedGroups = 256;  // workgroups of a reduction pass
+constexpr int kMaxLevels = 5;
+
+struct Level {
+  int w, h;
+  float fx, fy, cx, cy, invfx, invfy;
+  double Kd[4];
+};
+
+struct TrackState {
+  double est[16];
+  double last_ok[16];
+  float sums_rgb[32];
+  float sums_icp[32];
+  float rgb_stats[4];  // sum r, count, mean, Huber threshold
+  float last_err;
+  int good;
+  int icp_inliers;
+  int rgb_inliers;
+};
+
+// ------------------------------------------------------------------ reductions
+// Wave-wide sums of 32 values by a transpose butterfly: at each step a lane
+// keeps half of its values and trades the other half with its partner, so
+// 32 values need 16+8+4+2+1+1 shuffles.  On return, lanes 2i and 2i+1 hold
+// the total of value i.  The step size is a template constant so every
+// register index is static.
+template <int S>
+__device__ inline void butterfly_step(float (&v)[32], int lane) {
+  const bool hi = (lane & (2 * S)) != 0;
+#pragma unroll
+  for (int i = 0; i < S; ++i) {
+    const float send = hi ? v[i] : v[i + S];
+    const float keep = hi ? v[i + S] : v[i];
+    v[i] = keep + __shfl_xor(send, 2 * S, 64);
+  }
+}
+__device__ inline float wave_sum32(float (&v)[32]) {
+  const int lane = threadIdx.x & 63;
+  butterfly_step<16>(v, lane);
+  butterfly_step<8>(v, lane);
+  butterfly_step<4>(v, lane);
+  butterfly_step<2>(v, lane);
+  butterfly_step<1>(v, lane);
+  return v[0] + __shfl_xor(v[0], 1, 64);
+}
+
+// Workgroup sums of 32 values; thread t < 32 returns the total of value t.
+__device__ inline float block_sum32(float (&v)[32], float* lds /* [nwaves * 32] */) {
+  const float w = wave_sum32(v);
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  if ((lane & 1) == 0) lds[wid * 32 + (lane >> 1)] = w;
+  __syncthreads();
+  float t = 0.0f;
+  if (threadIdx.x < 32) {
+    const int nw = blockDim.x >> 6;
+    t = lds[threadIdx.x];
+    for (int k = 1; k < nw; ++k) t += lds[k * 32 + threadIdx.x];
+  }
+  __syncthreads();
+  return t;
+}
+
+// Chan et al.'s pairwise update of (count, mean, M2) statistics.
+__device__ inline void chan_merge(float& n, float& mean, float& m2, float nb, float meanb, float m2b) {
+  const float nn = n + nb;
+  if (nb == 0.0f) return;
+  if (n == 0.0f) {
+    n = nb; mean = meanb; m2 = m2b;
+    return;
+  }
+  const float d = meanb - mean;
+  mean = mean + d * (nb / nn);
+  m2 = m2 + m2b + d * d * (n * nb / nn);
+  n = nn;
+}
+
+// Workgroup merge of per-thread statistics; the result is broadcast to all threads.
+__device__ inline void block_stats(float& n, float& mean, float& m2, float* lds /* [3 * nwaves] */) {
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    float nb = __shfl_xor(n, o, 64), mb = __shfl_xor(mean, o, 64), qb = __shfl_xor(m2, o, 64);
+    // merge in a lane-independent order so both partners get identical values
+    if ((threadIdx.x & o) == 0) {
+      chan_merge(n, mean, m2, nb, mb, qb);
+    } else {
+      float n0 = nb, m0 = mb, q0 = qb;
+      chan_merge(n0, m0, q0, n, mean, m2);
+      n = n0; mean = m0; m2 = q0;
+    }
+  }
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  if (lane == 0) {
+    lds[3 * wid] = n; lds[3 * wid + 1] = mean; lds[3 * wid + 2] = m2;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const int nw = blockDim.x >> 6;
+    float a = lds[0], b = lds[1], c = lds[2];
+    for (int k = 1; k < nw; ++k) chan_merge(a, b, c, lds[3 * k], lds[3 * k + 1], lds[3 * k + 2]);
+    lds[0] = a; lds[1] = b; lds[2] = c;
+  }
+  __syncthreads();
+  n = lds[0]; mean = lds[1]; m2 = lds[2];
+  __syncthreads();
+}
+
+__device__ inline void load_pose_f(const double* T, float* lds_pose /* 12 */) {
+  if (threadIdx.x < 12) {
+    int i = threadIdx.x / 4, j = threadIdx.x % 4;
+    int k = j < 3 ? 3 * i + j : 9 + i;
+    lds_pose[k] = (float)T[4 * i + j];
+  }
+}
+
+// ------------------------------------------------------------------ image ops
+__device__ inline float ld(const float* p, size_t pitch, int x, int y) {
+  return ((const float*)((const char*)p + (size_t)y * pitch))[x];
+}
+
+__device__ inline void copy_f32_at(const float* src, size_t sp, float* dst, int W, int H, int bx, int by) {
+  int x = bx * 64 + (threadIdx.x & 63), y = by * 4 + (threadIdx.x >> 6);
+  if (x < W && y < H) dst[(size_t)y * W + x] = ld(src, sp, x, y);
+}
+
+// mode 0: 0.299R + 0.587G + 0.114B (float); mode 1: 8-bit fixed point.
+__device__ inline void rgb_to_gray_at(const uint8_t* rgb, size_t sp, float* dst, size_t dp, int W,
+                                                     int H, int mode, int bx, int by) {
+  int x = bx * 64 + (threadIdx.x & 63), y = by * 4 + (threadIdx.x >> 6);
+  if (x >= W || y >= H) return;
+  const uint8_t* p = rgb + (size_t)y * sp + 3 * x;
+  int r = p[0], g = p[1], b = p[2];
+  float v = mode == 0 ? 0.299f * (float)r + 0.587f * (float)g + 0.114f * (float)b
+                      : (float)((r * 4899 + g * 9617 + b * 1868 + (1 << 13)) >> 14);
+  ((float*)((char*)dst + (size_t)y * dp))[x] = v;
+}
+
+// OpenCV pyrDown restated: binomial [1 4 6 4 1]/16 per axis (vertical sums of
+// the 5 source columns first, then horizontal), BORDER_REFLECT_101.
+__device__ inline int reflect101(int i, int n) {
+  if (i < 0) i = -i;
+  if (i >= n) i = 2 * n - 2 - i;
+  return i;
+}
+__device__ inline void pyrdown_gauss_at(const float* src, size_t sp, int sw, int sh, float* dst,
+                                                       size_t dp, int dw, int dh, int bx, int by) {
+  int x = bx * 16 + (threadIdx.x & 15), y = by * 16 + (threadIdx.x >> 4);
+  if (x >= dw || y >= dh) return;
+  const float k[5] = {0.0625f, 0.25f, 0.375f, 0.25f, 0.0625f};
+  int rows[5];
+#pragma unroll
+  for (int i = 0; i < 5; ++i) rows[i] = reflect101(2 * y + i - 2, sh);
+  float col[5];
+#pragma unroll
+  for (int j = 0; j < 5; ++j) {
+    int sx = reflect101(2 * x + j - 2, sw);
+    float acc = k[0] * ld(src, sp, sx, rows[0]);
+#pragma unroll
+    for (int i = 1; i < 5; ++i) acc = acc + k[i] * ld(src, sp, sx, rows[i]);
+    col[j] = acc;
+  }
+  float acc = k[0] * col[0];
+#pragma unroll
+  for (int j = 1; j < 5; ++j) acc = acc + k[j] * col[j];
+  ((float*)((char*)dst + (size_t)y * dp))[x] = acc;
+}
+
+// ImageProc.cu:7-25.
+__device__ inline void gradient_at(const float* s, size_t sp, float* gx, float* gy, size_t dp, int W,
+                                                  int H, int bx, int by) {
+  int x = bx * 64 + (threadIdx.x & 63), y = by * 4 + (threadIdx.x >> 6);
+  if (x >= W || y >= H) return;
+  float a = 0.0f, b = 0.0f;
+  if (!(x < 1 || y < 1 || x > W - 2 || y > H - 2)) {
+    a = (ld(s, sp, x + 1, y) - ld(s, sp, x - 1, y)) * 0.5f;
+    b = (ld(s, sp, x, y + 1) - ld(s, sp, x, y - 1)) * 0.5f;
+  }
+  ((float*)((char*)gx + (size_t)y * dp))[x] = a;
+  ((float*)((char*)gy + (size_t)y * dp))[x] = b;
+}
+
+// ImageProc.cu:141-153.
+__device__ inline void invdepth_at(const float* z, size_t sp, float* o, size_t dp, int W, int H, int bx, int by) {
+  int x = bx * 64 + (threadIdx.x & 63), y = by * 4 + (threadIdx.x >> 6);
+  if (x >= W || y >= H) return;
+  float v = ld(z, sp, x, y);
+  ((float*)((char*)o + (size_t)y * dp))[x] = (v > 0.2f && v < 8.0f) ? 1.0f / v : 0.0f;
+}
+
+// ImageProc.cu:166-175, last row/column (left unwritten there) set to 0.
+__device__ inline void pyrdown_depth_at(const float* s, size_t sp, float* o, size_t dp, int W, int H, int bx, int by) {
+  int x = bx * 64 + (threadIdx.x & 63), y = by * 4 + (threadIdx.x >> 6);
+  if (x >= W || y >= H) return;
+  float v = (x < W - 1 && y < H - 1) ? ld(s, sp, 2 * x, 2 * y) : 0.0f;
+  ((float*)((char*)o + (size_t)y * dp))[x] = v;
+}
+
+// Whole-float4 subsampling of the reference model (departure from the
+// reference's float kernel run over float4 data, RGBDTracking.cu:146).
+__device__ inline void pyrdown_vmap_at(const float4* s, size_t sp, float4* o, size_t dp, int W, int H, int bx, int by) {
+  int x = bx * 64 + (threadIdx.x & 63), y = by * 4 + (threadIdx.x >> 6);
+  if (x >= W || y >= H) return;
+  float4 v = make_float4(0.f, 0.f, 0.f, -1.f);
+  if (x < W - 1 && y < H - 1) v = ((const float4*)((const char*)s + (size_t)(2 * y) * sp))[2 * x];
+  ((float4*)((char*)o + (size_t)y * dp))[x] = v;
+}
+
+__device__ inline void copy_f4_at(const float4* s, size_t sp, float4* o, int W, int H, int bx, int by) {
+  int x = bx * 64 + (threadIdx.x & 63), y = by * 4 + (threadIdx.x >> 6);
+  if (x < W && y < H) o[(size_t)y * W + x] = ((const float4*)((const char*)s + (size_t)y * sp))[x];
+}
+
+// ImageProc.cu:189-209.
+__device__ inline void vertex_at(const float* inv, size_t sp, float4* v, size_t dp, int W, int H,
+                                                float invfx, float invfy, float cx, float cy, float cut, int bx, int by) {
+  int x = bx * 64 + (threadIdx.x & 63), y = by * 4 + (threadIdx.x >> 6);
+  if (x >= W || y >= H) return;
+  float iz = ld(inv, sp, x, y);
+  float z = 1.0f / iz;
+  float4 o = make_float4(0.f, 0.f, 0.f, -1.f);
+  if (iz > 0.0f && z < cut) o = make_float4(z * ((float)x - cx) * invfx, z * ((float)y - cy) * invfy, z, 1.0f);
+  ((float4*)((char*)v + (size_t)y * dp))[x] = o;
+}
+
+// ImageProc.cu:222-249.
+__device__ inline void normal_at(const float4* v, size_t sp, float4* n, size_t dp, int W, int H, int bx, int by) {
+  int x = bx * 64 + (threadIdx.x & 63), y = by * 4 + (threadIdx.x >> 6);
+  if (x >= W || y >= H) return;
+  float4 o = make_float4(0.f, 0.f, 0.f, -1.f);
+  if (x != W - 1 && y != H - 1) {
+    const float4 a = ((const float4*)((const char*)v + (size_t)y * sp))[x];
+    const float4 b = ((const float4*)((const char*)v + (size_t)y * sp))[x + 1];
+    const float4 c = ((const float4*)((const char*)v + (size_t)(y + 1) * sp))[x];
+    if (a.w > 0.f && b.w > 0.f && c.w > 0.f) {
+      float p0 = c.x - a.x, p1 = c.y - a.y, p2 = c.z - a.z;
+      float q0 = b.x - a.x, q1 = b.y - a.y, q2 = b.z - a.z;
+      float r0 = p1 * q2 - p2 * q1, r1 = p2 * q0 - p0 * q2, r2 = p0 * q1 - p1 * q0;
+      float n2 = r0 * r0 + r1 * r1 + r2 * r2;
+      if (n2 > 0.0f) {
+        float nn = sqrtf(n2);
+        r0 = r0 / nn; r1 = r1 / nn; r2 = r2 / nn;
+      }
+      o = make_float4(r0, r1, r2, 1.0f);
+    }
+  }
+  ((float4*)((char*)n + (size_t)y * dp))[x] = o;
+}
+
+// ImageProc.cu:262-275.
+__device__ inline void vmap_to_depth_at(const float4* v, size_t sp, float* d, size_t dp, int W, int H, int bx, int by) {
+  int x = bx * 64 + (threadIdx.x & 63), y = by * 4 + (threadIdx.x >> 6);
+  if (x >= W || y >= H) return;
+  float4 a = ((const float4*)((const char*)v + (size_t)y * sp))[x];
+  ((float*)((char*)d + (size_t)y * dp))[x] = a.w > 0.f ? a.z : 0.0f;
+}
+
+// ------------------------------------------------------------------ per-pixel terms
+struct TrackBufs {
+  const float4* ref_v;
+  const float4* ref_n;
+  const float4* cur_v;
+  const float4* cur_n;
+  const float* ref_inv;
+  const float* ref_int;
+  const float* cur_int;
+  const float* gx;
+  const float* gy;
+  float4* warped;
+  float4* resid;
+};
+
+// RGBDTracking.cu:316-372: projective association of reference pixel k into
+// the current frame, gated at sin(20 deg) normal angle and 1 cm; on a match
+// the point-to-plane row [n, v x n, n.(v_c - v)] is accumulated.
+__device__ inline void icp_pixel(const Level& L, const TrackBufs& B, const PoseF& T, float angle_th, float dist_th,
+                                 int k, float (&acc)[32]) {
+  const float4 vl = B.ref_v[k];
+  if (!(vl.w >= 0.0f)) return;
+  float vx, vy, vz;
+  xform(T, vl.x, vl.y, vl.z, vx, vy, vz);
+  const float invz = 1.0f / vz;
+  const int u = floor_sat(L.fx * vx * invz + L.cx + 0.5f);
+  const int v = floor_sat(L.fy * vy * invz + L.cy + 0.5f);
+  if (u < 0 || v < 0 || u >= L.w || v >= L.h) return;
+  const float4 vc = B.cur_v[v * L.w + u];
+  if (vc.w < 0.0f) return;
+  const float4 nl = B.ref_n[k];
+  float nx, ny, nz;
+  rot3(T, nl.x, nl.y, nl.z, nx, ny, nz);
+  const float4 nc = B.cur_n[v * L.w + u];
+  const float dx = vx - vc.x, dy = vy - vc.y, dz = vz - vc.z;
+  const float dist = sqrtf(dx * dx + dy * dy + dz * dz);
+  const float c0 = nc.y * nz - nc.z * ny, c1 = nc.z * nx - nc.x * nz, c2 = nc.x * ny - nc.y * nx;
+  const float ang = sqrtf(c0 * c0 + c1 * c1 + c2 * c2);
+  if (!(ang < angle_th && dist <= dist_th && nl.w > 0.0f && nc.w > 0.0f)) return;
+  float row[7];
+  row[0] = nx; row[1] = ny; row[2] = nz;
+  row[3] = vy * nz - vz * ny;
+  row[4] = vz * nx - vx * nz;
+  row[5] = vx * ny - vy * nx;
+  row[6] = nx * (vc.x - vx) + ny * (vc.y - vy) + nz * (vc.z - vz);
+  int c = 0;
+#pragma unroll
+  for (int i = 0; i < 7; ++i)
+#pragma unroll
+    for (int j = i; j < 7; ++j) acc[c++] += row[i] * row[j];
+  acc[28] += 1.0f;
+}
+
+// Warp parameters of ImageProc.cu:42-78: RK = R K^-1 (formed in double), t.
+__device__ inline void load_warp(const double* T, const double* Kd, float* lds /* 12 */) {
+  if (threadIdx.x >= 16 && threadIdx.x < 28) {
+    int q = threadIdx.x - 16;
+    int i = q / 4, j = q % 4;
+    if (j < 3) {
+      double Ki[9] = {1.0 / Kd[0], 0.0, -Kd[2] / Kd[0], 0.0, 1.0 / Kd[1], -Kd[3] / Kd[1], 0.0, 0.0, 1.0};
+      lds[3 * i + j] = (float)(T[4 * i] * Ki[j] + T[4 * i + 1] * Ki[3 + j] + T[4 * i + 2] * Ki[6 + j]);
+    } else {
+      lds[9 + i] = (float)T[4 * i + 3];
+    }
+  }
+}
+
+__device__ inline float bilerp(const float* m, int w, float x, float y) {
+  int u = (int)floorf(x), v = (int)floorf(y);
+  float cx = x - (float)u, cy = y - (float)v;
+  const float* r0 = m + (size_t)v * w;
+  const float* r1 = r0 + w;
+  return (r0[u] * (1.0f - cx) + r0[u + 1] * cx) * (1.0f - cy) + (r1[u] * (1.0f - cx) + r1[u + 1] * cx) * cy;
+}
+
+// TrackingUtils.h:73-151 with the warp of ImageProc.cu:42-61 fused in; the
+// residual statistics are accumulated with Welford's update.
+__device__ inline void rgb_pixel(const Level& L, const TrackBufs& B, const float* wp, int k, float& n, float& mean,
+                                 float& m2) {
+  const int y = k / L.w, x = k - y * L.w;
+  const float iz = B.ref_inv[k];
+  float4 p = make_float4(0.f, 0.f, 0.f, -1.f);
+  if (iz > 0.0f) {
+    float z = 1.0f / iz;
+    float px = (wp[0] * (float)x + wp[1] * (float)y + wp[2]) * z + wp[9];
+    float py = (wp[3] * (float)x + wp[4] * (float)y + wp[5]) * z + wp[10];
+    float pz = (wp[6] * (float)x + wp[7] * (float)y + wp[8]) * z + wp[11];
+    p = make_float4(px, py, pz, 1.0f);
+  }
+  B.warped[k] = p;
+  float r = 0.0f, gx = 0.0f, gy = 0.0f;
+  bool ok = false;
+  if (p.w > 0.0f) {
+    float u = L.fx * p.x / p.z + L.cx;
+    float v = L.fy * p.y / p.z + L.cy;
+    if (u > 0.0f && v > 0.0f && u < (float)(L.w - 1) && v < (float)(L.h - 1)) {
+      float ref = B.ref_int[k];
+      float cur = bilerp(B.cur_int, L.w, u, v);
+      r = cur - ref;
+      gx = bilerp(B.gx, L.w, u, v);
+      gy = bilerp(B.gy, L.w, u, v);
+      ok = (gx * gx + gy * gy) > 0.0f && isfinite(r);
+    }
+  }
+  B.resid[k] = make_float4(r, gx, gy, ok ? 0.5f : -0.5f);
+  if (ok) {
+    n += 1.0f;
+    float d = r - mean;
+    mean = mean + d / n;
+    m2 = m2 + d * (r - mean);
+  }
+}
+
+// Pass 1: ICP rows and/or photometric residuals of one linearisation.
+template <bool ICP, bool RGB>
+__device__ inline void pass1_at(const Level& L, const TrackBufs& B, const TrackState* st, float* part_icp,
+                                float* part_res, float angle_th, float dist_th, int bx, int nb) {
+  __shared__ float pose[12];
+  __shared__ float wp[12];
+  __shared__ float lds[4 * 32];
+  load_pose_f(st->est, pose);
+  if (RGB) load_warp(st->est, L.Kd, wp);
+  __syncthreads();
+  PoseF T;
+#pragma unroll
+  for (int i = 0; i < 9; ++i) T.r[i] = pose[i];
+  T.t[0] = pose[9]; T.t[1] = pose[10]; T.t[2] = pose[11];
+  float acc[32];
+#pragma unroll
+  for (int i = 0; i < 32; ++i) acc[i] = 0.0f;
+  float n = 0.0f, mean = 0.0f, m2 = 0.0f;
+  const int N = L.w * L.h;
+  for (int k = bx * blockDim.x + threadIdx.x; k < N; k += nb * blockDim.x) {
+    if (ICP) icp_pixel(L, B, T, angle_th, dist_th, k, acc);
+    if (RGB) rgb_pixel(L, B, wp, k, n, mean, m2);
+  }
+  if (ICP) {
+    float t = block_sum32(acc, lds);
+    if (threadIdx.x < 32) part_icp[bx * 32 + threadIdx.x] = t;
+  }
+  if (RGB) {
+    block_stats(n, mean, m2, lds);
+    if (threadIdx.x == 0) {
+      part_res[bx * 4] = n;
+      part_res[bx * 4 + 1] = mean;
+      part_res[bx * 4 + 2] = m2;
+    }
+  }
+}
+
+// Pass 2 (TrackingUtils.h:153-220, RGBDTracking.cu:262-290): fold the
+// residual statistics of pass 1 (identically in every workgroup), derive
+// sigma and the Huber threshold 4.685 sigma, and reduce the weighted rows.
+__device__ inline void pass2_at(const Level& L, const TrackBufs& B, TrackState* st, const float* part_res, int g1,
+                                float* part_rgb, int bx, int nb) {
+  __shared__ float lds[4 * 32];
+  float n = 0.0f, mean = 0.0f, m2 = 0.0f;
+  for (int g = threadIdx.x; g < g1; g += blockDim.x)
+    chan_merge(n, mean, m2, part_res[g * 4], part_res[g * 4 + 1], part_res[g * 4 + 2]);
+  block_stats(n, mean, m2, lds);
+  const float sigma = sqrtf(m2 / (n - 1.0f));
+  const float th = (float)(4.685 * (double)sigma);
+  if (bx == 0 && threadIdx.x == 0) {
+    st->rgb_stats[0] = n * mean;
+    st->rgb_stats[1] = n;
+    st->rgb_stats[2] = mean;
+    st->rgb_stats[3] = th;
+  }
+  float acc[32];
+#pragma unroll
+  for (int i = 0; i < 32; ++i) acc[i] = 0.0f;
+  const int N = L.w * L.h;
+  for (int k = bx * blockDim.x + threadIdx.x; k < N; k += nb * blockDim.x) {
+    const float4 e = B.resid[k];
+    if (!(e.w > 0.0f)) continue;
+    const float4 p = B.warped[k];
+    const float zinv = 1.0f / p.z;
+    float wt = 1.0f;
+    if (fabsf(e.x) > th) wt = th / fabsf(e.x);
+    float row[7];
+    row[0] = e.y * L.fx * zinv;
+    row[1] = e.z * L.fy * zinv;
+    row[2] = -(row[0] * p.x + row[1] * p.y) * zinv;
+    row[3] = row[2] * p.y - e.z * L.fy;
+    row[4] = -row[2] * p.x + e.y * L.fx;
+    row[5] = -row[0] * p.y + row[1] * p.x;
+    row[6] = -e.x;
+    int c = 0;
+#pragma unroll
+    for (int i = 0; i < 7; ++i)
+#pragma unroll
+      for (int j = i; j < 7; ++j) acc[c++] += wt * row[i] * row[j];
+    acc[28] += 1.0f;
+  }
+  float t = block_sum32(acc, lds);
+  if (threadIdx.x < 32) part_rgb[bx * 32 + threadIdx.x] = t;
+}
+
+// ------------------------------------------------------------------ GN update
+// Eigen 3.3 LDLT restated (diagonal pivoting, pseudo-inverse of D).
+__device__ inline void ldlt_solve6(double* A, double* y, int* tr) {
+  const int n = 6;
+  for (int k = 0; k < n; ++k) {
+    int big = k;
+    double bv = fabs(A[k * n + k]);
+    for (int i = k + 1; i < n; ++i)
+      if (fabs(A[i * n + i]) > bv) { bv = fabs(A[i * n + i]); big = i; }
+    tr[k] = big;
+    if (big != k) {
+      for (int j = 0; j < k; ++j) { double t = A[k * n + j]; A[k * n + j] = A[big * n + j]; A[big * n + j] = t; }
+      for (int i = big + 1; i < n; ++i) { double t = A[i * n + k]; A[i * n + k] = A[i * n + big]; A[i * n + big] = t; }
+      { double t = A[k * n + k]; A[k * n + k] = A[big * n + big]; A[big * n + big] = t; }
+      for (int i = k + 1; i < big; ++i) {
+        double t = A[i * n + k];
+        A[i * n + k] = A[big * n + i];
+        A[big * n + i] = t;
+      }
+    }
+    double temp[6];
+    if (k > 0) {
+      for (int j = 0; j < k; ++j) temp[j] = A[j * n + j] * A[k * n + j];
+      double s = 0.0;
+      for (int j = 0; j < k; ++j) s += A[k * n + j] * temp[j];
+      A[k * n + k] -= s;
+      for (int i = k + 1; i < n; ++i) {
+        double r = 0.0;
+        for (int j = 0; j < k; ++j) r += A[i * n + j] * temp[j];
+        A[i * n + k] -= r;
+      }
+    }
+    double akk = A[k * n + k];
+    bool valid = fabs(akk) > 0.0;
+    if (k == 0 && !valid) {
+      for (int j = 0; j < n; ++j) tr[j] = j;
+      break;
+    }
+    if (valid)
+      for (int i = k + 1; i < n; ++i) A[i * n + k] /= akk;
+  }
+  for (int k = 0; k < n; ++k) { double t = y[k]; y[k] = y[tr[k]]; y[tr[k]] = t; }
+  for (int i = 0; i < n; ++i)
+    for (int j = 0; j < i; ++j) y[i] -= A[i * n + j] * y[j];
+  for (int i = 0; i < n; ++i) {
+    double d = A[i * n + i];
+    y[i] = fabs(d) > DBL_MIN ? y[i] / d : 0.0;
+  }
+  for (int i = n - 1; i >= 0; --i)
+    for (int j = i + 1; j < n; ++j) y[i] -= A[j * n + i] * y[j];
+  for (int k = n - 1; k >= 0; --k) { double t = y[k]; y[k] = y[tr[k]]; y[tr[k]] = t; }
+}
+
+// SE(3) exponential, twist (rho, omega), Rodrigues' formula.
+__device__ inline void se3_exp(const double* xi, double* T) {
+  const double* w = xi + 3;
+  double th2 = w[0] * w[0] + w[1] * w[1] + w[2] * w[2];
+  double th = sqrt(th2);
+  double A, B, C;
+  if (th < 1e-8) {
+    A = 1.0 - th2 / 6.0;
+    B = 0.5 - th2 / 24.0;
+    C = 1.0 / 6.0 - th2 / 120.0;
+  } else {
+    A = sin(th) / th;
+    B = (1.0 - cos(th)) / th2;
+    C = (th - sin(th)) / (th2 * th);
+  }
+  double W[9] = {0, -w[2], w[1], w[2], 0, -w[0], -w[1], w[0], 0};
+  double W2[9], Rm[9], V[9];
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) W2[3 * i + j] = W[3 * i] * W[j] + W[3 * i + 1] * W[3 + j] + W[3 * i + 2] * W[6 + j];
+  for (int i = 0; i < 9; ++i) {
+    double I = (i % 4 == 0) ? 1.0 : 0.0;
+    Rm[i] = I + A * W[i] + B * W2[i];
+    V[i] = I + B * W[i] + C * W2[i];
+  }
+  for (int i = 0; i < 3; ++i) {
+    for (int j = 0; j < 3; ++j) T[4 * i + j] = Rm[3 * i + j];
+    T[4 * i + 3] = V[3 * i] * xi[0] + V[3 * i + 1] * xi[1] + V[3 * i + 2] * xi[2];
+  }
+  T[12] = T[13] = T[14] = 0.0;
+  T[15] = 1.0;
+}
+
+// RankUpdateHessian<6,7> (TrackingUtils.h:45-58).
+__device__ inline void unpack29(const float* s, float* H, float* b) {
+  int k = 0;
+  for (int i = 0; i < 6; ++i)
+    for (int j = i; j < 7; ++j) {
+      float v = s[k++];
+      if (j == 6) b[i] = v;
+      else H[6 * i + j] = H[6 * j + i] = v;
+    }
+}
+
+// Fold the partial rows of one linearisation and, when `do_step`, take one
+// Gauss-Newton step of RGBDTracking.cu:161-204.  One workgroup.
+__device__ inline void update_at(TrackState* st, const float* part_icp, int gi, const float* part_rgb, int gr,
+                                 int mode, int first_of_level, int do_step, int record, float wrh, float wrb, float wih,
+                                 float wib) {
+  __shared__ float lds[4 * 32];
+  __shared__ float sums[2][32];
+  __shared__ double A[36];
+  __shared__ double y[6];
+  __shared__ int tr[6];
+  for (int which = 0; which < 2; ++which) {
+    const float* part = which == 0 ? part_icp : part_rgb;
+    const int g = which == 0 ? gi : gr;
+    if (g == 0) continue;
+    float v[32];
+#pragma unroll
+    for (int i = 0; i < 32; ++i) v[i] = 0.0f;
+    for (int r = threadIdx.x; r < g; r += blockDim.x)
+#pragma unroll
+      for (int i = 0; i < 32; ++i) v[i] += part[r * 32 + i];
+    float t = block_sum32(v, lds);
+    if (threadIdx.x < 32) sums[which][threadIdx.x] = t;
+  }
+  __syncthreads();
+  if (threadIdx.x != 0) return;
+  if (gi) {
+    for (int i = 0; i < 29; ++i) st->sums_icp[i] = sums[0][i];
+    if (record) st->icp_inliers = (int)sums[0][28];
+  }
+  if (gr) {
+    for (int i = 0; i < 29; ++i) st->sums_rgb[i] = sums[1][i];
+    if (record) st->rgb_inliers = (int)sums[1][28];
+  }
+  if (!do_step) return;
+  if (first_of_level) st->last_err = FLT_MAX;
+  if (!st->good) return;
+  float H[36], b[6];
+#pragma unroll
+  for (int i = 0; i < 36; ++i) H[i] = 0.0f;
+#pragma unroll
+  for (int i = 0; i < 6; ++i) b[i] = 0.0f;
+  float rsum = 0.0f, nres = 0.0f;
+  if (mode != YAS_TRACK_DEPTH) {
+    float Hr[36], br[6];
+    unpack29(sums[1], Hr, br);
+    const float wh = mode == YAS_TRACK_RGBD ? wrh : 1.0f, wb = mode == YAS_TRACK_RGBD ? wrb : 1.0f;
+#pragma unroll
+    for (int i = 0; i < 36; ++i) H[i] = H[i] + wh * Hr[i];
+#pragma unroll
+    for (int i = 0; i < 6; ++i) b[i] = b[i] + wb * br[i];
+    rsum = sums[1][27];
+    nres = sums[1][28];
+  }
+  if (mode != YAS_TRACK_RGB) {
+    float Hi[36], bi[6];
+    unpack29(sums[0], Hi, bi);
+    const float wh = mode == YAS_TRACK_RGBD ? wih : 1.0f, wb = mode == YAS_TRACK_RGBD ? wib : 1.0f;
+#pragma unroll
+    for (int i = 0; i < 36; ++i) H[i] = H[i] + wh * Hi[i];
+#pragma unroll
+    for (int i = 0; i < 6; ++i) b[i] = b[i] + wb * bi[i];
+    rsum = sums[0][27];
+  }
+  const float err = sqrtf(rsum) / (nres + 6.0f);
+#pragma unroll
+  for (int i = 0; i < 36; ++i) A[i] = (double)H[i];
+#pragma unroll
+  for (int i = 0; i < 6; ++i) y[i] = (double)b[i];
+  ldlt_solve6(A, y, tr);
+  double xi[6];
+#pragma unroll
+  for (int i = 0; i < 6; ++i) xi[i] = y[i];
+  if (isnan(xi[0])) {
+    st->good = 0;
+    return;
+  }
+  double E[16], est[16];
+  se3_exp(xi, E);
+#pragma unroll
+  for (int i = 0; i < 16; ++i) est[i] = st->est[i];
+  mul44(E, est, est);
+#pragma unroll
+  for (int i = 0; i < 16; ++i) st->est[i] = est[i];
+  if (err < st->last_err) {
+#pragma unroll
+    for (int i = 0; i < 16; ++i) st->last_ok[i] = est[i];
+    st->last_err = err;
+  }
+}
+
+__device__ inline void track_init_at(TrackState* st, const double* T) {
+  if (threadIdx.x != 0) return;
+  for (int i = 0; i < 16; ++i) st->est[i] = st->last_ok[i] = T[i];
+  st->good = 1;
+  st->last_err = FLT_MAX;
+}
+__device__ inline void track_init_identity_at(TrackState* st) {
+  if (threadIdx.x != 0) return;
+  for (int i = 0; i < 16; ++i) st->est[i] = st->last_ok[i] = (i % 5 == 0) ? 1.0 : 0.0;
+  st->good = 1;
+  st->last_err = FLT_MAX;
+}
+
+}  // namespace yas

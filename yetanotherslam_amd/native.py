@@ -31,6 +31,15 @@ class BatchConfig(ctypes.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+class Result(ctypes.Structure):
+    _fields_ = [("stream", I), ("tracked", I), ("request", ctypes.c_longlong), ("pose", ctypes.c_double * 16)]
+
+
+class ServerStats(ctypes.Structure):
+    _fields_ = [("streams", I), ("frames_done", ctypes.c_longlong), ("batches_done", ctypes.c_longlong),
+                ("bytes_in_use", ctypes.c_size_t), ("bytes_budget", ctypes.c_size_t)]
+
+
 class MapStats(ctypes.Structure):
     _fields_ = [("num_allocated", I), ("num_visible", I), ("flags", I), ("reserved", I)]
 
@@ -76,6 +85,16 @@ SIGNATURES = {
     "yas_mesh_extract_indexed": (I, [P, P, PI, PI]),
     "yas_mesher_write_ply": (I, [P, ctypes.c_char_p]),
     "yas_map_fuse_map": (I, [P, P]),
+    "yas_server_create": (I, [P, I, I, I, P, I, ctypes.c_size_t, P]),
+    "yas_server_destroy": (I, [P]),
+    "yas_server_add_stream": (I, [P, I, I, P, P, ctypes.c_float, I, PI]),
+    "yas_server_remove_stream": (I, [P, I]),
+    "yas_server_stream_map": (I, [P, I, P]),
+    "yas_server_submit": (I, [P, I, P, ctypes.c_size_t, P, ctypes.c_size_t, P]),
+    "yas_server_pump": (I, [P, PI]),
+    "yas_server_poll": (I, [P, P, I, PI]),
+    "yas_server_drain": (I, [P]),
+    "yas_server_get_stats": (I, [P, P]),
     "yas_batch_create": (I, [P, I, P, P]),
     "yas_batch_destroy": (I, [P]),
     "yas_batch_plan": (I, [P, I, P, P, P, P, ctypes.c_float, P]),

@@ -16,7 +16,7 @@ import numpy as np
 import torch
 
 from . import native
-from .native import BatchConfig, MapParams, MapStats, TrackerParams, TrackStats, call
+from .native import BatchConfig, MapParams, MapStats, Result, ServerStats, TrackerParams, TrackStats, call
 
 VOXEL_I16_U8, VOXEL_F32_F32, VOXEL_F16_U16 = 0, 1, 2
 TRACK_RGB, TRACK_DEPTH, TRACK_RGBD = 0, 1, 2
@@ -219,6 +219,83 @@ class MapStruct:
         call("yas_map_get_params", h, ctypes.byref(m.params), _ptr(m.K))
         m.voxel_format = m.params.voxel_format
         return m
+
+
+class Server:
+    """Many RGB-D streams tracked and fused with continuous batching
+    (yas_server_*)."""
+
+    def __init__(self, max_width, max_height, max_streams, tracker_params, voxel_format=VOXEL_F32_F32,
+                 hbm_budget=200 << 30, ctx=None):
+        self.ctx = ctx or default_context()
+        h = ctypes.c_void_p()
+        call("yas_server_create", self.ctx.h, max_width, max_height, max_streams, ctypes.byref(tracker_params),
+             voxel_format, hbm_budget, ctypes.byref(h))
+        self.h = h
+        self._inputs = {}
+
+    def close(self):
+        if getattr(self, "h", None):
+            native.lib().yas_server_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def add_stream(self, W, H, K, map_params, depth_scale=1000.0, gray_mode=0):
+        Kf = np.asarray(K, np.float32)
+        sid = ctypes.c_int()
+        call("yas_server_add_stream", self.h, W, H, _ptr(Kf), ctypes.byref(map_params), depth_scale, gray_mode,
+             ctypes.byref(sid))
+        return sid.value
+
+    def remove_stream(self, sid):
+        call("yas_server_remove_stream", self.h, sid)
+
+    def submit(self, sid, depth_u16, rgb_u8):
+        """Queue a frame (device tensors, ready now, kept until its result is polled)."""
+        req = ctypes.c_longlong()
+        call("yas_server_submit", self.h, sid, _dev(depth_u16), _pitch(depth_u16), _dev(rgb_u8), _pitch(rgb_u8),
+             ctypes.byref(req))
+        self._inputs[req.value] = (depth_u16, rgb_u8)
+        return req.value
+
+    def pump(self):
+        n = ctypes.c_int()
+        call("yas_server_pump", self.h, ctypes.byref(n))
+        return n.value
+
+    def poll(self, capacity=256):
+        buf = (Result * capacity)()
+        n = ctypes.c_int()
+        call("yas_server_poll", self.h, buf, capacity, ctypes.byref(n))
+        out = []
+        for r in buf[:n.value]:
+            self._inputs.pop(r.request, None)
+            out.append({"stream": r.stream, "request": r.request, "tracked": bool(r.tracked),
+                        "pose": np.array(r.pose[:], np.float64).reshape(4, 4)})
+        return out
+
+    def drain(self):
+        call("yas_server_drain", self.h)
+
+    def stats(self):
+        st = ServerStats()
+        call("yas_server_get_stats", self.h, ctypes.byref(st))
+        return {k: getattr(st, k) for k, _ in st._fields_}
+
+    def stream_map(self, sid, voxel_format=VOXEL_F32_F32):
+        """A non-owning MapStruct view of the stream's volume."""
+        m = ctypes.c_void_p()
+        call("yas_server_stream_map", self.h, sid, ctypes.byref(m))
+        view = MapStruct.__new__(MapStruct)
+        view.h = m
+        view.voxel_format = voxel_format
+        view.close = lambda: None
+        return view
 
 
 class FusionBatch:
