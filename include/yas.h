@@ -246,6 +246,9 @@ int yas_server_add_stream(yas_server* sv, int W, int H, const float K[4], const 
 int yas_server_remove_stream(yas_server* sv, int stream_id);
 /* The stream's volume (for meshing or export between batches). */
 int yas_server_stream_map(yas_server* sv, int stream_id, yas_map** map);
+/* The stream's pipeline (its map, ray caster and tracker) for inspection
+ * between batches; its step and timing calls must not be used. */
+int yas_server_stream_pipeline(yas_server* sv, int stream_id, yas_pipeline** pipe);
 /* Queue a frame: uint16 depth and RGB8 colour in device memory, ready when
  * submitted and valid until its result is polled. */
 int yas_server_submit(yas_server* sv, int stream_id, const uint16_t* depth, size_t dpitch, const uint8_t* rgb,

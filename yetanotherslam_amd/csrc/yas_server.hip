@@ -292,6 +292,7 @@ struct yas_server {
   std::map<int, hipGraphExec_t> graphs;  // batch-size class -> graph
   std::vector<hipGraph_t> graph_defs;
   std::deque<yas_result> done;
+  std::deque<int> launch_order;  // batches in flight, oldest first
   long long next_request;
   long long frames_done, batches_done, slots_done;
 };
@@ -387,15 +388,17 @@ static int graph_for(yas_server* sv, int count, hipGraphExec_t* out) {
   return YAS_OK;
 }
 
-// Completed batches -> results queue.
+// Completed batches -> results queue, in launch order (batches finish in
+// that order on the server's stream; the oldest unfinished one stops the
+// scan, so results of a stream always come out in submission order).
 static int collect(yas_server* sv, bool wait) {
-  for (int k = 0; k < kInFlight; ++k) {
-    Batch& b = sv->batches[k];
-    if (!b.busy) continue;
+  while (!sv->launch_order.empty()) {
+    Batch& b = sv->batches[sv->launch_order.front()];
     if (wait) YAS_HIP(hipEventSynchronize(b.done));
     hipError_t q = hipEventQuery(b.done);
-    if (q == hipErrorNotReady) continue;
+    if (q == hipErrorNotReady) break;
     YAS_HIP(q);
+    sv->launch_order.pop_front();
     for (int i = 0; i < b.count; ++i) {
       yas_result r;
       r.stream = b.stream_of[i];
@@ -544,6 +547,13 @@ int yas_server_stream_map(yas_server* sv, int stream_id, yas_map** map) {
   return YAS_OK;
 }
 
+int yas_server_stream_pipeline(yas_server* sv, int stream_id, yas_pipeline** pipe) {
+  YAS_REQUIRE(sv && pipe && stream_id >= 0 && stream_id < sv->max_streams && sv->streams[stream_id].pipe,
+              "yas_server_stream_pipeline: no such stream");
+  *pipe = sv->streams[stream_id].pipe;
+  return YAS_OK;
+}
+
 int yas_server_submit(yas_server* sv, int stream_id, const uint16_t* depth, size_t dpitch, const uint8_t* rgb,
                       size_t rpitch, long long* request_id) {
   YAS_REQUIRE(sv && depth && rgb && stream_id >= 0 && stream_id < sv->max_streams && sv->streams[stream_id].pipe,
@@ -595,6 +605,7 @@ int yas_server_pump(yas_server* sv, int* launched) {
                            sizeof(double) * kResultStride * count, hipMemcpyDeviceToHost, s));
     YAS_HIP(hipEventRecord(b.done, s));
     b.busy = true;
+    sv->launch_order.push_back(sv->next_batch);
     sv->slots_done += count;
     sv->next_batch = (sv->next_batch + 1) % kInFlight;
     ++n_launched;

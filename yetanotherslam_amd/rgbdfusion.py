@@ -250,6 +250,8 @@ class Server:
         sid = ctypes.c_int()
         call("yas_server_add_stream", self.h, W, H, _ptr(Kf), ctypes.byref(map_params), depth_scale, gray_mode,
              ctypes.byref(sid))
+        self._dims = getattr(self, "_dims", {})
+        self._dims[sid.value] = (H, W)
         return sid.value
 
     def remove_stream(self, sid):
@@ -257,6 +259,9 @@ class Server:
 
     def submit(self, sid, depth_u16, rgb_u8):
         """Queue a frame (device tensors, ready now, kept until its result is polled)."""
+        H, W = self._dims[sid]
+        if tuple(depth_u16.shape) != (H, W) or tuple(rgb_u8.shape) != (H, W, 3):
+            raise ValueError("frame size differs from the stream's %dx%d" % (W, H))
         req = ctypes.c_longlong()
         call("yas_server_submit", self.h, sid, _dev(depth_u16), _pitch(depth_u16), _dev(rgb_u8), _pitch(rgb_u8),
              ctypes.byref(req))
