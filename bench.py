@@ -10,7 +10,7 @@ the synthetic room sequence at 640x480, voxel 5 mm, truncation 4 voxels,
 pyramid levels with 10/5/3 Gauss-Newton iterations.  All frames are uploaded
 to HBM before the timed region.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline] [--no-mesh] [--no-batch]
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline] [--no-mesh] [--no-batch] [--no-serving]
 Prints one JSON line on stdout; progress goes to stderr.
 """
 import argparse
@@ -214,6 +214,103 @@ def batch_bench(dev_depth, nseq, n_maps=64, seed=1):
     return out
 
 
+def serving_bench(dev_depth, dev_rgb, nseq, n_streams=32, seed=2, frames_sat=60, duration_s=4.0):
+    """Serving form: 32 streams, each a segment of the room sequence starting
+    at its own seeded frame (so its own start angle) at 640x480 or 320x240
+    (seeded), voxel 5 mm, 2^20 buckets and a 2^16-block pool per stream, f32
+    storage, the tracker of the main workload.
+      saturation: every stream's first `frames_sat` frames queued at once;
+                  aggregate frames/s from first submit to last result.
+      poisson:    frames arrive at 15-30 Hz per stream (seeded rate,
+                  exponential gaps) for `duration_s`; per-frame latency from
+                  the scheduled arrival to the result's collection."""
+    import torch
+    from yetanotherslam_amd import rgbdfusion as rf
+    from yetanotherslam_amd import synth
+
+    rng = np.random.default_rng(seed)
+    hi = rng.integers(0, 2, n_streams).astype(bool)
+    p640 = synth.params(width=W, height=H, theta0=THETA0)
+    p320 = synth.params(width=W // 2, height=H // 2, theta0=THETA0)
+    lo_frames = [synth.frame(p320, t) for t in range(nseq)]
+    d320 = torch.from_numpy(np.stack([f[0] for f in lo_frames]).view(np.int16)).cuda()
+    c320 = torch.from_numpy(np.stack([f[1] for f in lo_frames])).cuda()
+    K640, K320 = synth.intrinsics(p640), synth.intrinsics(p320)
+    tp = rf.tracker_params(LEVELS, ITERS, rf.TRACK_RGBD, WEIGHTS)
+    mp = rf.map_params(1 << 20, 1 << 16, VOXEL, TRUNC, rf.VOXEL_F32_F32)
+    torch.cuda.synchronize()
+
+    def frame(i, k):
+        return (dev_depth[k], dev_rgb[k]) if hi[i] else (d320[k], c320[k])
+
+    def new_server():
+        sv = rf.Server(W, H, n_streams, tp, hbm_budget=200 << 30)
+        ids = [sv.add_stream(W, H, K640, mp) if hi[i] else sv.add_stream(W // 2, H // 2, K320, mp)
+               for i in range(n_streams)]
+        return sv, ids
+
+    out = {"streams": n_streams, "streams_640x480": int(hi.sum()), "streams_320x240": int((~hi).sum())}
+    # saturation
+    start = rng.integers(0, nseq - frames_sat, n_streams)
+    sv, ids = new_server()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k in range(frames_sat):
+        for i, sid in enumerate(ids):
+            sv.submit(sid, *frame(i, int(start[i]) + k))
+    sv.drain()
+    t_sat = time.perf_counter() - t0
+    res = sv.poll(n_streams * frames_sat + 1)
+    st = sv.stats()
+    out["saturation"] = {"frames": len(res), "frames_per_s": round(len(res) / t_sat, 1),
+                         "tracked": int(sum(r["tracked"] for r in res)),
+                         "mean_batch": round(st["frames_done"] / max(st["batches_done"], 1), 1),
+                         "hbm_bytes_in_use": int(st["bytes_in_use"])}
+    sv.close()
+    # Poisson arrivals
+    rate = rng.uniform(15.0, 30.0, n_streams)
+    arrivals = []
+    for i in range(n_streams):
+        t, k = float(rng.exponential(1.0 / rate[i])), 0
+        while t < duration_s and k < nseq:
+            arrivals.append((t, i, k))
+            t += float(rng.exponential(1.0 / rate[i]))
+            k += 1
+    arrivals.sort()
+    start = rng.integers(0, max(1, nseq - max(a[2] for a in arrivals) - 1), n_streams)
+    sv, ids = new_server()
+    torch.cuda.synchronize()
+    due = {}
+    lat = []
+    nxt = 0
+    t0 = time.perf_counter()
+    while len(lat) < len(arrivals):
+        now = time.perf_counter() - t0
+        while nxt < len(arrivals) and arrivals[nxt][0] <= now:
+            ta, i, k = arrivals[nxt]
+            due[sv.submit(ids[i], *frame(i, int(start[i]) + k))] = ta
+            nxt += 1
+        sv.pump()
+        got = sv.poll()
+        now = time.perf_counter() - t0
+        for r in got:
+            lat.append(now - due.pop(r["request"]))
+        if not got and nxt < len(arrivals) and arrivals[nxt][0] > now:
+            time.sleep(min(2e-4, arrivals[nxt][0] - now))
+    t_end = time.perf_counter() - t0
+    st = sv.stats()
+    lat_ms = np.array(lat) * 1e3
+    out["poisson"] = {"frames": len(lat), "offered_frames_per_s": round(len(arrivals) / duration_s, 1),
+                      "frames_per_s": round(len(lat) / t_end, 1),
+                      "latency_ms": {"p50": round(float(np.percentile(lat_ms, 50)), 3),
+                                     "p99": round(float(np.percentile(lat_ms, 99)), 3),
+                                     "max": round(float(lat_ms.max()), 3)},
+                      "mean_batch": round(st["frames_done"] / max(st["batches_done"], 1), 1),
+                      "rates_hz": [round(float(rate.min()), 1), round(float(rate.max()), 1)]}
+    sv.close()
+    return out
+
+
 def copy_probe():
     """A plain device copy of known size (512 MiB read + 512 MiB written),
     used to calibrate rocprofv3's FETCH_SIZE / WRITE_SIZE counters."""
@@ -237,6 +334,7 @@ def main():
     ap.add_argument("--copy-probe", action="store_true", help="run only the counter-calibration copy")
     ap.add_argument("--no-mesh", action="store_true", help="skip the marching-cubes measurement")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-submap measurement")
+    ap.add_argument("--no-serving", action="store_true", help="skip the serving measurement")
     args = ap.parse_args()
     if args.copy_probe:
         copy_probe()
@@ -323,6 +421,12 @@ def main():
         "roofline": dict(rl[dominant], kernel=dominant),
         "roofline_by_kernel": rl,
     }
+    if not args.no_serving:
+        log("serving form: 32 streams")
+        result["serving"] = serving_bench(dev_depth, dev_rgb, n)
+        sat = result["serving"]["saturation"]["frames_per_s"]
+        result["serving"]["saturation"]["vs_plain_single_stream"] = round(sat / fps, 2)
+        log("serving:", result["serving"])
     if not args.no_batch:
         log("batching form: 64 submaps")
         result["batching"] = batch_bench(dev_depth, n)
