@@ -250,23 +250,27 @@ def serving_bench(dev_depth, dev_rgb, nseq, n_streams=32, seed=2, frames_sat=60,
         return sv, ids
 
     out = {"streams": n_streams, "streams_640x480": int(hi.sum()), "streams_320x240": int((~hi).sum())}
-    # saturation
+    mixed = hi.copy()
+    # saturation: all streams at 640x480, then the seeded resolution mix
     start = rng.integers(0, nseq - frames_sat, n_streams)
-    sv, ids = new_server()
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for k in range(frames_sat):
-        for i, sid in enumerate(ids):
-            sv.submit(sid, *frame(i, int(start[i]) + k))
-    sv.drain()
-    t_sat = time.perf_counter() - t0
-    res = sv.poll(n_streams * frames_sat + 1)
-    st = sv.stats()
-    out["saturation"] = {"frames": len(res), "frames_per_s": round(len(res) / t_sat, 1),
-                         "tracked": int(sum(r["tracked"] for r in res)),
-                         "mean_batch": round(st["frames_done"] / max(st["batches_done"], 1), 1),
-                         "hbm_bytes_in_use": int(st["bytes_in_use"])}
-    sv.close()
+    for name, hires in (("saturation_640x480", np.ones(n_streams, bool)), ("saturation", mixed)):
+        hi[:] = hires
+        sv, ids = new_server()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for k in range(frames_sat):
+            for i, sid in enumerate(ids):
+                sv.submit(sid, *frame(i, int(start[i]) + k))
+        sv.drain()
+        t_sat = time.perf_counter() - t0
+        res = sv.poll(n_streams * frames_sat + 1)
+        st = sv.stats()
+        out[name] = {"frames": len(res), "frames_per_s": round(len(res) / t_sat, 1),
+                     "tracked": int(sum(r["tracked"] for r in res)),
+                     "mean_batch": round(st["frames_done"] / max(st["batches_done"], 1), 1),
+                     "hbm_bytes_in_use": int(st["bytes_in_use"])}
+        sv.close()
+    hi[:] = mixed
     # Poisson arrivals
     rate = rng.uniform(15.0, 30.0, n_streams)
     arrivals = []
@@ -352,8 +356,9 @@ def main():
     p = synth.params(width=W, height=H, theta0=THETA0)
     K = synth.intrinsics(p)
     n = args.warmup + args.steps + 1
-    log("generating %d frames" % n)
-    depth, rgb = gen_frames(p, n)
+    npool = max(n, 200)  # the batching and serving forms draw segments from this sequence too
+    log("generating %d frames" % npool)
+    depth, rgb = gen_frames(p, npool)
     dev_depth = torch.from_numpy(depth.view(np.int16)).cuda()
     dev_rgb = torch.from_numpy(rgb).cuda()
     torch.cuda.synchronize()
@@ -423,13 +428,14 @@ def main():
     }
     if not args.no_serving:
         log("serving form: 32 streams")
-        result["serving"] = serving_bench(dev_depth, dev_rgb, n)
-        sat = result["serving"]["saturation"]["frames_per_s"]
-        result["serving"]["saturation"]["vs_plain_single_stream"] = round(sat / fps, 2)
+        result["serving"] = serving_bench(dev_depth, dev_rgb, npool)
+        for key in ("saturation_640x480", "saturation"):
+            sat = result["serving"][key]["frames_per_s"]
+            result["serving"][key]["vs_plain_single_stream"] = round(sat / fps, 2)
         log("serving:", result["serving"])
     if not args.no_batch:
         log("batching form: 64 submaps")
-        result["batching"] = batch_bench(dev_depth, n)
+        result["batching"] = batch_bench(dev_depth, npool)
         log("batching:", result["batching"])
     if not args.no_mesh:
         log("marching cubes over the fused volume")
