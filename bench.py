@@ -218,7 +218,8 @@ def serving_bench(dev_depth, dev_rgb, nseq, n_streams=32, seed=2, frames_sat=60,
     """Serving form: 32 streams, each a segment of the room sequence starting
     at its own seeded frame (so its own start angle) at 640x480 or 320x240
     (seeded), voxel 5 mm, 2^20 buckets and a 2^16-block pool per stream, f32
-    storage, the tracker of the main workload.
+    storage (and, for one saturation run, the precision form's float16 sdf +
+    uint16 weight), the tracker of the main workload.
       saturation: every stream's first `frames_sat` frames queued at once;
                   aggregate frames/s from first submit to last result.
       poisson:    frames arrive at 15-30 Hz per stream (seeded rate,
@@ -237,15 +238,15 @@ def serving_bench(dev_depth, dev_rgb, nseq, n_streams=32, seed=2, frames_sat=60,
     c320 = torch.from_numpy(np.stack([f[1] for f in lo_frames])).cuda()
     K640, K320 = synth.intrinsics(p640), synth.intrinsics(p320)
     tp = rf.tracker_params(LEVELS, ITERS, rf.TRACK_RGBD, WEIGHTS)
-    mp = rf.map_params(1 << 20, 1 << 16, VOXEL, TRUNC, rf.VOXEL_F32_F32)
     torch.cuda.synchronize()
 
     def frame(i, k):
         return (dev_depth[k], dev_rgb[k]) if hi[i] else (d320[k], c320[k])
 
-    def new_server():
-        sv = rf.Server(W, H, n_streams, tp, hbm_budget=200 << 30)
-        ids = [sv.add_stream(W, H, K640, mp) if hi[i] else sv.add_stream(W // 2, H // 2, K320, mp)
+    def new_server(fmt=rf.VOXEL_F32_F32):
+        m = rf.map_params(1 << 20, 1 << 16, VOXEL, TRUNC, fmt)
+        sv = rf.Server(W, H, n_streams, tp, voxel_format=fmt, hbm_budget=200 << 30)
+        ids = [sv.add_stream(W, H, K640, m) if hi[i] else sv.add_stream(W // 2, H // 2, K320, m)
                for i in range(n_streams)]
         return sv, ids
 
@@ -253,9 +254,11 @@ def serving_bench(dev_depth, dev_rgb, nseq, n_streams=32, seed=2, frames_sat=60,
     mixed = hi.copy()
     # saturation: all streams at 640x480, then the seeded resolution mix
     start = rng.integers(0, nseq - frames_sat, n_streams)
-    for name, hires in (("saturation_640x480", np.ones(n_streams, bool)), ("saturation", mixed)):
+    for name, hires, fmt in (("saturation_640x480", np.ones(n_streams, bool), rf.VOXEL_F32_F32),
+                             ("saturation_640x480_f16u16", np.ones(n_streams, bool), rf.VOXEL_F16_U16),
+                             ("saturation", mixed, rf.VOXEL_F32_F32)):
         hi[:] = hires
-        sv, ids = new_server()
+        sv, ids = new_server(fmt)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         for k in range(frames_sat):
@@ -429,7 +432,7 @@ def main():
     if not args.no_serving:
         log("serving form: 32 streams")
         result["serving"] = serving_bench(dev_depth, dev_rgb, npool)
-        for key in ("saturation_640x480", "saturation"):
+        for key in ("saturation_640x480", "saturation_640x480_f16u16", "saturation"):
             sat = result["serving"][key]["frames_per_s"]
             result["serving"][key]["vs_plain_single_stream"] = round(sat / fps, 2)
         log("serving:", result["serving"])

@@ -469,56 +469,95 @@ __device__ inline void pass2_at(const Level& L, const TrackBufs& B, TrackState* 
 }
 
 // ------------------------------------------------------------------ GN update
-// Eigen 3.3 LDLT restated (diagonal pivoting, pseudo-inverse of D).
-__device__ inline void ldlt_solve6(double* A, double* y, int* tr) {
-  const int n = 6;
+// Eigen 3.3 LDLT restated (diagonal pivoting, pseudo-inverse of D), solving
+// A x = y in place.  Fully unrolled on register arrays: the pivot row is a
+// run-time value, so swaps are predicated over the candidate rows, keeping
+// every index static; the arithmetic is the textbook sequence in order.
+template <class T>
+__device__ inline void swap_d(T& a, T& b) {
+  const T t = a;
+  a = b;
+  b = t;
+}
+__device__ inline void ldlt_solve6(double (&A)[36], double (&y)[6]) {
+  constexpr int n = 6;
+  int tr[n];
+  bool stop = false;
+#pragma unroll
   for (int k = 0; k < n; ++k) {
+    if (stop) continue;
     int big = k;
     double bv = fabs(A[k * n + k]);
+#pragma unroll
     for (int i = k + 1; i < n; ++i)
-      if (fabs(A[i * n + i]) > bv) { bv = fabs(A[i * n + i]); big = i; }
-    tr[k] = big;
-    if (big != k) {
-      for (int j = 0; j < k; ++j) { double t = A[k * n + j]; A[k * n + j] = A[big * n + j]; A[big * n + j] = t; }
-      for (int i = big + 1; i < n; ++i) { double t = A[i * n + k]; A[i * n + k] = A[i * n + big]; A[i * n + big] = t; }
-      { double t = A[k * n + k]; A[k * n + k] = A[big * n + big]; A[big * n + big] = t; }
-      for (int i = k + 1; i < big; ++i) {
-        double t = A[i * n + k];
-        A[i * n + k] = A[big * n + i];
-        A[big * n + i] = t;
+      if (fabs(A[i * n + i]) > bv) {
+        bv = fabs(A[i * n + i]);
+        big = i;
       }
+    tr[k] = big;
+#pragma unroll
+    for (int b = k + 1; b < n; ++b) {
+      if (b != big) continue;
+#pragma unroll
+      for (int j = 0; j < k; ++j) swap_d(A[k * n + j], A[b * n + j]);
+#pragma unroll
+      for (int i = b + 1; i < n; ++i) swap_d(A[i * n + k], A[i * n + b]);
+      swap_d(A[k * n + k], A[b * n + b]);
+#pragma unroll
+      for (int i = k + 1; i < b; ++i) swap_d(A[i * n + k], A[b * n + i]);
     }
-    double temp[6];
     if (k > 0) {
+      double temp[n];
+#pragma unroll
       for (int j = 0; j < k; ++j) temp[j] = A[j * n + j] * A[k * n + j];
       double s = 0.0;
+#pragma unroll
       for (int j = 0; j < k; ++j) s += A[k * n + j] * temp[j];
       A[k * n + k] -= s;
+#pragma unroll
       for (int i = k + 1; i < n; ++i) {
         double r = 0.0;
+#pragma unroll
         for (int j = 0; j < k; ++j) r += A[i * n + j] * temp[j];
         A[i * n + k] -= r;
       }
     }
-    double akk = A[k * n + k];
-    bool valid = fabs(akk) > 0.0;
+    const double akk = A[k * n + k];
+    const bool valid = fabs(akk) > 0.0;
     if (k == 0 && !valid) {
+#pragma unroll
       for (int j = 0; j < n; ++j) tr[j] = j;
-      break;
+      stop = true;
+      continue;
     }
-    if (valid)
+    if (valid) {
+#pragma unroll
       for (int i = k + 1; i < n; ++i) A[i * n + k] /= akk;
+    }
   }
-  for (int k = 0; k < n; ++k) { double t = y[k]; y[k] = y[tr[k]]; y[tr[k]] = t; }
+#pragma unroll
+  for (int k = 0; k < n; ++k)
+#pragma unroll
+    for (int m = k; m < n; ++m)
+      if (m == tr[k] && m != k) swap_d(y[k], y[m]);
+#pragma unroll
   for (int i = 0; i < n; ++i)
+#pragma unroll
     for (int j = 0; j < i; ++j) y[i] -= A[i * n + j] * y[j];
+#pragma unroll
   for (int i = 0; i < n; ++i) {
-    double d = A[i * n + i];
+    const double d = A[i * n + i];
     y[i] = fabs(d) > DBL_MIN ? y[i] / d : 0.0;
   }
+#pragma unroll
   for (int i = n - 1; i >= 0; --i)
+#pragma unroll
     for (int j = i + 1; j < n; ++j) y[i] -= A[j * n + i] * y[j];
-  for (int k = n - 1; k >= 0; --k) { double t = y[k]; y[k] = y[tr[k]]; y[tr[k]] = t; }
+#pragma unroll
+  for (int k = n - 1; k >= 0; --k)
+#pragma unroll
+    for (int m = k; m < n; ++m)
+      if (m == tr[k] && m != k) swap_d(y[k], y[m]);
 }
 
 // SE(3) exponential, twist (rho, omega), Rodrigues' formula.
@@ -571,9 +610,6 @@ __device__ inline void update_at(TrackState* st, const float* part_icp, int gi, 
                                  float wib) {
   __shared__ float lds[4 * 32];
   __shared__ float sums[2][32];
-  __shared__ double A[36];
-  __shared__ double y[6];
-  __shared__ int tr[6];
   for (int which = 0; which < 2; ++which) {
     const float* part = which == 0 ? part_icp : part_rgb;
     const int g = which == 0 ? gi : gr;
@@ -628,11 +664,12 @@ __device__ inline void update_at(TrackState* st, const float* part_icp, int gi, 
     rsum = sums[0][27];
   }
   const float err = sqrtf(rsum) / (nres + 6.0f);
+  double A[36], y[6];
 #pragma unroll
   for (int i = 0; i < 36; ++i) A[i] = (double)H[i];
 #pragma unroll
   for (int i = 0; i < 6; ++i) y[i] = (double)b[i];
-  ldlt_solve6(A, y, tr);
+  ldlt_solve6(A, y);
   double xi[6];
 #pragma unroll
   for (int i = 0; i < 6; ++i) xi[i] = y[i];
