@@ -237,6 +237,10 @@ typedef struct {
 int yas_server_create(yas_ctx* ctx, int max_width, int max_height, int max_streams, const yas_tracker_params* tp,
                       int voxel_format, size_t hbm_budget_bytes, yas_server** out);
 int yas_server_destroy(yas_server* sv);
+/* Frames per batch, 1..64 (default 64).  Smaller batches keep the
+ * tracker's per-level images of the batch resident in the last-level cache
+ * across Gauss-Newton iterations. */
+int yas_server_set_max_batch(yas_server* sv, int max_batch);
 /* Admit a stream (its volume, ray caster, tracker and pose chain are
  * allocated now); YAS_E_CAPACITY when its buffers would exceed the budget or
  * all stream slots are taken. Synchronous. */
@@ -352,7 +356,8 @@ int yas_tracker_step_sums(yas_tracker* trk, int level, const double T[16], int k
 /* Device pointer and pitch of one pyramid image (test access):
  * which = 0 current intensity, 1 reference intensity, 2 gx, 3 gy,
  * 4 current inverse depth, 5 reference inverse depth (float);
- * 6 current vmap, 7 current nmap, 8 reference vmap, 9 reference nmap (float4). */
+ * 6 current vmap, 7 current nmap, 8 reference vmap, 9 reference nmap,
+ * 10 photometric residual rows (r, gx, gy, valid) (float4). */
 int yas_tracker_level(yas_tracker* trk, int which, int level, void** ptr, size_t* pitch, int* width,
                       int* height);
 

@@ -267,7 +267,7 @@ size_t stream_bytes(int W, int H, const yas_map_params& mp, int levels) {
   b += (size_t)W * H * 16 + (size_t)(W / 8) * (H / 8) * 8;  // vertex map, z-ranges
   size_t px = 0;
   for (int l = 0; l < levels; ++l) px += (size_t)(W >> l) * (H >> l);
-  b += px * (6 * 4 + 6 * 16);                                 // tracker pyramids
+  b += px * (6 * 4 + 5 * 16);                                 // tracker pyramids
   b += 256 * 68 * 4 + 1024;                                    // partials, state
   b += (size_t)W * H * 8 + 4096 + 1024 * (16 * 8 + 4 + 32);    // inputs, pose records
   return b;
@@ -283,6 +283,7 @@ struct yas_server {
   int maxW, maxH;
   size_t budget, used;
   int max_streams;
+  int max_batch;  // frames per batch (<= kMaxSlots)
   std::vector<HostStream> streams;  // slot per stream id (pipe == nullptr: free)
   StreamDev* dev_streams;
   Table* dev_table;
@@ -294,6 +295,7 @@ struct yas_server {
   std::deque<yas_result> done;
   std::deque<int> launch_order;  // batches in flight, oldest first
   long long next_request;
+  int rr;  // stream the next batch starts looking at
   long long frames_done, batches_done, slots_done;
 };
 
@@ -433,9 +435,11 @@ int yas_server_create(yas_ctx* ctx, int max_width, int max_height, int max_strea
   sv->budget = hbm_budget_bytes;
   sv->used = 0;
   sv->max_streams = max_streams;
+  sv->max_batch = kMaxSlots;
   sv->streams.resize(max_streams);
   sv->next_batch = 0;
   sv->next_request = 0;
+  sv->rr = 0;
   sv->frames_done = sv->batches_done = sv->slots_done = 0;
   int rc = yas_ctx_create(ctx->device, nullptr, &sv->ctx);
   if (rc) {
@@ -547,6 +551,12 @@ int yas_server_stream_map(yas_server* sv, int stream_id, yas_map** map) {
   return YAS_OK;
 }
 
+int yas_server_set_max_batch(yas_server* sv, int max_batch) {
+  YAS_REQUIRE(sv && max_batch >= 1 && max_batch <= kMaxSlots, "yas_server_set_max_batch: 1..64");
+  sv->max_batch = max_batch;
+  return YAS_OK;
+}
+
 int yas_server_stream_pipeline(yas_server* sv, int stream_id, yas_pipeline** pipe) {
   YAS_REQUIRE(sv && pipe && stream_id >= 0 && stream_id < sv->max_streams && sv->streams[stream_id].pipe,
               "yas_server_stream_pipeline: no such stream");
@@ -574,7 +584,9 @@ int yas_server_pump(yas_server* sv, int* launched) {
     Batch& b = sv->batches[sv->next_batch];
     if (b.busy) break;  // kInFlight batches queued already
     int count = 0;
-    for (int i = 0; i < sv->max_streams && count < kMaxSlots; ++i) {
+    // round-robin start, so a batch cap does not favour low stream ids
+    for (int j = 0; j < sv->max_streams && count < sv->max_batch; ++j) {
+      const int i = (sv->rr + j) % sv->max_streams;
       HostStream& hs = sv->streams[i];
       if (!hs.pipe || hs.waiting.empty()) continue;
       const Request r = hs.waiting.front();
@@ -594,6 +606,7 @@ int yas_server_pump(yas_server* sv, int* launched) {
       ++count;
     }
     if (count == 0) break;
+    sv->rr = (b.stream_of[count - 1] + 1) % sv->max_streams;
     b.staged->count = count;
     b.count = count;
     hipGraphExec_t ex = nullptr;

@@ -108,7 +108,7 @@ struct yas_tracker {
   float *cur_int[kMaxLevels], *ref_int[kMaxLevels], *gx[kMaxLevels], *gy[kMaxLevels];
   float *cur_inv[kMaxLevels], *ref_inv[kMaxLevels];
   float4 *cur_v[kMaxLevels], *cur_n[kMaxLevels], *ref_v[kMaxLevels], *ref_n[kMaxLevels];
-  float4 *warped[kMaxLevels], *resid[kMaxLevels];
+  float4* resid[kMaxLevels];
   float* part_icp;  // [kRedGroups][32]
   float* part_res;  // [kRedGroups][4]
   float* part_rgb;  // [kRedGroups][32]
@@ -177,7 +177,7 @@ static TrackBufs bufs(yas_tracker* t, int l) {
   TrackBufs B;
   B.ref_v = t->ref_v[l]; B.ref_n = t->ref_n[l]; B.cur_v = t->cur_v[l]; B.cur_n = t->cur_n[l];
   B.ref_inv = t->ref_inv[l]; B.ref_int = t->ref_int[l]; B.cur_int = t->cur_int[l];
-  B.gx = t->gx[l]; B.gy = t->gy[l]; B.warped = t->warped[l]; B.resid = t->resid[l];
+  B.gx = t->gx[l]; B.gy = t->gy[l]; B.resid = t->resid[l];
   return B;
 }
 
@@ -298,7 +298,7 @@ int yas_tracker_create(yas_ctx* ctx, int W, int H, const double K[4], const yas_
     size_t n = (size_t)a.w * a.h;
     for (float** p : {&t->cur_int[l], &t->ref_int[l], &t->gx[l], &t->gy[l], &t->cur_inv[l], &t->ref_inv[l]})
       alloc((void**)p, n * sizeof(float));
-    for (float4** p : {&t->cur_v[l], &t->cur_n[l], &t->ref_v[l], &t->ref_n[l], &t->warped[l], &t->resid[l]})
+    for (float4** p : {&t->cur_v[l], &t->cur_n[l], &t->ref_v[l], &t->ref_n[l], &t->resid[l]})
       alloc((void**)p, n * sizeof(float4));
   }
   alloc((void**)&t->part_icp, sizeof(float) * 32 * kRedGroups);
@@ -316,7 +316,7 @@ int yas_tracker_create(yas_ctx* ctx, int W, int H, const double K[4], const yas_
     size_t n = (size_t)t->lv[l].w * t->lv[l].h;
     for (float* p : {t->cur_int[l], t->ref_int[l], t->gx[l], t->gy[l], t->cur_inv[l], t->ref_inv[l]})
       YAS_HIP(hipMemsetAsync(p, 0, n * sizeof(float), ctx->stream));
-    for (float4* p : {t->cur_v[l], t->cur_n[l], t->ref_v[l], t->ref_n[l], t->warped[l], t->resid[l]})
+    for (float4* p : {t->cur_v[l], t->cur_n[l], t->ref_v[l], t->ref_n[l], t->resid[l]})
       YAS_HIP(hipMemsetAsync(p, 0, n * sizeof(float4), ctx->stream));
   }
   YAS_HIP(hipMemsetAsync(t->st, 0, sizeof(TrackState), ctx->stream));
@@ -335,7 +335,7 @@ int yas_tracker_destroy(yas_tracker* t) {
   hipStreamSynchronize(t->ctx->stream);
   for (int l = 0; l < kMaxLevels; ++l) {
     for (float* p : {t->cur_int[l], t->ref_int[l], t->gx[l], t->gy[l], t->cur_inv[l], t->ref_inv[l]}) hipFree(p);
-    for (float4* p : {t->cur_v[l], t->cur_n[l], t->ref_v[l], t->ref_n[l], t->warped[l], t->resid[l]}) hipFree(p);
+    for (float4* p : {t->cur_v[l], t->cur_n[l], t->ref_v[l], t->ref_n[l], t->resid[l]}) hipFree(p);
   }
   hipFree(t->part_icp);
   hipFree(t->part_res);
@@ -438,8 +438,8 @@ int yas_tracker_level(yas_tracker* t, int which, int level, void** ptr, size_t* 
   YAS_REQUIRE(level >= 0 && level < t->P.levels, "bad level");
   const Level& a = t->lv[level];
   void* one[6] = {t->cur_int[level], t->ref_int[level], t->gx[level], t->gy[level], t->cur_inv[level], t->ref_inv[level]};
-  void* four[6] = {t->cur_v[level], t->cur_n[level], t->ref_v[level], t->ref_n[level], t->warped[level], t->resid[level]};
-  YAS_REQUIRE(which >= 0 && which < 12, "bad buffer id");
+  void* four[5] = {t->cur_v[level], t->cur_n[level], t->ref_v[level], t->ref_n[level], t->resid[level]};
+  YAS_REQUIRE(which >= 0 && which < 11, "bad buffer id");
   *ptr = which < 6 ? one[which] : four[which - 6];
   if (pitch) *pitch = (size_t)a.w * (which < 6 ? 4 : 16);
   if (w) *w = a.w;

@@ -281,7 +281,6 @@ struct TrackBufs {
   const float* cur_int;
   const float* gx;
   const float* gy;
-  float4* warped;
   float4* resid;
 };
 
@@ -345,10 +344,10 @@ __device__ inline float bilerp(const float* m, int w, float x, float y) {
   return (r0[u] * (1.0f - cx) + r0[u + 1] * cx) * (1.0f - cy) + (r1[u] * (1.0f - cx) + r1[u + 1] * cx) * cy;
 }
 
-// TrackingUtils.h:73-151 with the warp of ImageProc.cu:42-61 fused in; the
-// residual statistics are accumulated with Welford's update.
-__device__ inline void rgb_pixel(const Level& L, const TrackBufs& B, const float* wp, int k, float& n, float& mean,
-                                 float& m2) {
+// ImageProc.cu:42-61: reference pixel k at its inverse depth, moved by the
+// warp (RK, t); w = -1 where the inverse depth is 0.  Both passes evaluate it
+// (the same operations, so the same floats) instead of storing it.
+__device__ inline float4 warp_ref(const Level& L, const TrackBufs& B, const float* wp, int k) {
   const int y = k / L.w, x = k - y * L.w;
   const float iz = B.ref_inv[k];
   float4 p = make_float4(0.f, 0.f, 0.f, -1.f);
@@ -359,7 +358,14 @@ __device__ inline void rgb_pixel(const Level& L, const TrackBufs& B, const float
     float pz = (wp[6] * (float)x + wp[7] * (float)y + wp[8]) * z + wp[11];
     p = make_float4(px, py, pz, 1.0f);
   }
-  B.warped[k] = p;
+  return p;
+}
+
+// TrackingUtils.h:73-151 with the warp of ImageProc.cu:42-61 fused in; the
+// residual statistics are accumulated with Welford's update.
+__device__ inline void rgb_pixel(const Level& L, const TrackBufs& B, const float* wp, int k, float& n, float& mean,
+                                 float& m2) {
+  const float4 p = warp_ref(L, B, wp, k);
   float r = 0.0f, gx = 0.0f, gy = 0.0f;
   bool ok = false;
   if (p.w > 0.0f) {
@@ -426,6 +432,8 @@ __device__ inline void pass1_at(const Level& L, const TrackBufs& B, const TrackS
 __device__ inline void pass2_at(const Level& L, const TrackBufs& B, TrackState* st, const float* part_res, int g1,
                                 float* part_rgb, int bx, int nb) {
   __shared__ float lds[4 * 32];
+  __shared__ float wp[12];
+  load_warp(st->est, L.Kd, wp);
   float n = 0.0f, mean = 0.0f, m2 = 0.0f;
   for (int g = threadIdx.x; g < g1; g += blockDim.x)
     chan_merge(n, mean, m2, part_res[g * 4], part_res[g * 4 + 1], part_res[g * 4 + 2]);
@@ -445,7 +453,7 @@ __device__ inline void pass2_at(const Level& L, const TrackBufs& B, TrackState* 
   for (int k = bx * blockDim.x + threadIdx.x; k < N; k += nb * blockDim.x) {
     const float4 e = B.resid[k];
     if (!(e.w > 0.0f)) continue;
-    const float4 p = B.warped[k];
+    const float4 p = warp_ref(L, B, wp, k);
     const float zinv = 1.0f / p.z;
     float wt = 1.0f;
     if (fabsf(e.x) > th) wt = th / fabsf(e.x);
