@@ -214,7 +214,7 @@ def batch_bench(dev_depth, nseq, n_maps=64, seed=1):
     return out
 
 
-def serving_bench(dev_depth, dev_rgb, nseq, n_streams=32, seed=2, frames_sat=60, duration_s=4.0):
+def serving_bench(dev_depth, dev_rgb, nseq, n_streams=32, seed=2, frames_sat=60, duration_s=4.0, runs=None):
     """Serving form: 32 streams, each a segment of the room sequence starting
     at its own seeded frame (so its own start angle) at 640x480 or 320x240
     (seeded), voxel 5 mm, 2^20 buckets and a 2^16-block pool per stream, f32
@@ -257,6 +257,8 @@ def serving_bench(dev_depth, dev_rgb, nseq, n_streams=32, seed=2, frames_sat=60,
     for name, hires, fmt in (("saturation_640x480", np.ones(n_streams, bool), rf.VOXEL_F32_F32),
                              ("saturation_640x480_f16u16", np.ones(n_streams, bool), rf.VOXEL_F16_U16),
                              ("saturation", mixed, rf.VOXEL_F32_F32)):
+        if runs and name not in runs:
+            continue
         hi[:] = hires
         sv, ids = new_server(fmt)
         torch.cuda.synchronize()
@@ -274,6 +276,8 @@ def serving_bench(dev_depth, dev_rgb, nseq, n_streams=32, seed=2, frames_sat=60,
                      "hbm_bytes_in_use": int(st["bytes_in_use"])}
         sv.close()
     hi[:] = mixed
+    if runs and "poisson" not in runs:
+        return out
     # Poisson arrivals
     rate = rng.uniform(15.0, 30.0, n_streams)
     arrivals = []
@@ -342,6 +346,8 @@ def main():
     ap.add_argument("--no-mesh", action="store_true", help="skip the marching-cubes measurement")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-submap measurement")
     ap.add_argument("--no-serving", action="store_true", help="skip the serving measurement")
+    ap.add_argument("--serving-runs", default="",
+                    help="comma-separated subset of saturation_640x480, saturation_640x480_f16u16, saturation, poisson")
     args = ap.parse_args()
     if args.copy_probe:
         copy_probe()
@@ -431,8 +437,11 @@ def main():
     }
     if not args.no_serving:
         log("serving form: 32 streams")
-        result["serving"] = serving_bench(dev_depth, dev_rgb, npool)
+        runs = set(args.serving_runs.split(",")) if args.serving_runs else None
+        result["serving"] = serving_bench(dev_depth, dev_rgb, npool, runs=runs)
         for key in ("saturation_640x480", "saturation_640x480_f16u16", "saturation"):
+            if key not in result["serving"]:
+                continue
             sat = result["serving"][key]["frames_per_s"]
             result["serving"][key]["vs_plain_single_stream"] = round(sat / fps, 2)
         log("serving:", result["serving"])
