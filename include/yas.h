@@ -241,6 +241,9 @@ int yas_server_destroy(yas_server* sv);
  * tracker's per-level images of the batch resident in the last-level cache
  * across Gauss-Newton iterations. */
 int yas_server_set_max_batch(yas_server* sv, int max_batch);
+/* Run the Gauss-Newton iterations on sub-ranges of `slots` batch slots in
+ * turn (0, the default: the whole batch at once).  Results are unchanged. */
+int yas_server_set_gn_group(yas_server* sv, int slots);
 /* Admit a stream (its volume, ray caster, tracker and pose chain are
  * allocated now); YAS_E_CAPACITY when its buffers would exceed the budget or
  * all stream slots are taken. Synchronous. */

@@ -59,9 +59,12 @@ struct Table {
   Slot slot[kMaxSlots];
 };
 
-#define SLOT_OR_RETURN                                    \
-  if ((int)blockIdx.z >= tab->count) return;              \
-  const Slot& sl = tab->slot[blockIdx.z];                 \
+#define SLOT_OR_RETURN SLOT_FROM_OR_RETURN(0)
+// Slot slot0 + blockIdx.z (the Gauss-Newton kernels can run on a sub-range
+// of the batch's slots).
+#define SLOT_FROM_OR_RETURN(slot0)                                    \
+  if ((int)blockIdx.z + (slot0) >= tab->count) return;                \
+  const Slot& sl = tab->slot[blockIdx.z + (slot0)];                   \
   const StreamDev& S = streams[sl.stream]
 
 // ------------------------------------------------------------------ kernels
@@ -170,29 +173,29 @@ __global__ void __launch_bounds__(256) k_s_image(const StreamDev* streams, const
   }
 }
 
-__global__ void k_s_track_init(const StreamDev* streams, const Table* tab) {
-  SLOT_OR_RETURN;
+__global__ void k_s_track_init(const StreamDev* streams, const Table* tab, int slot0) {
+  SLOT_FROM_OR_RETURN(slot0);
   if (threadIdx.x != 0 || sl.first) return;
   track_init_identity_at(S.st);
 }
 
 template <bool ICP, bool RGB>
-__global__ void __launch_bounds__(256) k_s_pass1(const StreamDev* streams, const Table* tab, int l) {
-  SLOT_OR_RETURN;
+__global__ void __launch_bounds__(256) k_s_pass1(const StreamDev* streams, const Table* tab, int l, int slot0) {
+  SLOT_FROM_OR_RETURN(slot0);
   if (sl.first || (int)blockIdx.x >= S.groups[l]) return;
   pass1_at<ICP, RGB>(S.lv[l], S.tb[l], S.st, S.part_icp, S.part_res, S.angle_th, S.dist_th, blockIdx.x, S.groups[l]);
 }
 
-__global__ void __launch_bounds__(256) k_s_pass2(const StreamDev* streams, const Table* tab, int l) {
-  SLOT_OR_RETURN;
+__global__ void __launch_bounds__(256) k_s_pass2(const StreamDev* streams, const Table* tab, int l, int slot0) {
+  SLOT_FROM_OR_RETURN(slot0);
   if (sl.first || (int)blockIdx.x >= S.groups[l]) return;
   pass2_at(S.lv[l], S.tb[l], S.st, S.part_res, S.groups[l], S.part_rgb, blockIdx.x, S.groups[l]);
 }
 
 __global__ void __launch_bounds__(256) k_s_update(const StreamDev* streams, const Table* tab, int l, int icp, int rgb,
                                                   int mode, int first_of_level, float wrh, float wrb, float wih,
-                                                  float wib) {
-  SLOT_OR_RETURN;
+                                                  float wib, int slot0) {
+  SLOT_FROM_OR_RETURN(slot0);
   if (sl.first) return;
   update_at(S.st, S.part_icp, icp ? S.groups[l] : 0, S.part_rgb, rgb ? S.groups[l] : 0, mode, first_of_level, 1,
             l == 0 ? 1 : 0, wrh, wrb, wih, wib);
@@ -284,6 +287,7 @@ struct yas_server {
   size_t budget, used;
   int max_streams;
   int max_batch;  // frames per batch (<= kMaxSlots)
+  int gn_group;   // slots per Gauss-Newton sub-range (0: the whole batch)
   std::vector<HostStream> streams;  // slot per stream id (pipe == nullptr: free)
   StreamDev* dev_streams;
   Table* dev_table;
@@ -334,17 +338,24 @@ static int launch_batch_kernels(yas_server* sv, hipStream_t s, int nz) {
     hipLaunchKernelGGL(k_s_image, level_grid(l), dim3(256), 0, s, S, T, 7, l);
     hipLaunchKernelGGL(k_s_image, level_grid(l), dim3(256), 0, s, S, T, 8, l);
   }
-  // Gauss-Newton, coarse to fine
-  hipLaunchKernelGGL(k_s_track_init, dim3(1, 1, nz), dim3(64), 0, s, S, T);
-  for (int l = L - 1; l >= 0; --l) {
-    const int G = std::max(1, std::min(256, ((W >> l) * (H >> l) + 1023) / 1024));
-    for (int it = 0; it < sv->tp.iters[l]; ++it) {
-      if (icp && rgb) hipLaunchKernelGGL((k_s_pass1<true, true>), dim3(G, 1, nz), dim3(256), 0, s, S, T, l);
-      else if (icp) hipLaunchKernelGGL((k_s_pass1<true, false>), dim3(G, 1, nz), dim3(256), 0, s, S, T, l);
-      else hipLaunchKernelGGL((k_s_pass1<false, true>), dim3(G, 1, nz), dim3(256), 0, s, S, T, l);
-      if (rgb) hipLaunchKernelGGL(k_s_pass2, dim3(G, 1, nz), dim3(256), 0, s, S, T, l);
-      hipLaunchKernelGGL(k_s_update, dim3(1, 1, nz), dim3(256), 0, s, S, T, l, icp ? 1 : 0, rgb ? 1 : 0, sv->tp.mode,
-                         it == 0 ? 1 : 0, sv->tp.w_rgb_h, sv->tp.w_rgb_b, sv->tp.w_icp_h, sv->tp.w_icp_b);
+  // Gauss-Newton, coarse to fine; optionally on sub-ranges of the slots in
+  // turn, so one sub-range's pyramids stay in the last-level cache across
+  // the iterations
+  const int gg = sv->gn_group > 0 ? std::min(sv->gn_group, nz) : nz;
+  for (int s0 = 0; s0 < nz; s0 += gg) {
+    const int gz = std::min(gg, nz - s0);
+    hipLaunchKernelGGL(k_s_track_init, dim3(1, 1, gz), dim3(64), 0, s, S, T, s0);
+    for (int l = L - 1; l >= 0; --l) {
+      const int G = std::max(1, std::min(256, ((W >> l) * (H >> l) + 1023) / 1024));
+      for (int it = 0; it < sv->tp.iters[l]; ++it) {
+        if (icp && rgb) hipLaunchKernelGGL((k_s_pass1<true, true>), dim3(G, 1, gz), dim3(256), 0, s, S, T, l, s0);
+        else if (icp) hipLaunchKernelGGL((k_s_pass1<true, false>), dim3(G, 1, gz), dim3(256), 0, s, S, T, l, s0);
+        else hipLaunchKernelGGL((k_s_pass1<false, true>), dim3(G, 1, gz), dim3(256), 0, s, S, T, l, s0);
+        if (rgb) hipLaunchKernelGGL(k_s_pass2, dim3(G, 1, gz), dim3(256), 0, s, S, T, l, s0);
+        hipLaunchKernelGGL(k_s_update, dim3(1, 1, gz), dim3(256), 0, s, S, T, l, icp ? 1 : 0, rgb ? 1 : 0,
+                           sv->tp.mode, it == 0 ? 1 : 0, sv->tp.w_rgb_h, sv->tp.w_rgb_b, sv->tp.w_icp_h,
+                           sv->tp.w_icp_b, s0);
+      }
     }
   }
   // gate, pose chain, result record
@@ -436,6 +447,7 @@ int yas_server_create(yas_ctx* ctx, int max_width, int max_height, int max_strea
   sv->used = 0;
   sv->max_streams = max_streams;
   sv->max_batch = kMaxSlots;
+  sv->gn_group = 0;
   sv->streams.resize(max_streams);
   sv->next_batch = 0;
   sv->next_request = 0;
@@ -554,6 +566,18 @@ int yas_server_stream_map(yas_server* sv, int stream_id, yas_map** map) {
 int yas_server_set_max_batch(yas_server* sv, int max_batch) {
   YAS_REQUIRE(sv && max_batch >= 1 && max_batch <= kMaxSlots, "yas_server_set_max_batch: 1..64");
   sv->max_batch = max_batch;
+  return YAS_OK;
+}
+
+int yas_server_set_gn_group(yas_server* sv, int slots) {
+  YAS_REQUIRE(sv && slots >= 0 && slots <= kMaxSlots, "yas_server_set_gn_group: 0..64");
+  if (slots == sv->gn_group) return YAS_OK;
+  YAS_HIP(hipStreamSynchronize(sv->ctx->stream));
+  for (auto& kv : sv->graphs) hipGraphExecDestroy(kv.second);
+  for (hipGraph_t g : sv->graph_defs) hipGraphDestroy(g);
+  sv->graphs.clear();
+  sv->graph_defs.clear();
+  sv->gn_group = slots;
   return YAS_OK;
 }
 

@@ -254,6 +254,9 @@ class Server:
         self._dims[sid.value] = (H, W)
         return sid.value
 
+    def set_gn_group(self, n):
+        call("yas_server_set_gn_group", self.h, n)
+
     def set_max_batch(self, n):
         call("yas_server_set_max_batch", self.h, n)
 
