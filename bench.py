@@ -69,6 +69,13 @@ def cpu_baseline(p, K, depth, rgb, budget_s=20.0, max_frames=40):
     return n / dt, n, dt
 
 
+# HBM bytes per launch of the main loop's kernels on this workload, from
+# rocprofv3 FETCH_SIZE / WRITE_SIZE passes scaled by a plain copy of known
+# size (profiles/dense_640x480_5mm_hbm_traffic.md).  The profiler cannot run
+# inside this script, so the measured figures are carried here.
+TRAFFIC_BYTES = {"k_march": 56.0e6, "k_integrate": 109.5e6, "k_alloc": 4.5e6}
+
+
 def walk_steps(stats):
     """Bresenham block visits over the timed frames, from the cumulative
     uint32 counter (frame 0's visits are estimated by the mean of the rest)."""
@@ -101,7 +108,7 @@ def rooflines(stats, stage_ms, steps):
         t = stage_ms[i] / 1e3
         ach = nbytes / t / 1e9 if t > 0 else 0.0
         out[name] = {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
+                     "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": TRAFFIC_BYTES.get(name),
                      "algorithmic_bytes_per_launch": round(nbytes / steps),
                      "avg_launch_ms": round(stage_ms[i] / steps, 5)}
     return out
