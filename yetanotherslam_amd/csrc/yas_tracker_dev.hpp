@@ -618,6 +618,13 @@ __device__ inline void update_at(TrackState* st, const float* part_icp, int gi, 
                                  float wib) {
   __shared__ float lds[4 * 32];
   __shared__ float sums[2][32];
+  __shared__ double s_est[16];
+  __shared__ int s_good;
+  __shared__ float s_last_err;
+  // the state the step reads, loaded alongside the partial rows
+  if (threadIdx.x < 16) s_est[threadIdx.x] = st->est[threadIdx.x];
+  if (threadIdx.x == 16) s_good = st->good;
+  if (threadIdx.x == 17) s_last_err = st->last_err;
   for (int which = 0; which < 2; ++which) {
     const float* part = which == 0 ? part_icp : part_rgb;
     const int g = which == 0 ? gi : gr;
@@ -632,18 +639,20 @@ __device__ inline void update_at(TrackState* st, const float* part_icp, int gi, 
     if (threadIdx.x < 32) sums[which][threadIdx.x] = t;
   }
   __syncthreads();
+  if (threadIdx.x < 29) {
+    if (gi) st->sums_icp[threadIdx.x] = sums[0][threadIdx.x];
+    if (gr) st->sums_rgb[threadIdx.x] = sums[1][threadIdx.x];
+  }
   if (threadIdx.x != 0) return;
-  if (gi) {
-    for (int i = 0; i < 29; ++i) st->sums_icp[i] = sums[0][i];
-    if (record) st->icp_inliers = (int)sums[0][28];
-  }
-  if (gr) {
-    for (int i = 0; i < 29; ++i) st->sums_rgb[i] = sums[1][i];
-    if (record) st->rgb_inliers = (int)sums[1][28];
-  }
+  if (record && gi) st->icp_inliers = (int)sums[0][28];
+  if (record && gr) st->rgb_inliers = (int)sums[1][28];
   if (!do_step) return;
-  if (first_of_level) st->last_err = FLT_MAX;
-  if (!st->good) return;
+  float last_err = s_last_err;
+  if (first_of_level) {
+    last_err = FLT_MAX;
+    st->last_err = FLT_MAX;
+  }
+  if (!s_good) return;
   float H[36], b[6];
 #pragma unroll
   for (int i = 0; i < 36; ++i) H[i] = 0.0f;
@@ -688,11 +697,11 @@ __device__ inline void update_at(TrackState* st, const float* part_icp, int gi, 
   double E[16], est[16];
   se3_exp(xi, E);
 #pragma unroll
-  for (int i = 0; i < 16; ++i) est[i] = st->est[i];
+  for (int i = 0; i < 16; ++i) est[i] = s_est[i];
   mul44(E, est, est);
 #pragma unroll
   for (int i = 0; i < 16; ++i) st->est[i] = est[i];
-  if (err < st->last_err) {
+  if (err < last_err) {
 #pragma unroll
     for (int i = 0; i < 16; ++i) st->last_ok[i] = est[i];
     st->last_err = err;
