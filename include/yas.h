@@ -217,7 +217,7 @@ int yas_batch_get_config(yas_batch* b, yas_batch_config* config);
  * replay of a captured graph over all its streams, and batches queue back to
  * back on the server's own HIP stream.  A stream's results are identical
  * alone and in any mix.  Streams are admitted while their device buffers fit
- * the HBM budget. */
+ * the HBM budget.  A server is driven from one host thread. */
 typedef struct yas_server yas_server;
 typedef struct {
   int stream;          /* stream id */

@@ -354,10 +354,17 @@ int yas_map_reserve(yas_map* m, int num_buckets, int num_blocks) {
     YAS_HIP(hipMemcpyAsync(nv.block_keys, old.block_keys, sizeof(unsigned long long) * n, hipMemcpyDeviceToDevice, s));
     YAS_HIP(hipMemcpyAsync(nv.voxels, old.voxels, vb * 512 * (size_t)n, hipMemcpyDeviceToDevice, s));
   }
-  if ((rc = rebuild(m, nv, flags, n))) return rc;
-  YAS_HIP(hipStreamSynchronize(s));
-  free_device_arrays(old);
+  if ((rc = rebuild(m, nv, flags, n)) == YAS_OK && hipStreamSynchronize(s) != hipSuccess) {
+    yas_set_error("yas_map_reserve: %s", hipGetErrorString(hipGetLastError()));
+    rc = YAS_E_HIP;
+  }
   hipFree(flags);
+  if (rc) {
+    free_device_arrays(nv);
+    m->P = oldP;
+    return rc;
+  }
+  free_device_arrays(old);
   m->v = nv;
   return YAS_OK;
 }
@@ -491,9 +498,15 @@ int yas_map_reactivate(yas_map* m) {
     YAS_HIP(hipMemcpyAsync(flags, m->hib_flags, sizeof(int) * n, hipMemcpyHostToDevice, s));
     YAS_HIP(hipMemcpyAsync(nv.voxels, m->hib_voxels, vb * 512 * (size_t)n, hipMemcpyHostToDevice, s));
   }
-  if ((rc = rebuild(m, nv, flags, n))) return rc;
-  YAS_HIP(hipStreamSynchronize(s));
+  if ((rc = rebuild(m, nv, flags, n)) == YAS_OK && hipStreamSynchronize(s) != hipSuccess) {
+    yas_set_error("yas_map_reactivate: %s", hipGetErrorString(hipGetLastError()));
+    rc = YAS_E_HIP;
+  }
   hipFree(flags);
+  if (rc) {
+    free_device_arrays(nv);  // still hibernating; the host copy is intact
+    return rc;
+  }
   hipHostFree(m->hib_keys);
   hipHostFree(m->hib_flags);
   hipHostFree(m->hib_voxels);

@@ -607,6 +607,13 @@ int yas_server_pump(yas_server* sv, int* launched) {
   for (;;) {
     Batch& b = sv->batches[sv->next_batch];
     if (b.busy) break;  // kInFlight batches queued already
+    // size of the next batch, and its graph, before any frame leaves its queue
+    int want = 0;
+    for (int i = 0; i < sv->max_streams && want < sv->max_batch; ++i)
+      if (sv->streams[i].pipe && !sv->streams[i].waiting.empty()) ++want;
+    if (want == 0) break;
+    hipGraphExec_t ex = nullptr;
+    if ((rc = graph_for(sv, want, &ex))) return rc;
     int count = 0;
     // round-robin start, so a batch cap does not favour low stream ids
     for (int j = 0; j < sv->max_streams && count < sv->max_batch; ++j) {
@@ -629,12 +636,9 @@ int yas_server_pump(yas_server* sv, int* launched) {
       hs.in_flight++;
       ++count;
     }
-    if (count == 0) break;
     sv->rr = (b.stream_of[count - 1] + 1) % sv->max_streams;
     b.staged->count = count;
     b.count = count;
-    hipGraphExec_t ex = nullptr;
-    if ((rc = graph_for(sv, count, &ex))) return rc;
     hipStream_t s = sv->ctx->stream;
     YAS_HIP(hipMemcpyAsync(sv->dev_table, b.staged, sizeof(int) * 2 + sizeof(Slot) * count, hipMemcpyHostToDevice, s));
     YAS_HIP(hipGraphLaunch(ex, s));
