@@ -121,6 +121,12 @@ __device__ inline void block_stats(float& n, float& mean, float& m2, float* lds 
   __syncthreads();
 }
 
+// A value every lane holds alike (one LDS word read by all), moved into a
+// scalar register so that the pixel loops spend no vector register on it.
+__device__ inline float wave_uniform(float x) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_readfirstlane(__builtin_bit_cast(int, x)));
+}
+
 __device__ inline void load_pose_f(const double* T, float* lds_pose /* 12 */) {
   if (threadIdx.x < 12) {
     int i = threadIdx.x / 4, j = threadIdx.x % 4;
@@ -401,8 +407,11 @@ __device__ inline void pass1_at(const Level& L, const TrackBufs& B, const TrackS
   __syncthreads();
   PoseF T;
 #pragma unroll
-  for (int i = 0; i < 9; ++i) T.r[i] = pose[i];
-  T.t[0] = pose[9]; T.t[1] = pose[10]; T.t[2] = pose[11];
+  for (int i = 0; i < 9; ++i) T.r[i] = wave_uniform(pose[i]);
+  T.t[0] = wave_uniform(pose[9]); T.t[1] = wave_uniform(pose[10]); T.t[2] = wave_uniform(pose[11]);
+  float wu[12];
+#pragma unroll
+  for (int i = 0; i < 12; ++i) wu[i] = RGB ? wave_uniform(wp[i]) : 0.0f;
   float acc[32];
 #pragma unroll
   for (int i = 0; i < 32; ++i) acc[i] = 0.0f;
@@ -410,7 +419,7 @@ __device__ inline void pass1_at(const Level& L, const TrackBufs& B, const TrackS
   const int N = L.w * L.h;
   for (int k = bx * blockDim.x + threadIdx.x; k < N; k += nb * blockDim.x) {
     if (ICP) icp_pixel(L, B, T, angle_th, dist_th, k, acc);
-    if (RGB) rgb_pixel(L, B, wp, k, n, mean, m2);
+    if (RGB) rgb_pixel(L, B, wu, k, n, mean, m2);
   }
   if (ICP) {
     float t = block_sum32(acc, lds);
@@ -446,6 +455,9 @@ __device__ inline void pass2_at(const Level& L, const TrackBufs& B, TrackState* 
     st->rgb_stats[2] = mean;
     st->rgb_stats[3] = th;
   }
+  float wu[12];
+#pragma unroll
+  for (int i = 0; i < 12; ++i) wu[i] = wave_uniform(wp[i]);
   float acc[32];
 #pragma unroll
   for (int i = 0; i < 32; ++i) acc[i] = 0.0f;
@@ -453,7 +465,7 @@ __device__ inline void pass2_at(const Level& L, const TrackBufs& B, TrackState* 
   for (int k = bx * blockDim.x + threadIdx.x; k < N; k += nb * blockDim.x) {
     const float4 e = B.resid[k];
     if (!(e.w > 0.0f)) continue;
-    const float4 p = warp_ref(L, B, wp, k);
+    const float4 p = warp_ref(L, B, wu, k);
     const float zinv = 1.0f / p.z;
     float wt = 1.0f;
     if (fabsf(e.x) > th) wt = th / fabsf(e.x);
