@@ -55,7 +55,7 @@ __global__ void k_bprep(const BatchItem* items, const double* poses, PosePair* p
   const int s = *cur;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  const BatchItem& it = items[(size_t)s * n + i];
+  const YAS_CONSTANT BatchItem& it = ((const YAS_CONSTANT BatchItem*)items)[(size_t)s * n + i];
   if (!it.active) return;
   PosePair p;
   pose_pair_from(poses + ((size_t)s * n + i) * 16, p);
@@ -78,7 +78,7 @@ __global__ void __launch_bounds__(256) k_balloc(const BatchItem* items, const Po
     if (ws[mid] <= b) lo = mid;
     else hi = mid - 1;
   }
-  const BatchItem& it = items[(size_t)s * n + lo];
+  const YAS_CONSTANT BatchItem& it = ((const YAS_CONSTANT BatchItem*)items)[(size_t)s * n + lo];
   constexpr int TH = 256 / TW;
   const int t = b - ws[lo];
   const int tiles_x = (it.W + TW - 1) / TW;
@@ -94,7 +94,7 @@ __global__ void __launch_bounds__(256) k_balloc(const BatchItem* items, const Po
 __global__ void __launch_bounds__(256) k_bvisible(const BatchItem* items, const PosePair* pp, const int* cur, int n) {
   const int s = *cur;
   const int i = blockIdx.y;
-  const BatchItem& it = items[(size_t)s * n + i];
+  const YAS_CONSTANT BatchItem& it = ((const YAS_CONSTANT BatchItem*)items)[(size_t)s * n + i];
   if (!it.active) return;
   visible_scan(it.m, pp[i].Tinv, it.K, it.W, it.H, blockIdx.x * blockDim.x, gridDim.x * blockDim.x);
 }
@@ -103,7 +103,7 @@ template <class VT>
 __global__ void __launch_bounds__(256) k_bintegrate(const BatchItem* items, const PosePair* pp, const int* cur, int n) {
   const int s = *cur;
   const int i = blockIdx.y;
-  const BatchItem& it = items[(size_t)s * n + i];
+  const YAS_CONSTANT BatchItem& it = ((const YAS_CONSTANT BatchItem*)items)[(size_t)s * n + i];
   if (!it.active) return;
   integrate_waves<VT>(it.m, pp[i].Tinv, it.K, DepthU16{it.depth, it.pitch, it.depth_scale}, it.W, it.H,
                       blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6), gridDim.x * (blockDim.x >> 6));

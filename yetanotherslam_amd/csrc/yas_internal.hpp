@@ -18,6 +18,16 @@
 
 #include "../../include/yas.h"
 
+// Qualifier of descriptor tables that only the host writes: kernels read them
+// through the constant address space (scalar loads), and device pointers
+// loaded from there are taken as global pointers by the compiler.  The host
+// pass of a HIP source sees the kernels too and has no such address space.
+#if defined(__HIP_DEVICE_COMPILE__)
+#define YAS_CONSTANT __attribute__((address_space(4)))
+#else
+#define YAS_CONSTANT
+#endif
+
 // ------------------------------------------------------------------ errors
 void yas_set_error(const char* fmt, ...);
 

@@ -61,11 +61,15 @@ struct Table {
 
 #define SLOT_OR_RETURN SLOT_FROM_OR_RETURN(0)
 // Slot slot0 + blockIdx.z (the Gauss-Newton kernels can run on a sub-range
-// of the batch's slots).
-#define SLOT_FROM_OR_RETURN(slot0)                                    \
-  if ((int)blockIdx.z + (slot0) >= tab->count) return;                \
-  const Slot& sl = tab->slot[blockIdx.z + (slot0)];                   \
-  const StreamDev& S = streams[sl.stream]
+// of the batch's slots).  The table and the stream descriptors are written
+// by the host only, so the kernels read them as constant memory (scalar
+// loads); the device pointers loaded from there are then known to be
+// global, which gives global rather than flat loads and stores.
+#define SLOT_FROM_OR_RETURN(slot0)                                                        \
+  const YAS_CONSTANT Table& ctab = *(const YAS_CONSTANT Table*)tab;                       \
+  if ((int)blockIdx.z + (slot0) >= ctab.count) return;                                    \
+  const YAS_CONSTANT Slot& sl = ctab.slot[blockIdx.z + (slot0)];                          \
+  const YAS_CONSTANT StreamDev& S = ((const YAS_CONSTANT StreamDev*)streams)[sl.stream]
 
 // ------------------------------------------------------------------ kernels
 __global__ void __launch_bounds__(256) k_s_convert(const StreamDev* streams, const Table* tab) {

@@ -164,8 +164,9 @@ __device__ inline int reflect101(int i, int n) {
   if (i >= n) i = 2 * n - 2 - i;
   return i;
 }
-__device__ inline void pyrdown_gauss_at(const float* src, size_t sp, int sw, int sh, float* dst,
-                                                       size_t dp, int dw, int dh, int bx, int by) {
+__device__ __attribute__((always_inline)) inline void pyrdown_gauss_at(const float* src, size_t sp, int sw, int sh,
+                                                                       float* dst, size_t dp, int dw, int dh, int bx,
+                                                                       int by) {
   int x = bx * 16 + (threadIdx.x & 15), y = by * 16 + (threadIdx.x >> 4);
   if (x >= dw || y >= dh) return;
   const float k[5] = {0.0625f, 0.25f, 0.375f, 0.25f, 0.0625f};
