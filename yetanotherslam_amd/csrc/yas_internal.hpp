@@ -111,12 +111,13 @@ __host__ __device__ inline int vox_to_local(int v) {
 }
 
 // ------------------------------------------------------------------ conversions
+// Branch-free: floor, clamp into the int range, convert; 2^31 and above give
+// INT32_MAX, NaN gives 0.
 __device__ inline int floor_sat(float f) {
-  if (f != f) return 0;
-  f = floorf(f);
-  if (f <= -2147483648.0f) return INT32_MIN;
-  if (f >= 2147483648.0f) return INT32_MAX;
-  return (int)f;
+  const float g = floorf(f);
+  int i = (int)fminf(fmaxf(g, -2147483648.0f), 2147483520.0f);
+  i = g >= 2147483648.0f ? INT32_MAX : i;
+  return f != f ? 0 : i;
 }
 __device__ inline int ceil_sat(float f) {
   if (f != f) return 0;

@@ -85,8 +85,48 @@ struct VoxelReader {
   // Blocks summarised as all-unobserved or all-free answer without a voxel
   // load (the values they would return are exactly (1, invalid) / (1, valid)).
   __device__ inline float read(float x, float y, float z, bool& valid) {
-    int vx = floor_sat(x), vy = floor_sat(y), vz = floor_sat(z);
-    int bx = vox_to_block(vx), by = vox_to_block(vy), bz = vox_to_block(vz);
+    const int vx = floor_sat(x), vy = floor_sat(y), vz = floor_sat(z);
+    return read_at(block_of(vx), block_of(vy), block_of(vz),
+                   (vox_to_local(vz) * 8 + vox_to_local(vy)) * 8 + vox_to_local(vx), valid);
+  }
+  // RayTraceEngine.cu:294-329: trilinear read, valid iff all 8 reads are.
+  // The corners are floor(x) and floor(x + 1) per axis, as the reference
+  // rounds them; each is converted once and the 8 reads run in the
+  // reference's order.
+  __device__ inline float interp(float x, float y, float z, bool& valid) {
+    const float fx = x - floorf(x), fy = y - floorf(y), fz = z - floorf(z);
+    const int x0 = floor_sat(x), x1 = floor_sat(x + 1.0f);
+    const int y0 = floor_sat(y), y1 = floor_sat(y + 1.0f);
+    const int z0 = floor_sat(z), z1 = floor_sat(z + 1.0f);
+    const int bx0 = block_of(x0), bx1 = block_of(x1), by0 = block_of(y0), by1 = block_of(y1);
+    const int bz0 = block_of(z0), bz1 = block_of(z1);
+    const int lx0 = vox_to_local(x0), lx1 = vox_to_local(x1);
+    const int r00 = (vox_to_local(z0) * 8 + vox_to_local(y0)) * 8, r01 = (vox_to_local(z0) * 8 + vox_to_local(y1)) * 8;
+    const int r10 = (vox_to_local(z1) * 8 + vox_to_local(y0)) * 8, r11 = (vox_to_local(z1) * 8 + vox_to_local(y1)) * 8;
+    bool vp, v;
+    float s0 = read_at(bx0, by0, bz0, r00 + lx0, vp);
+    float s1 = read_at(bx1, by0, bz0, r00 + lx1, v); vp &= v;
+    float r0 = (1.0f - fx) * s0 + fx * s1;
+    s0 = read_at(bx0, by1, bz0, r01 + lx0, v); vp &= v;
+    s1 = read_at(bx1, by1, bz0, r01 + lx1, v); vp &= v;
+    float r1 = (1.0f - fx) * s0 + fx * s1;
+    float r2 = (1.0f - fy) * r0 + fy * r1;
+    s0 = read_at(bx0, by0, bz1, r10 + lx0, v); vp &= v;
+    s1 = read_at(bx1, by0, bz1, r10 + lx1, v); vp &= v;
+    r0 = (1.0f - fx) * s0 + fx * s1;
+    s0 = read_at(bx0, by1, bz1, r11 + lx0, v); vp &= v;
+    s1 = read_at(bx1, by1, bz1, r11 + lx1, v); vp &= v;
+    r1 = (1.0f - fx) * s0 + fx * s1;
+    float r3 = (1.0f - fy) * r0 + fy * r1;
+    valid = vp;
+    return (1.0f - fz) * r2 + fz * r3;
+  }
+  // Block of a voxel coordinate: floor division by 8.  It equals
+  // vox_to_block except below INT32_MIN + 7 (a saturated coordinate), where
+  // both lie outside the key range and the lookup finds nothing.
+  __device__ static inline int block_of(int v) { return v >> 3; }
+  // Read of local voxel `l` of block (bx, by, bz) through the one-entry cache.
+  __device__ inline float read_at(int bx, int by, int bz, int l, bool& valid) {
     if (bx != cbx || by != cby || bz != cbz) {
       cbx = bx; cby = by; cbz = bz;
       cflags = 0;
@@ -97,7 +137,7 @@ struct VoxelReader {
       return 1.0f;
     }
     if (cblk >= 0 && cflags != kBlockUnobserved) {
-      typename VT::T v = vox[(size_t)cblk * 512 + (vox_to_local(vz) * 8 + vox_to_local(vy)) * 8 + vox_to_local(vx)];
+      typename VT::T v = vox[(size_t)cblk * 512 + l];
       if (VT::observed(v)) {
         valid = true;
         return VT::sdf(v);
@@ -105,27 +145,6 @@ struct VoxelReader {
     }
     valid = false;
     return 1.0f;
-  }
-  // RayTraceEngine.cu:294-329: trilinear read, valid iff all 8 reads are.
-  __device__ inline float interp(float x, float y, float z, bool& valid) {
-    float fx = x - floorf(x), fy = y - floorf(y), fz = z - floorf(z);
-    bool vp, v;
-    float s0 = read(x, y, z, vp);
-    float s1 = read(x + 1.0f, y, z, v); vp &= v;
-    float r0 = (1.0f - fx) * s0 + fx * s1;
-    s0 = read(x, y + 1.0f, z, v); vp &= v;
-    s1 = read(x + 1.0f, y + 1.0f, z, v); vp &= v;
-    float r1 = (1.0f - fx) * s0 + fx * s1;
-    float r2 = (1.0f - fy) * r0 + fy * r1;
-    s0 = read(x, y, z + 1.0f, v); vp &= v;
-    s1 = read(x + 1.0f, y, z + 1.0f, v); vp &= v;
-    r0 = (1.0f - fx) * s0 + fx * s1;
-    s0 = read(x, y + 1.0f, z + 1.0f, v); vp &= v;
-    s1 = read(x + 1.0f, y + 1.0f, z + 1.0f, v); vp &= v;
-    r1 = (1.0f - fx) * s0 + fx * s1;
-    float r3 = (1.0f - fy) * r0 + fy * r1;
-    valid = vp;
-    return (1.0f - fz) * r2 + fz * r3;
   }
 };
 
