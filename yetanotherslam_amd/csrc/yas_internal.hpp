@@ -94,8 +94,10 @@ __host__ __device__ inline void unpack_key(uint64_t k, int& x, int& y, int& z) {
 }
 // Reference spatial hash (VoxelStructUtils.h:16-21): int32 wrapping
 // multiplies, xor, remainder fixed into [0, n).  It picks the home slot.
+// For a power-of-two n that remainder is the low bits of the hash.
 __host__ __device__ inline int ref_hash(int x, int y, int z, int n) {
   uint32_t h = ((uint32_t)x * 73856093u) ^ ((uint32_t)y * 19349669u) ^ ((uint32_t)z * 83492791u);
+  if ((n & (n - 1)) == 0) return (int)(h & (uint32_t)(n - 1));
   int32_t r = (int32_t)h;
   r %= n;
   return r < 0 ? r + n : r;

@@ -73,8 +73,17 @@ __device__ inline void zrange_at(const MapView& m, const PosePair* pose, const I
   }
 }
 
+// Voxel coordinate of a sample: floor, saturated into the int range, NaN to
+// 0.  A ray whose end points lie within +-2^28 voxels stays within +-2^30
+// (it advances at most its length plus one step past its end), where a
+// plain conversion of the floor gives the same integer.
+template <bool kInRange>
+__device__ inline int voxel_of(float x) {
+  return kInRange ? (int)floorf(x) : floor_sat(x);
+}
+
 // Voxel reads with a one-entry block cache.
-template <class VT>
+template <class VT, bool kInRange = false>
 struct VoxelReader {
   const MapView& m;
   const typename VT::T* vox;
@@ -85,7 +94,7 @@ struct VoxelReader {
   // Blocks summarised as all-unobserved or all-free answer without a voxel
   // load (the values they would return are exactly (1, invalid) / (1, valid)).
   __device__ inline float read(float x, float y, float z, bool& valid) {
-    const int vx = floor_sat(x), vy = floor_sat(y), vz = floor_sat(z);
+    const int vx = voxel_of<kInRange>(x), vy = voxel_of<kInRange>(y), vz = voxel_of<kInRange>(z);
     return read_at(block_of(vx), block_of(vy), block_of(vz),
                    (vox_to_local(vz) * 8 + vox_to_local(vy)) * 8 + vox_to_local(vx), valid);
   }
@@ -95,9 +104,9 @@ struct VoxelReader {
   // reference's order.
   __device__ inline float interp(float x, float y, float z, bool& valid) {
     const float fx = x - floorf(x), fy = y - floorf(y), fz = z - floorf(z);
-    const int x0 = floor_sat(x), x1 = floor_sat(x + 1.0f);
-    const int y0 = floor_sat(y), y1 = floor_sat(y + 1.0f);
-    const int z0 = floor_sat(z), z1 = floor_sat(z + 1.0f);
+    const int x0 = voxel_of<kInRange>(x), x1 = voxel_of<kInRange>(x + 1.0f);
+    const int y0 = voxel_of<kInRange>(y), y1 = voxel_of<kInRange>(y + 1.0f);
+    const int z0 = voxel_of<kInRange>(z), z1 = voxel_of<kInRange>(z + 1.0f);
     const int bx0 = block_of(x0), bx1 = block_of(x1), by0 = block_of(y0), by1 = block_of(y1);
     const int bz0 = block_of(z0), bz1 = block_of(z1);
     const int lx0 = vox_to_local(x0), lx1 = vox_to_local(x1);
@@ -148,6 +157,34 @@ struct VoxelReader {
   }
 };
 
+// The sample loop of RayTraceEngine.cu:367-384 and the two refinement steps
+// of :386-400 after a zero crossing; on a valid hit, (rx, ry, rz) is the
+// surface point in voxel units.
+template <class VT, bool kInRange>
+__device__ inline bool march_ray(const MapView& m, float& rx, float& ry, float& rz, float dx, float dy, float dz,
+                                 float dist_s, float dist_e, float step_scale) {
+  VoxelReader<VT, kInRange> rd(m);
+  float sdf = 1.0f, last;
+  bool valid = false;
+  while (dist_s < dist_e) {
+    last = sdf;
+    sdf = rd.read(rx, ry, rz, valid);
+    if (sdf <= 0.5f && sdf >= -0.5f) sdf = rd.interp(rx, ry, rz, valid);
+    if (sdf <= 0.0f) break;
+    if (sdf >= 0.0f && last < 0.0f) return false;  // abandoned
+    float step = valid ? fmaxf(sdf * step_scale, 1.0f) : 1.0f;
+    rx = rx + step * dx; ry = ry + step * dy; rz = rz + step * dz;
+    dist_s = dist_s + step;
+  }
+  if (!(sdf <= 0.0f)) return false;
+  float step = sdf * step_scale;
+  rx = rx + step * dx; ry = ry + step * dy; rz = rz + step * dz;
+  sdf = rd.interp(rx, ry, rz, valid);
+  step = sdf * step_scale;
+  rx = rx + step * dx; ry = ry + step * dy; rz = rz + step * dz;
+  return valid;
+}
+
 // RayTraceEngine.cu:331-410.  Departure: pixels the reference leaves
 // unwritten get w = -1 (x = NaN, as the reference writes first).
 // Workgroup b of the image's (W/16)*(H/16) tiles.
@@ -181,35 +218,17 @@ __device__ inline void march_at(const MapView& m, const PosePair* pose, const In
       float nn = sqrtf(n2);
       dx = dx / nn; dy = dy / nn; dz = dz / nn;
     }
-    VoxelReader<VT> rd(m);
+    const float lim = 268435456.0f;  // 2^28 voxels
+    const bool in_range = fabsf(bsx) < lim && fabsf(bsy) < lim && fabsf(bsz) < lim && fabsf(bex) < lim &&
+                          fabsf(bey) < lim && fabsf(bez) < lim && step_scale < 1048576.0f;
     float rx = bsx, ry = bsy, rz = bsz;
-    float sdf = 1.0f, last;
-    bool valid = false, abandoned = false;
-    while (dist_s < dist_e) {
-      last = sdf;
-      sdf = rd.read(rx, ry, rz, valid);
-      if (sdf <= 0.5f && sdf >= -0.5f) sdf = rd.interp(rx, ry, rz, valid);
-      if (sdf <= 0.0f) break;
-      if (sdf >= 0.0f && last < 0.0f) {
-        abandoned = true;
-        break;
-      }
-      float step = valid ? fmaxf(sdf * step_scale, 1.0f) : 1.0f;
-      rx = rx + step * dx; ry = ry + step * dy; rz = rz + step * dz;
-      dist_s = dist_s + step;
-    }
-    if (!abandoned && sdf <= 0.0f) {
-      float step = sdf * step_scale;
-      rx = rx + step * dx; ry = ry + step * dy; rz = rz + step * dz;
-      sdf = rd.interp(rx, ry, rz, valid);
-      step = sdf * step_scale;
-      rx = rx + step * dx; ry = ry + step * dy; rz = rz + step * dz;
-      if (valid) {
-        const PoseF Ti = pose->Tinv;
-        float cx, cy, cz;
-        xform(Ti, rx / vsinv, ry / vsinv, rz / vsinv, cx, cy, cz);
-        out = make_float4(cx, cy, cz, 1.0f);
-      }
+    const bool hit = __all(in_range) ? march_ray<VT, true>(m, rx, ry, rz, dx, dy, dz, dist_s, dist_e, step_scale)
+                                     : march_ray<VT, false>(m, rx, ry, rz, dx, dy, dz, dist_s, dist_e, step_scale);
+    if (hit) {
+      const PoseF Ti = pose->Tinv;
+      float cx, cy, cz;
+      xform(Ti, rx / vsinv, ry / vsinv, rz / vsinv, cx, cy, cz);
+      out = make_float4(cx, cy, cz, 1.0f);
     }
   }
   vmap[(size_t)y * W + x] = out;
