@@ -500,6 +500,13 @@ __device__ inline void swap_d(T& a, T& b) {
   a = b;
   b = t;
 }
+// Swap by selects on both values, so the run-time pivot never becomes an
+// array index (which would move the array to scratch memory).
+__device__ inline void swap_if(bool p, double& a, double& b) {
+  const double x = a, z = b;
+  a = p ? z : x;
+  b = p ? x : z;
+}
 __device__ inline void ldlt_solve6(double (&A)[36], double (&y)[6]) {
   constexpr int n = 6;
   int tr[n];
@@ -559,8 +566,7 @@ __device__ inline void ldlt_solve6(double (&A)[36], double (&y)[6]) {
 #pragma unroll
   for (int k = 0; k < n; ++k)
 #pragma unroll
-    for (int m = k; m < n; ++m)
-      if (m == tr[k] && m != k) swap_d(y[k], y[m]);
+    for (int m = k + 1; m < n; ++m) swap_if(m == tr[k], y[k], y[m]);
 #pragma unroll
   for (int i = 0; i < n; ++i)
 #pragma unroll
@@ -577,8 +583,7 @@ __device__ inline void ldlt_solve6(double (&A)[36], double (&y)[6]) {
 #pragma unroll
   for (int k = n - 1; k >= 0; --k)
 #pragma unroll
-    for (int m = k; m < n; ++m)
-      if (m == tr[k] && m != k) swap_d(y[k], y[m]);
+    for (int m = k + 1; m < n; ++m) swap_if(m == tr[k], y[k], y[m]);
 }
 
 // SE(3) exponential, twist (rho, omega), Rodrigues' formula.
