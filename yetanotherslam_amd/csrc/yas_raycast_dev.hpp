@@ -88,32 +88,48 @@ struct VoxelReader {
   const MapView& m;
   const typename VT::T* vox;
   int cbx, cby, cbz, cblk, cflags;
+  int rx0 = 0, ry0 = 0, rz0 = 0;  // voxel of the last read()
+  float rs = 1.0f;                // its answer
+  bool rv = false;
   __device__ VoxelReader(const MapView& mv)
       : m(mv), vox((const typename VT::T*)mv.voxels), cbx(INT32_MIN), cby(0), cbz(0), cblk(-1), cflags(0) {}
   // RayTraceEngine.cu:279-292: nearest read at floor(p), valid iff observed.
   // Blocks summarised as all-unobserved or all-free answer without a voxel
   // load (the values they would return are exactly (1, invalid) / (1, valid)).
+  // The voxel and its answer are kept for interp_after_read.
   __device__ inline float read(float x, float y, float z, bool& valid) {
-    const int vx = voxel_of<kInRange>(x), vy = voxel_of<kInRange>(y), vz = voxel_of<kInRange>(z);
-    return read_at(block_of(vx), block_of(vy), block_of(vz),
-                   (vox_to_local(vz) * 8 + vox_to_local(vy)) * 8 + vox_to_local(vx), valid);
+    rx0 = voxel_of<kInRange>(x); ry0 = voxel_of<kInRange>(y); rz0 = voxel_of<kInRange>(z);
+    rs = read_at(block_of(rx0), block_of(ry0), block_of(rz0),
+                 (vox_to_local(rz0) * 8 + vox_to_local(ry0)) * 8 + vox_to_local(rx0), valid);
+    rv = valid;
+    return rs;
   }
   // RayTraceEngine.cu:294-329: trilinear read, valid iff all 8 reads are.
   // The corners are floor(x) and floor(x + 1) per axis, as the reference
   // rounds them; each is converted once and the 8 reads run in the
   // reference's order.
   __device__ inline float interp(float x, float y, float z, bool& valid) {
+    const int x0 = voxel_of<kInRange>(x), y0 = voxel_of<kInRange>(y), z0 = voxel_of<kInRange>(z);
+    bool v0;
+    const float s0 = read_at(block_of(x0), block_of(y0), block_of(z0),
+                             (vox_to_local(z0) * 8 + vox_to_local(y0)) * 8 + vox_to_local(x0), v0);
+    return interp_rest(x, y, z, x0, y0, z0, s0, v0, valid);
+  }
+  // The same at the point of the last read(): its voxel is the first corner,
+  // read just before with the same answer and leaving the cache on its block.
+  __device__ inline float interp_after_read(float x, float y, float z, bool& valid) {
+    return interp_rest(x, y, z, rx0, ry0, rz0, rs, rv, valid);
+  }
+  __device__ inline float interp_rest(float x, float y, float z, int x0, int y0, int z0, float s0, bool vp,
+                                      bool& valid) {
     const float fx = x - floorf(x), fy = y - floorf(y), fz = z - floorf(z);
-    const int x0 = voxel_of<kInRange>(x), x1 = voxel_of<kInRange>(x + 1.0f);
-    const int y0 = voxel_of<kInRange>(y), y1 = voxel_of<kInRange>(y + 1.0f);
-    const int z0 = voxel_of<kInRange>(z), z1 = voxel_of<kInRange>(z + 1.0f);
+    const int x1 = voxel_of<kInRange>(x + 1.0f), y1 = voxel_of<kInRange>(y + 1.0f), z1 = voxel_of<kInRange>(z + 1.0f);
     const int bx0 = block_of(x0), bx1 = block_of(x1), by0 = block_of(y0), by1 = block_of(y1);
     const int bz0 = block_of(z0), bz1 = block_of(z1);
     const int lx0 = vox_to_local(x0), lx1 = vox_to_local(x1);
     const int r00 = (vox_to_local(z0) * 8 + vox_to_local(y0)) * 8, r01 = (vox_to_local(z0) * 8 + vox_to_local(y1)) * 8;
     const int r10 = (vox_to_local(z1) * 8 + vox_to_local(y0)) * 8, r11 = (vox_to_local(z1) * 8 + vox_to_local(y1)) * 8;
-    bool vp, v;
-    float s0 = read_at(bx0, by0, bz0, r00 + lx0, vp);
+    bool v;
     float s1 = read_at(bx1, by0, bz0, r00 + lx1, v); vp &= v;
     float r0 = (1.0f - fx) * s0 + fx * s1;
     s0 = read_at(bx0, by1, bz0, r01 + lx0, v); vp &= v;
@@ -169,7 +185,7 @@ __device__ inline bool march_ray(const MapView& m, float& rx, float& ry, float& 
   while (dist_s < dist_e) {
     last = sdf;
     sdf = rd.read(rx, ry, rz, valid);
-    if (sdf <= 0.5f && sdf >= -0.5f) sdf = rd.interp(rx, ry, rz, valid);
+    if (sdf <= 0.5f && sdf >= -0.5f) sdf = rd.interp_after_read(rx, ry, rz, valid);
     if (sdf <= 0.0f) break;
     if (sdf >= 0.0f && last < 0.0f) return false;  // abandoned
     float step = valid ? fmaxf(sdf * step_scale, 1.0f) : 1.0f;
