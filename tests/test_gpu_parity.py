@@ -195,6 +195,28 @@ def test_mesh_cases_and_triangles_bit_exact(fmt):
     assert_bits_equal(mo["normals"], normals, "normals")
 
 
+@pytest.mark.parametrize("fmt", FMTS[1:], ids=["f32", "f16u16"])
+def test_mesh_of_a_large_5mm_volume_bit_exact(fmt):
+    """Marching cubes over a 5 mm volume fused from 60 frames at 640x480
+    (about 30 k blocks, 2.6 M triangles), float32 and float16 sdf: block
+    keys, cube cases and per-block triangle counts are bit-exact, and so are
+    the vertices and normals."""
+    _, _, _, _, om, gm = fuse_both(fmt, n=60, width=640, height=480, voxel=0.005, trunc=0.02, buckets=1 << 20,
+                                   blocks=1 << 17)
+    mo = om.mesh()
+    me = rf.MeshEngine(4_000_000)
+    mg = me.Meshify(gm, with_cases=True)
+    keys, counts, cases, tris, normals = rf.canonical_mesh(mg)
+    print("large mesh:", len(keys), "blocks,", mg["num_triangles"], "triangles")
+    assert len(keys) > 20000 and mg["num_triangles"] > 2_000_000
+    np.testing.assert_array_equal(mo["keys"], keys)
+    np.testing.assert_array_equal(mo["cases"], cases)
+    np.testing.assert_array_equal(mo["counts"], counts)
+    assert mg["num_triangles"] == len(mo["tris"])
+    assert_bits_equal(mo["tris"], tris, "vertices")
+    assert_bits_equal(mo["normals"], normals, "normals")
+
+
 @pytest.mark.parametrize("fmt", FMTS, ids=["i16u8", "f32", "f16u16"])
 def test_indexed_mesh_matches_the_soup_and_shares_vertices(fmt):
     """The indexed mesh expands to the reference-form soup (same triangle
