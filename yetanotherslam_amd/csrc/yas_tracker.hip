@@ -81,7 +81,7 @@ __global__ void __launch_bounds__(256) k_vmap_to_depth(const float4* v, size_t s
 template <bool ICP, bool RGB>
 __global__ void __launch_bounds__(256) k_pass1(Level L, TrackBufs B, const TrackState* st, float* part_icp,
                                                float* part_res, float angle_th, float dist_th) {
-  pass1_at<ICP, RGB>(L, B, st, part_icp, part_res, angle_th, dist_th, blockIdx.x, gridDim.x);
+  pass1_at<ICP, RGB, true>(L, B, st, part_icp, part_res, angle_th, dist_th, blockIdx.x, gridDim.x);
 }
 __global__ void __launch_bounds__(256) k_pass2(Level L, TrackBufs B, TrackState* st, const float* part_res, int g1,
                                                float* part_rgb) {

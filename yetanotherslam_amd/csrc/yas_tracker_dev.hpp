@@ -295,8 +295,7 @@ struct TrackBufs {
 // the current frame, gated at sin(20 deg) normal angle and 1 cm; on a match
 // the point-to-plane row [n, v x n, n.(v_c - v)] is accumulated.
 __device__ inline void icp_pixel(const Level& L, const TrackBufs& B, const PoseF& T, float angle_th, float dist_th,
-                                 int k, float (&acc)[32]) {
-  const float4 vl = B.ref_v[k];
+                                 const float4& vl, const float4& nl, float (&acc)[32]) {
   if (!(vl.w >= 0.0f)) return;
   float vx, vy, vz;
   xform(T, vl.x, vl.y, vl.z, vx, vy, vz);
@@ -305,11 +304,10 @@ __device__ inline void icp_pixel(const Level& L, const TrackBufs& B, const PoseF
   const int v = floor_sat(L.fy * vy * invz + L.cy + 0.5f);
   if (u < 0 || v < 0 || u >= L.w || v >= L.h) return;
   const float4 vc = B.cur_v[v * L.w + u];
+  const float4 nc = B.cur_n[v * L.w + u];  // read with the vertex: one round trip
   if (vc.w < 0.0f) return;
-  const float4 nl = B.ref_n[k];
   float nx, ny, nz;
   rot3(T, nl.x, nl.y, nl.z, nx, ny, nz);
-  const float4 nc = B.cur_n[v * L.w + u];
   const float dx = vx - vc.x, dy = vy - vc.y, dz = vz - vc.z;
   const float dist = sqrtf(dx * dx + dy * dy + dz * dz);
   const float c0 = nc.y * nz - nc.z * ny, c1 = nc.z * nx - nc.x * nz, c2 = nc.x * ny - nc.y * nx;
@@ -354,9 +352,8 @@ __device__ inline float bilerp(const float* m, int w, float x, float y) {
 // ImageProc.cu:42-61: reference pixel k at its inverse depth, moved by the
 // warp (RK, t); w = -1 where the inverse depth is 0.  Both passes evaluate it
 // (the same operations, so the same floats) instead of storing it.
-__device__ inline float4 warp_ref(const Level& L, const TrackBufs& B, const float* wp, int k) {
+__device__ inline float4 warp_ref(const Level& L, const float* wp, int k, float iz) {
   const int y = k / L.w, x = k - y * L.w;
-  const float iz = B.ref_inv[k];
   float4 p = make_float4(0.f, 0.f, 0.f, -1.f);
   if (iz > 0.0f) {
     float z = 1.0f / iz;
@@ -370,16 +367,15 @@ __device__ inline float4 warp_ref(const Level& L, const TrackBufs& B, const floa
 
 // TrackingUtils.h:73-151 with the warp of ImageProc.cu:42-61 fused in; the
 // residual statistics are accumulated with Welford's update.
-__device__ inline void rgb_pixel(const Level& L, const TrackBufs& B, const float* wp, int k, float& n, float& mean,
-                                 float& m2) {
-  const float4 p = warp_ref(L, B, wp, k);
+__device__ inline void rgb_pixel(const Level& L, const TrackBufs& B, const float* wp, int k, float iz, float ref,
+                                 float& n, float& mean, float& m2) {
+  const float4 p = warp_ref(L, wp, k, iz);
   float r = 0.0f, gx = 0.0f, gy = 0.0f;
   bool ok = false;
   if (p.w > 0.0f) {
     float u = L.fx * p.x / p.z + L.cx;
     float v = L.fy * p.y / p.z + L.cy;
     if (u > 0.0f && v > 0.0f && u < (float)(L.w - 1) && v < (float)(L.h - 1)) {
-      float ref = B.ref_int[k];
       float cur = bilerp(B.cur_int, L.w, u, v);
       r = cur - ref;
       gx = bilerp(B.gx, L.w, u, v);
@@ -397,7 +393,30 @@ __device__ inline void rgb_pixel(const Level& L, const TrackBufs& B, const float
 }
 
 // Pass 1: ICP rows and/or photometric residuals of one linearisation.
+// Per-pixel reference values that pass 1 reads at pixel k before any
+// dependent (gathered) read.
+struct RefPixel {
+  float4 v, n;      // model vertex and normal (ICP)
+  float iz, inten;  // inverse depth and intensity (photometric)
+};
 template <bool ICP, bool RGB>
+__device__ inline RefPixel load_ref_pixel(const TrackBufs& B, int k) {
+  RefPixel r;
+  if (ICP) {
+    r.v = B.ref_v[k];
+    r.n = B.ref_n[k];
+  }
+  if (RGB) {
+    r.iz = B.ref_inv[k];
+    r.inten = B.ref_int[k];
+  }
+  return r;
+}
+
+// PREFETCH: the next pixel's reference values are loaded before the
+// current pixel's gathers, so their latencies overlap (more registers; the
+// values and the order of the sums are the same).
+template <bool ICP, bool RGB, bool PREFETCH = false>
 __device__ inline void pass1_at(const Level& L, const TrackBufs& B, const TrackState* st, float* part_icp,
                                 float* part_res, float angle_th, float dist_th, int bx, int nb) {
   __shared__ float pose[12];
@@ -418,9 +437,20 @@ __device__ inline void pass1_at(const Level& L, const TrackBufs& B, const TrackS
   for (int i = 0; i < 32; ++i) acc[i] = 0.0f;
   float n = 0.0f, mean = 0.0f, m2 = 0.0f;
   const int N = L.w * L.h;
-  for (int k = bx * blockDim.x + threadIdx.x; k < N; k += nb * blockDim.x) {
-    if (ICP) icp_pixel(L, B, T, angle_th, dist_th, k, acc);
-    if (RGB) rgb_pixel(L, B, wu, k, n, mean, m2);
+  const int stride = nb * blockDim.x;
+  int k = bx * blockDim.x + threadIdx.x;
+  RefPixel nxt = {};
+  if (PREFETCH && k < N) nxt = load_ref_pixel<ICP, RGB>(B, k);
+  for (; k < N; k += stride) {
+    RefPixel cur;
+    if (PREFETCH) {
+      cur = nxt;
+      if (k + stride < N) nxt = load_ref_pixel<ICP, RGB>(B, k + stride);
+    } else {
+      cur = load_ref_pixel<ICP, RGB>(B, k);
+    }
+    if (ICP) icp_pixel(L, B, T, angle_th, dist_th, cur.v, cur.n, acc);
+    if (RGB) rgb_pixel(L, B, wu, k, cur.iz, cur.inten, n, mean, m2);
   }
   if (ICP) {
     float t = block_sum32(acc, lds);
@@ -466,7 +496,7 @@ __device__ inline void pass2_at(const Level& L, const TrackBufs& B, TrackState* 
   for (int k = bx * blockDim.x + threadIdx.x; k < N; k += nb * blockDim.x) {
     const float4 e = B.resid[k];
     if (!(e.w > 0.0f)) continue;
-    const float4 p = warp_ref(L, B, wu, k);
+    const float4 p = warp_ref(L, wu, k, B.ref_inv[k]);
     const float zinv = 1.0f / p.z;
     float wt = 1.0f;
     if (fabsf(e.x) > th) wt = th / fabsf(e.x);
