@@ -187,7 +187,8 @@ template <bool ICP, bool RGB>
 __global__ void __launch_bounds__(256) k_s_pass1(const StreamDev* streams, const Table* tab, int l, int slot0) {
   SLOT_FROM_OR_RETURN(slot0);
   if (sl.first || (int)blockIdx.x >= S.groups[l]) return;
-  pass1_at<ICP, RGB>(S.lv[l], S.tb[l], S.st, S.part_icp, S.part_res, S.angle_th, S.dist_th, blockIdx.x, S.groups[l]);
+  pass1_at<ICP, RGB, true>(S.lv[l], S.tb[l], S.st, S.part_icp, S.part_res, S.angle_th, S.dist_th, blockIdx.x,
+                           S.groups[l]);
 }
 
 __global__ void __launch_bounds__(256) k_s_pass2(const StreamDev* streams, const Table* tab, int l, int slot0) {

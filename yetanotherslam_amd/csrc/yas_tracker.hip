@@ -85,7 +85,7 @@ __global__ void __launch_bounds__(256) k_pass1(Level L, TrackBufs B, const Track
 }
 __global__ void __launch_bounds__(256) k_pass2(Level L, TrackBufs B, TrackState* st, const float* part_res, int g1,
                                                float* part_rgb) {
-  pass2_at(L, B, st, part_res, g1, part_rgb, blockIdx.x, gridDim.x);
+  pass2_at<true>(L, B, st, part_res, g1, part_rgb, blockIdx.x, gridDim.x);
 }
 __global__ void __launch_bounds__(256) k_update(TrackState* st, const float* part_icp, int gi, const float* part_rgb,
                                                 int gr, int mode, int first_of_level, int do_step, int record,

@@ -469,10 +469,22 @@ __device__ inline void pass1_at(const Level& L, const TrackBufs& B, const TrackS
 // Pass 2 (TrackingUtils.h:153-220, RGBDTracking.cu:262-290): fold the
 // residual statistics of pass 1 (identically in every workgroup), derive
 // sigma and the Huber threshold 4.685 sigma, and reduce the weighted rows.
+// PREFETCH: each pixel's residual and inverse depth are loaded one pixel
+// ahead (the first pixel's while the statistics are folded).
+template <bool PREFETCH = false>
 __device__ inline void pass2_at(const Level& L, const TrackBufs& B, TrackState* st, const float* part_res, int g1,
                                 float* part_rgb, int bx, int nb) {
   __shared__ float lds[4 * 32];
   __shared__ float wp[12];
+  const int N = L.w * L.h;
+  const int stride = nb * blockDim.x;
+  int k = bx * blockDim.x + threadIdx.x;
+  float4 e_next = make_float4(0.f, 0.f, 0.f, 0.f);
+  float iz_next = 0.0f;
+  if (PREFETCH && k < N) {
+    e_next = B.resid[k];
+    iz_next = B.ref_inv[k];
+  }
   load_warp(st->est, L.Kd, wp);
   float n = 0.0f, mean = 0.0f, m2 = 0.0f;
   for (int g = threadIdx.x; g < g1; g += blockDim.x)
@@ -492,11 +504,22 @@ __device__ inline void pass2_at(const Level& L, const TrackBufs& B, TrackState* 
   float acc[32];
 #pragma unroll
   for (int i = 0; i < 32; ++i) acc[i] = 0.0f;
-  const int N = L.w * L.h;
-  for (int k = bx * blockDim.x + threadIdx.x; k < N; k += nb * blockDim.x) {
-    const float4 e = B.resid[k];
+  for (; k < N; k += stride) {
+    float4 e;
+    float iz;
+    if (PREFETCH) {
+      e = e_next;
+      iz = iz_next;
+      if (k + stride < N) {
+        e_next = B.resid[k + stride];
+        iz_next = B.ref_inv[k + stride];
+      }
+    } else {
+      e = B.resid[k];
+      iz = B.ref_inv[k];
+    }
     if (!(e.w > 0.0f)) continue;
-    const float4 p = warp_ref(L, wu, k, B.ref_inv[k]);
+    const float4 p = warp_ref(L, wu, k, iz);
     const float zinv = 1.0f / p.z;
     float wt = 1.0f;
     if (fabsf(e.x) > th) wt = th / fabsf(e.x);
