@@ -422,6 +422,11 @@ __device__ inline void pass1_at(const Level& L, const TrackBufs& B, const TrackS
   __shared__ float pose[12];
   __shared__ float wp[12];
   __shared__ float lds[4 * 32];
+  const int N = L.w * L.h;
+  const int stride = nb * blockDim.x;
+  int k = bx * blockDim.x + threadIdx.x;
+  RefPixel nxt = {};
+  if (PREFETCH && k < N) nxt = load_ref_pixel<ICP, RGB>(B, k);  // in flight while the pose loads
   load_pose_f(st->est, pose);
   if (RGB) load_warp(st->est, L.Kd, wp);
   __syncthreads();
@@ -436,11 +441,6 @@ __device__ inline void pass1_at(const Level& L, const TrackBufs& B, const TrackS
 #pragma unroll
   for (int i = 0; i < 32; ++i) acc[i] = 0.0f;
   float n = 0.0f, mean = 0.0f, m2 = 0.0f;
-  const int N = L.w * L.h;
-  const int stride = nb * blockDim.x;
-  int k = bx * blockDim.x + threadIdx.x;
-  RefPixel nxt = {};
-  if (PREFETCH && k < N) nxt = load_ref_pixel<ICP, RGB>(B, k);
   for (; k < N; k += stride) {
     RefPixel cur;
     if (PREFETCH) {
@@ -696,18 +696,28 @@ __device__ inline void update_at(TrackState* st, const float* part_icp, int gi, 
   if (threadIdx.x < 16) s_est[threadIdx.x] = st->est[threadIdx.x];
   if (threadIdx.x == 16) s_good = st->good;
   if (threadIdx.x == 17) s_last_err = st->last_err;
-  for (int which = 0; which < 2; ++which) {
-    const float* part = which == 0 ? part_icp : part_rgb;
-    const int g = which == 0 ? gi : gr;
-    if (g == 0) continue;
-    float v[32];
+  // both terms' rows are read in one pass, so their loads overlap
+  float vi[32], vr[32];
 #pragma unroll
-    for (int i = 0; i < 32; ++i) v[i] = 0.0f;
-    for (int r = threadIdx.x; r < g; r += blockDim.x)
+  for (int i = 0; i < 32; ++i) vi[i] = vr[i] = 0.0f;
+  const int g = gi > gr ? gi : gr;
+  for (int r = threadIdx.x; r < g; r += blockDim.x) {
+    if (r < gi) {
 #pragma unroll
-      for (int i = 0; i < 32; ++i) v[i] += part[r * 32 + i];
-    float t = block_sum32(v, lds);
-    if (threadIdx.x < 32) sums[which][threadIdx.x] = t;
+      for (int i = 0; i < 32; ++i) vi[i] += part_icp[r * 32 + i];
+    }
+    if (r < gr) {
+#pragma unroll
+      for (int i = 0; i < 32; ++i) vr[i] += part_rgb[r * 32 + i];
+    }
+  }
+  if (gi) {
+    const float t = block_sum32(vi, lds);
+    if (threadIdx.x < 32) sums[0][threadIdx.x] = t;
+  }
+  if (gr) {
+    const float t = block_sum32(vr, lds);
+    if (threadIdx.x < 32) sums[1][threadIdx.x] = t;
   }
   __syncthreads();
   if (threadIdx.x < 29) {
