@@ -322,12 +322,17 @@ __device__ inline int find_block(const MapView& m, uint64_t key, int home, int* 
   int slot = home;
   const int lim = m.num_buckets < kMaxProbe ? m.num_buckets : kMaxProbe;
   for (int p = 0; p < lim; ++p) {
-    const HashSlot e = m.slots[slot];
-    if (e.key == key) {
-      if (flags) *flags = e.flags;
-      return e.val;
+    // One 16-byte load.  The empty asm statement takes the pool index and
+    // flags as register operands right after it, so the compiler cannot
+    // defer their half of the slot to a second, dependent load on a match.
+    uint4 e = reinterpret_cast<const uint4*>(m.slots)[slot];
+    asm volatile("" : "+v"(e.z), "+v"(e.w));
+    const unsigned long long k = ((unsigned long long)e.y << 32) | e.x;
+    if (k == key) {
+      if (flags) *flags = (int)e.w;
+      return (int)e.z;
     }
-    if (e.key == kEmptyKey) return -1;
+    if (k == kEmptyKey) return -1;
     slot = slot + 1 == m.num_buckets ? 0 : slot + 1;
   }
   return -1;
