@@ -168,35 +168,47 @@ __device__ inline void integrate_waves(const MapView& m, const PoseF& Ti, const 
   typename VT::T* vox = (typename VT::T*)m.voxels;
   for (int w = wave; w < nvis; w += nwaves) {
     const int blk = m.visible[w];
-    int bx, by, bz;
-    unpack_key(m.block_keys[blk], bx, by, bz);
+    const unsigned long long key = m.block_keys[blk];
     typename VT::T* bv = vox + (size_t)blk * 512;
+    // the block's 8 slices are loaded together, then each pixel's depth is
+    // read (at a clamped pixel, used only when in the image) before any update
+    typename VT::T cur[8];
+#pragma unroll
+    for (int lz = 0; lz < 8; ++lz) cur[lz] = bv[lz * 64 + lane];
+    int bx, by, bz;
+    unpack_key(key, bx, by, bz);
     const float wx = (float)(bx * 8 + lx) * vs;
     const float wy = (float)(by * 8 + ly) * vs;
-    bool unobserved = true, free_space = true;
-#pragma unroll 2
+    float pzs[8], ds[8];
+    bool in[8];
+#pragma unroll
     for (int lz = 0; lz < 8; ++lz) {
-      const int li = lz * 64 + lane;
-      typename VT::T cur = bv[li];
       const float wz = (float)(bz * 8 + lz) * vs;
       float px, py, pz;
       xform(Ti, wx, wy, wz, px, py, pz);
-      int u = floor_sat(K.fx * px / pz + K.cx + 0.5f);
-      int v = floor_sat(K.fy * py / pz + K.cy + 0.5f);
-      if (!(u < 0 || v < 0 || u > W - 1 || v > H - 1)) {
-        float d = depth.at(u, v);
+      const int u = floor_sat(K.fx * px / pz + K.cx + 0.5f);
+      const int v = floor_sat(K.fy * py / pz + K.cy + 0.5f);
+      in[lz] = !(u < 0 || v < 0 || u > W - 1 || v > H - 1);
+      ds[lz] = depth.at(min(max(u, 0), W - 1), min(max(v, 0), H - 1));
+      pzs[lz] = pz;
+    }
+    bool unobserved = true, free_space = true;
+#pragma unroll
+    for (int lz = 0; lz < 8; ++lz) {
+      if (in[lz]) {
+        const float d = ds[lz];
         if (!(d != d || d > m.depth_max || d < m.depth_min)) {
-          float sdf = d - pz;
+          float sdf = d - pzs[lz];
           if (!(sdf < -mu)) {
             sdf = fminf(1.0f, sdf / mu);
-            cur = VT::update(cur, sdf, m.max_weight);
-            bv[li] = cur;
+            cur[lz] = VT::update(cur[lz], sdf, m.max_weight);
+            bv[lz * 64 + lane] = cur[lz];
           }
         }
       }
-      const bool obs = VT::observed(cur);
+      const bool obs = VT::observed(cur[lz]);
       unobserved = unobserved && !obs;
-      free_space = free_space && obs && VT::sdf(cur) == 1.0f;
+      free_space = free_space && obs && VT::sdf(cur[lz]) == 1.0f;
     }
     const unsigned long long un = __ballot(unobserved), fr = __ballot(free_space);
     if (lane == 0) {
