@@ -28,6 +28,8 @@ int yas_tracker_init_identity_async(yas_tracker* t);
 const double* yas_tracker_result_dev(yas_tracker* t);
 const int* yas_tracker_good_dev(yas_tracker* t);
 int yas_tracker_set_tracking_image_dev(yas_tracker* t, const float* g, size_t pitch);
+int yas_tracker_prepare_frame_dev(yas_tracker* t, const float* vmap, size_t vpitch, const float* g, size_t gpitch,
+                                  const float* d, size_t dpitch);
 int yas_tracker_set_tracking_depth_dev(yas_tracker* t, const float* d, size_t pitch);
 int yas_rgb_to_gray_async(yas_ctx* ctx, const uint8_t* rgb, size_t sp, float* dst, size_t dp, int W, int H, int mode);
 
@@ -252,10 +254,9 @@ int yas_pipeline_step(yas_pipeline* p, const uint16_t* depth, size_t dpitch, con
     // TrackRGBD (Tracking.cpp:162-197)
     if ((rc = yas_raycast_dev(p->rc, p->map, p->T_last, ev[2]))) return rc;
     if ((rc = mark(3))) return rc;
-    if ((rc = yas_tracker_set_reference_model(p->trk, (const float*)yas_raycaster_vmap_dev(p->rc), (size_t)W * 16)))
+    if ((rc = yas_tracker_prepare_frame_dev(p->trk, (const float*)yas_raycaster_vmap_dev(p->rc), (size_t)W * 16,
+                                            p->gray, fp, p->depth, fp)))
       return rc;
-    if ((rc = yas_tracker_set_tracking_image_dev(p->trk, p->gray, fp))) return rc;
-    if ((rc = yas_tracker_set_tracking_depth_dev(p->trk, p->depth, fp))) return rc;
     if ((rc = mark(4))) return rc;
     if ((rc = yas_tracker_init_identity_async(p->trk))) return rc;
     if ((rc = yas_tracker_solve_async(p->trk))) return rc;

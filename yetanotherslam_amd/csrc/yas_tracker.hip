@@ -92,6 +92,64 @@ __global__ void __launch_bounds__(256) k_update(TrackState* st, const float* par
                                                 float wrh, float wrb, float wih, float wib) {
   update_at(st, part_icp, gi, part_rgb, gr, mode, first_of_level, do_step, record, wrh, wrb, wih, wib);
 }
+// A frame's tracker inputs in two launches per level instead of seven:
+// first the pointwise products of level l from level l - 1 (or from the
+// frame at level 0): intensity, inverse depth and vertices of the current
+// frame, and the model's vertices; then the neighbourhood products of level
+// l: gradients and both normal maps.  Each pixel gets the value the separate
+// kernels give it.
+struct FrameLevel {
+  int l, w, h, pw, ph;  // level, its size, the size of level l - 1
+  float invfx, invfy, cx, cy;
+  const float* gray;   // level 0 sources
+  size_t gp;
+  const float* depth;
+  size_t dp;
+  const float4* vmap;
+  size_t vp;
+  const float* int_prev;  // level l - 1
+  const float* inv_prev;
+  const float4* refv_prev;
+  float* cur_int;  // level l
+  float* cur_inv;
+  float4* cur_v;
+  float4* ref_v;
+  float* gx;
+  float* gy;
+  float4* cur_n;
+  float4* ref_n;
+};
+__global__ void __launch_bounds__(256) k_frame_level(FrameLevel a) {
+  const int x = blockIdx.x * 64 + (threadIdx.x & 63), y = blockIdx.y * 4 + (threadIdx.x >> 6);
+  if (x >= a.w || y >= a.h) return;
+  const size_t k = (size_t)y * a.w + x;
+  float in, iv;
+  float4 rv;
+  if (a.l == 0) {
+    in = ld(a.gray, a.gp, x, y);
+    iv = invdepth_of(ld(a.depth, a.dp, x, y));
+    rv = ((const float4*)((const char*)a.vmap + (size_t)y * a.vp))[x];
+  } else {
+    in = pyrdown_gauss_px(a.int_prev, (size_t)a.pw * 4, a.pw, a.ph, x, y);
+    iv = pyrdown_depth_px(a.inv_prev, (size_t)a.pw * 4, a.w, a.h, x, y);
+    rv = pyrdown_vmap_px(a.refv_prev, (size_t)a.pw * 16, a.w, a.h, x, y);
+  }
+  a.cur_int[k] = in;
+  a.cur_inv[k] = iv;
+  a.cur_v[k] = vertex_of(iv, x, y, a.invfx, a.invfy, a.cx, a.cy, 3.0f);
+  a.ref_v[k] = rv;
+}
+__global__ void __launch_bounds__(256) k_frame_level_nb(FrameLevel a) {
+  const int x = blockIdx.x * 64 + (threadIdx.x & 63), y = blockIdx.y * 4 + (threadIdx.x >> 6);
+  if (x >= a.w || y >= a.h) return;
+  const size_t k = (size_t)y * a.w + x;
+  float gxv, gyv;
+  gradient_px(a.cur_int, (size_t)a.w * 4, a.w, a.h, x, y, gxv, gyv);
+  a.gx[k] = gxv;
+  a.gy[k] = gyv;
+  a.cur_n[k] = normal_px(a.cur_v, (size_t)a.w * 16, a.w, a.h, x, y);
+  a.ref_n[k] = normal_px(a.ref_v, (size_t)a.w * 16, a.w, a.h, x, y);
+}
 __global__ void k_track_init(TrackState* st, Mat44 T) { track_init_at(st, T.m); }
 __global__ void k_track_init_identity(TrackState* st) { track_init_identity_at(st); }
 
@@ -254,6 +312,39 @@ int yas_tracker_set_tracking_image_dev(yas_tracker* t, const float* g, size_t pi
 }
 int yas_tracker_set_tracking_depth_dev(yas_tracker* t, const float* d, size_t pitch) {
   return depth_pyramid(t, d, pitch, t->cur_inv, t->cur_v, t->cur_n);
+}
+
+// The pipeline's per-frame tracker inputs: the model from a ray-cast vertex
+// map and the current frame from its intensity and depth, as
+// set_reference_model + set_tracking_image + set_tracking_depth give them.
+int yas_tracker_prepare_frame_dev(yas_tracker* t, const float* vmap, size_t vpitch, const float* g, size_t gpitch,
+                                  const float* d, size_t dpitch) {
+  YAS_REQUIRE(t && vmap && g && d && vpitch >= (size_t)t->W * 16, "yas_tracker_prepare_frame_dev: bad argument");
+  hipStream_t s = t->ctx->stream;
+  FrameLevel a[kMaxLevels];
+  for (int l = 0; l < t->P.levels; ++l) {
+    FrameLevel& f = a[l];
+    const Level& v = t->lv[l];
+    f.l = l;
+    f.w = v.w;
+    f.h = v.h;
+    f.pw = l > 0 ? t->lv[l - 1].w : 0;
+    f.ph = l > 0 ? t->lv[l - 1].h : 0;
+    f.invfx = v.invfx; f.invfy = v.invfy; f.cx = v.cx; f.cy = v.cy;
+    f.gray = g; f.gp = gpitch;
+    f.depth = d; f.dp = dpitch;
+    f.vmap = (const float4*)vmap; f.vp = vpitch;
+    f.int_prev = l > 0 ? t->cur_int[l - 1] : nullptr;
+    f.inv_prev = l > 0 ? t->cur_inv[l - 1] : nullptr;
+    f.refv_prev = l > 0 ? t->ref_v[l - 1] : nullptr;
+    f.cur_int = t->cur_int[l]; f.cur_inv = t->cur_inv[l]; f.cur_v = t->cur_v[l]; f.ref_v = t->ref_v[l];
+    f.gx = t->gx[l]; f.gy = t->gy[l]; f.cur_n = t->cur_n[l]; f.ref_n = t->ref_n[l];
+    hipLaunchKernelGGL(k_frame_level, grid64x4(v.w, v.h), dim3(256), 0, s, f);
+  }
+  for (int l = 0; l < t->P.levels; ++l)
+    hipLaunchKernelGGL(k_frame_level_nb, grid64x4(t->lv[l].w, t->lv[l].h), dim3(256), 0, s, a[l]);
+  YAS_LAUNCH_CHECK();
+  return YAS_OK;
 }
 
 int yas_rgb_to_gray_async(yas_ctx* ctx, const uint8_t* rgb, size_t sp, float* dst, size_t dp, int W, int H, int mode) {

@@ -164,11 +164,8 @@ __device__ inline int reflect101(int i, int n) {
   if (i >= n) i = 2 * n - 2 - i;
   return i;
 }
-__device__ __attribute__((always_inline)) inline void pyrdown_gauss_at(const float* src, size_t sp, int sw, int sh,
-                                                                       float* dst, size_t dp, int dw, int dh, int bx,
-                                                                       int by) {
-  int x = bx * 16 + (threadIdx.x & 15), y = by * 16 + (threadIdx.x >> 4);
-  if (x >= dw || y >= dh) return;
+__device__ __attribute__((always_inline)) inline float pyrdown_gauss_px(const float* src, size_t sp, int sw, int sh,
+                                                                       int x, int y) {
   const float k[5] = {0.0625f, 0.25f, 0.375f, 0.25f, 0.0625f};
   int rows[5];
 #pragma unroll
@@ -185,47 +182,64 @@ __device__ __attribute__((always_inline)) inline void pyrdown_gauss_at(const flo
   float acc = k[0] * col[0];
 #pragma unroll
   for (int j = 1; j < 5; ++j) acc = acc + k[j] * col[j];
-  ((float*)((char*)dst + (size_t)y * dp))[x] = acc;
+  return acc;
+}
+__device__ __attribute__((always_inline)) inline void pyrdown_gauss_at(const float* src, size_t sp, int sw, int sh,
+                                                                       float* dst, size_t dp, int dw, int dh, int bx,
+                                                                       int by) {
+  int x = bx * 16 + (threadIdx.x & 15), y = by * 16 + (threadIdx.x >> 4);
+  if (x >= dw || y >= dh) return;
+  ((float*)((char*)dst + (size_t)y * dp))[x] = pyrdown_gauss_px(src, sp, sw, sh, x, y);
 }
 
 // ImageProc.cu:7-25.
-__device__ inline void gradient_at(const float* s, size_t sp, float* gx, float* gy, size_t dp, int W,
-                                                  int H, int bx, int by) {
-  int x = bx * 64 + (threadIdx.x & 63), y = by * 4 + (threadIdx.x >> 6);
-  if (x >= W || y >= H) return;
-  float a = 0.0f, b = 0.0f;
+__device__ inline void gradient_px(const float* s, size_t sp, int W, int H, int x, int y, float& a, float& b) {
+  a = 0.0f;
+  b = 0.0f;
   if (!(x < 1 || y < 1 || x > W - 2 || y > H - 2)) {
     a = (ld(s, sp, x + 1, y) - ld(s, sp, x - 1, y)) * 0.5f;
     b = (ld(s, sp, x, y + 1) - ld(s, sp, x, y - 1)) * 0.5f;
   }
+}
+__device__ inline void gradient_at(const float* s, size_t sp, float* gx, float* gy, size_t dp, int W,
+                                                  int H, int bx, int by) {
+  int x = bx * 64 + (threadIdx.x & 63), y = by * 4 + (threadIdx.x >> 6);
+  if (x >= W || y >= H) return;
+  float a, b;
+  gradient_px(s, sp, W, H, x, y, a, b);
   ((float*)((char*)gx + (size_t)y * dp))[x] = a;
   ((float*)((char*)gy + (size_t)y * dp))[x] = b;
 }
 
 // ImageProc.cu:141-153.
+__device__ inline float invdepth_of(float v) { return (v > 0.2f && v < 8.0f) ? 1.0f / v : 0.0f; }
 __device__ inline void invdepth_at(const float* z, size_t sp, float* o, size_t dp, int W, int H, int bx, int by) {
   int x = bx * 64 + (threadIdx.x & 63), y = by * 4 + (threadIdx.x >> 6);
   if (x >= W || y >= H) return;
-  float v = ld(z, sp, x, y);
-  ((float*)((char*)o + (size_t)y * dp))[x] = (v > 0.2f && v < 8.0f) ? 1.0f / v : 0.0f;
+  ((float*)((char*)o + (size_t)y * dp))[x] = invdepth_of(ld(z, sp, x, y));
 }
 
 // ImageProc.cu:166-175, last row/column (left unwritten there) set to 0.
+__device__ inline float pyrdown_depth_px(const float* s, size_t sp, int W, int H, int x, int y) {
+  return (x < W - 1 && y < H - 1) ? ld(s, sp, 2 * x, 2 * y) : 0.0f;
+}
 __device__ inline void pyrdown_depth_at(const float* s, size_t sp, float* o, size_t dp, int W, int H, int bx, int by) {
   int x = bx * 64 + (threadIdx.x & 63), y = by * 4 + (threadIdx.x >> 6);
   if (x >= W || y >= H) return;
-  float v = (x < W - 1 && y < H - 1) ? ld(s, sp, 2 * x, 2 * y) : 0.0f;
-  ((float*)((char*)o + (size_t)y * dp))[x] = v;
+  ((float*)((char*)o + (size_t)y * dp))[x] = pyrdown_depth_px(s, sp, W, H, x, y);
 }
 
 // Whole-float4 subsampling of the reference model (departure from the
 // reference's float kernel run over float4 data, RGBDTracking.cu:146).
+__device__ inline float4 pyrdown_vmap_px(const float4* s, size_t sp, int W, int H, int x, int y) {
+  float4 v = make_float4(0.f, 0.f, 0.f, -1.f);
+  if (x < W - 1 && y < H - 1) v = ((const float4*)((const char*)s + (size_t)(2 * y) * sp))[2 * x];
+  return v;
+}
 __device__ inline void pyrdown_vmap_at(const float4* s, size_t sp, float4* o, size_t dp, int W, int H, int bx, int by) {
   int x = bx * 64 + (threadIdx.x & 63), y = by * 4 + (threadIdx.x >> 6);
   if (x >= W || y >= H) return;
-  float4 v = make_float4(0.f, 0.f, 0.f, -1.f);
-  if (x < W - 1 && y < H - 1) v = ((const float4*)((const char*)s + (size_t)(2 * y) * sp))[2 * x];
-  ((float4*)((char*)o + (size_t)y * dp))[x] = v;
+  ((float4*)((char*)o + (size_t)y * dp))[x] = pyrdown_vmap_px(s, sp, W, H, x, y);
 }
 
 __device__ inline void copy_f4_at(const float4* s, size_t sp, float4* o, int W, int H, int bx, int by) {
@@ -234,21 +248,21 @@ __device__ inline void copy_f4_at(const float4* s, size_t sp, float4* o, int W, 
 }
 
 // ImageProc.cu:189-209.
+__device__ inline float4 vertex_of(float iz, int x, int y, float invfx, float invfy, float cx, float cy, float cut) {
+  float z = 1.0f / iz;
+  float4 o = make_float4(0.f, 0.f, 0.f, -1.f);
+  if (iz > 0.0f && z < cut) o = make_float4(z * ((float)x - cx) * invfx, z * ((float)y - cy) * invfy, z, 1.0f);
+  return o;
+}
 __device__ inline void vertex_at(const float* inv, size_t sp, float4* v, size_t dp, int W, int H,
                                                 float invfx, float invfy, float cx, float cy, float cut, int bx, int by) {
   int x = bx * 64 + (threadIdx.x & 63), y = by * 4 + (threadIdx.x >> 6);
   if (x >= W || y >= H) return;
-  float iz = ld(inv, sp, x, y);
-  float z = 1.0f / iz;
-  float4 o = make_float4(0.f, 0.f, 0.f, -1.f);
-  if (iz > 0.0f && z < cut) o = make_float4(z * ((float)x - cx) * invfx, z * ((float)y - cy) * invfy, z, 1.0f);
-  ((float4*)((char*)v + (size_t)y * dp))[x] = o;
+  ((float4*)((char*)v + (size_t)y * dp))[x] = vertex_of(ld(inv, sp, x, y), x, y, invfx, invfy, cx, cy, cut);
 }
 
 // ImageProc.cu:222-249.
-__device__ inline void normal_at(const float4* v, size_t sp, float4* n, size_t dp, int W, int H, int bx, int by) {
-  int x = bx * 64 + (threadIdx.x & 63), y = by * 4 + (threadIdx.x >> 6);
-  if (x >= W || y >= H) return;
+__device__ inline float4 normal_px(const float4* v, size_t sp, int W, int H, int x, int y) {
   float4 o = make_float4(0.f, 0.f, 0.f, -1.f);
   if (x != W - 1 && y != H - 1) {
     const float4 a = ((const float4*)((const char*)v + (size_t)y * sp))[x];
@@ -266,7 +280,12 @@ __device__ inline void normal_at(const float4* v, size_t sp, float4* n, size_t d
       o = make_float4(r0, r1, r2, 1.0f);
     }
   }
-  ((float4*)((char*)n + (size_t)y * dp))[x] = o;
+  return o;
+}
+__device__ inline void normal_at(const float4* v, size_t sp, float4* n, size_t dp, int W, int H, int bx, int by) {
+  int x = bx * 64 + (threadIdx.x & 63), y = by * 4 + (threadIdx.x >> 6);
+  if (x >= W || y >= H) return;
+  ((float4*)((char*)n + (size_t)y * dp))[x] = normal_px(v, sp, W, H, x, y);
 }
 
 // ImageProc.cu:262-275.
