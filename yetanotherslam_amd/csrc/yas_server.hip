@@ -7,7 +7,7 @@
 // takes the oldest waiting frame of every stream (one per stream, since a
 // stream's frames depend on each other through its pose and reference
 // images) and launches them as one batch.  A batch is one replay of a
-// captured hipGraph of ~90 kernels whose grids have one z-slice per batch
+// captured hipGraph of about 75 kernels whose grids have one z-slice per batch
 // slot; every kernel reads its slot's stream from a small table, and every
 // stream keeps the workgroup counts of the single-stream pipeline, so a
 // stream's results are bit-identical alone and in any mix.  Batches queue
@@ -124,56 +124,63 @@ __global__ void __launch_bounds__(256) k_s_march(const StreamDev* streams, const
                S.step_scale, blockIdx.x);
 }
 
-// Tracker inputs.  `what`: 0 model copy, 1 model pyrDown, 2 model normals,
-// 3 intensity copy, 4 intensity pyrDown, 5 gradients, 6 inverse depth /
-// depth pyrDown, 7 vertices, 8 current normals, 9 keep (current -> reference).
-__global__ void __launch_bounds__(256) k_s_image(const StreamDev* streams, const Table* tab, int what, int l) {
+// Tracker inputs, two launches per pyramid level (the per-pixel functions
+// of the separate image kernels, so the same values): the pointwise
+// products of level l from level l - 1 (or the frame at level 0) -- model
+// vertices, intensity, inverse depth, vertices -- then the neighbourhood
+// products of level l -- model normals, gradients, normals.  A stream's
+// first frame has no model.
+__global__ void __launch_bounds__(256) k_s_frame_level(const StreamDev* streams, const Table* tab, int l) {
   SLOT_OR_RETURN;
   if (l >= S.levels) return;
   const Level& a = S.lv[l];
   const TrackBufs& B = S.tb[l];
-  const int bx = blockIdx.x, by = blockIdx.y;
-  const size_t p4 = (size_t)a.w * 4, p16 = (size_t)a.w * 16;
-  switch (what) {
-    case 0:
-      if (!sl.first) copy_f4_at(S.vmap, (size_t)S.W * 16, (float4*)B.ref_v, S.W, S.H, bx, by);
-      break;
-    case 1:
-      if (!sl.first)
-        pyrdown_vmap_at(S.tb[l - 1].ref_v, (size_t)S.lv[l - 1].w * 16, (float4*)B.ref_v, p16, a.w, a.h, bx, by);
-      break;
-    case 2:
-      if (!sl.first) normal_at(B.ref_v, p16, (float4*)B.ref_n, p16, a.w, a.h, bx, by);
-      break;
-    case 3:
-      copy_f32_at(S.gray, (size_t)S.W * 4, (float*)B.cur_int, S.W, S.H, bx, by);
-      break;
-    case 4:
-      if (by * 16 < a.h && bx * 16 < a.w)
-        pyrdown_gauss_at(S.tb[l - 1].cur_int, (size_t)S.lv[l - 1].w * 4, S.lv[l - 1].w, S.lv[l - 1].h,
-                         (float*)B.cur_int, p4, a.w, a.h, bx, by);
-      break;
-    case 5:
-      gradient_at(B.cur_int, p4, (float*)B.gx, (float*)B.gy, p4, a.w, a.h, bx, by);
-      break;
-    case 6:
-      if (l == 0) invdepth_at(S.depth, (size_t)S.W * 4, S.cur_inv[0], p4, a.w, a.h, bx, by);
-      else pyrdown_depth_at(S.cur_inv[l - 1], (size_t)S.lv[l - 1].w * 4, S.cur_inv[l], p4, a.w, a.h, bx, by);
-      break;
-    case 7:
-      vertex_at(S.cur_inv[l], p4, (float4*)B.cur_v, p16, a.w, a.h, a.invfx, a.invfy, a.cx, a.cy, 3.0f, bx, by);
-      break;
-    case 8:
-      normal_at(B.cur_v, p16, (float4*)B.cur_n, p16, a.w, a.h, bx, by);
-      break;
-    default: {
-      const int x = bx * 64 + (threadIdx.x & 63), y = by * 4 + (threadIdx.x >> 6);
-      if (x < a.w && y < a.h) {
-        const size_t k = (size_t)y * a.w + x;
-        ((float*)B.ref_int)[k] = B.cur_int[k];
-        ((float*)B.ref_inv)[k] = S.cur_inv[l][k];
-      }
-    }
+  const int x = blockIdx.x * 64 + (threadIdx.x & 63), y = blockIdx.y * 4 + (threadIdx.x >> 6);
+  if (x >= a.w || y >= a.h) return;
+  const size_t k = (size_t)y * a.w + x;
+  float in, iv;
+  if (l == 0) {
+    in = ld(S.gray, (size_t)S.W * 4, x, y);
+    iv = invdepth_of(ld(S.depth, (size_t)S.W * 4, x, y));
+    if (!sl.first) ((float4*)B.ref_v)[k] = ((const float4*)((const char*)S.vmap + (size_t)y * S.W * 16))[x];
+  } else {
+    const Level& p = S.lv[l - 1];
+    in = pyrdown_gauss_px(S.tb[l - 1].cur_int, (size_t)p.w * 4, p.w, p.h, x, y);
+    iv = pyrdown_depth_px(S.cur_inv[l - 1], (size_t)p.w * 4, a.w, a.h, x, y);
+    if (!sl.first) ((float4*)B.ref_v)[k] = pyrdown_vmap_px(S.tb[l - 1].ref_v, (size_t)p.w * 16, a.w, a.h, x, y);
+  }
+  ((float*)B.cur_int)[k] = in;
+  S.cur_inv[l][k] = iv;
+  ((float4*)B.cur_v)[k] = vertex_of(iv, x, y, a.invfx, a.invfy, a.cx, a.cy, 3.0f);
+}
+__global__ void __launch_bounds__(256) k_s_frame_level_nb(const StreamDev* streams, const Table* tab, int l) {
+  SLOT_OR_RETURN;
+  if (l >= S.levels) return;
+  const Level& a = S.lv[l];
+  const TrackBufs& B = S.tb[l];
+  const int x = blockIdx.x * 64 + (threadIdx.x & 63), y = blockIdx.y * 4 + (threadIdx.x >> 6);
+  if (x >= a.w || y >= a.h) return;
+  const size_t k = (size_t)y * a.w + x;
+  const size_t p16 = (size_t)a.w * 16;
+  if (!sl.first) ((float4*)B.ref_n)[k] = normal_px(B.ref_v, p16, a.w, a.h, x, y);
+  float gxv, gyv;
+  gradient_px(B.cur_int, (size_t)a.w * 4, a.w, a.h, x, y, gxv, gyv);
+  ((float*)B.gx)[k] = gxv;
+  ((float*)B.gy)[k] = gyv;
+  ((float4*)B.cur_n)[k] = normal_px(B.cur_v, p16, a.w, a.h, x, y);
+}
+// After tracking: the current frame's intensity and inverse depth become the
+// reference of the next frame (copies; the single pipeline swaps pointers).
+__global__ void __launch_bounds__(256) k_s_keep(const StreamDev* streams, const Table* tab, int l) {
+  SLOT_OR_RETURN;
+  if (l >= S.levels) return;
+  const Level& a = S.lv[l];
+  const TrackBufs& B = S.tb[l];
+  const int x = blockIdx.x * 64 + (threadIdx.x & 63), y = blockIdx.y * 4 + (threadIdx.x >> 6);
+  if (x < a.w && y < a.h) {
+    const size_t k = (size_t)y * a.w + x;
+    ((float*)B.ref_int)[k] = B.cur_int[k];
+    ((float*)B.ref_inv)[k] = S.cur_inv[l][k];
   }
 }
 
@@ -329,20 +336,8 @@ static int launch_batch_kernels(yas_server* sv, hipStream_t s, int nz) {
     default: hipLaunchKernelGGL(k_s_march<VoxF32>, mg, dim3(256), 0, s, S, T);
   }
   // tracker inputs: reference model pyramid, current intensity and depth pyramids
-  hipLaunchKernelGGL(k_s_image, level_grid(0), dim3(256), 0, s, S, T, 0, 0);
-  for (int l = 0; l < L; ++l) {
-    if (l > 0) hipLaunchKernelGGL(k_s_image, level_grid(l), dim3(256), 0, s, S, T, 1, l);
-    hipLaunchKernelGGL(k_s_image, level_grid(l), dim3(256), 0, s, S, T, 2, l);
-  }
-  hipLaunchKernelGGL(k_s_image, level_grid(0), dim3(256), 0, s, S, T, 3, 0);
-  for (int l = 1; l < L; ++l)
-    hipLaunchKernelGGL(k_s_image, dim3(((W >> l) + 15) / 16, ((H >> l) + 15) / 16, nz), dim3(256), 0, s, S, T, 4, l);
-  for (int l = 0; l < L; ++l) hipLaunchKernelGGL(k_s_image, level_grid(l), dim3(256), 0, s, S, T, 5, l);
-  for (int l = 0; l < L; ++l) {
-    hipLaunchKernelGGL(k_s_image, level_grid(l), dim3(256), 0, s, S, T, 6, l);
-    hipLaunchKernelGGL(k_s_image, level_grid(l), dim3(256), 0, s, S, T, 7, l);
-    hipLaunchKernelGGL(k_s_image, level_grid(l), dim3(256), 0, s, S, T, 8, l);
-  }
+  for (int l = 0; l < L; ++l) hipLaunchKernelGGL(k_s_frame_level, level_grid(l), dim3(256), 0, s, S, T, l);
+  for (int l = 0; l < L; ++l) hipLaunchKernelGGL(k_s_frame_level_nb, level_grid(l), dim3(256), 0, s, S, T, l);
   // Gauss-Newton, coarse to fine; optionally on sub-ranges of the slots in
   // turn, so one sub-range's pyramids stay in the last-level cache across
   // the iterations
@@ -366,7 +361,7 @@ static int launch_batch_kernels(yas_server* sv, hipStream_t s, int nz) {
   // gate, pose chain, result record
   hipLaunchKernelGGL(k_s_post, dim3(1, 1, nz), dim3(64), 0, s, S, T);
   // the current pyramids become the reference of the next frame
-  for (int l = 0; l < L; ++l) hipLaunchKernelGGL(k_s_image, level_grid(l), dim3(256), 0, s, S, T, 9, l);
+  for (int l = 0; l < L; ++l) hipLaunchKernelGGL(k_s_keep, level_grid(l), dim3(256), 0, s, S, T, l);
   // fusion at the new pose
   hipLaunchKernelGGL(k_s_pose, dim3(1, 1, nz), dim3(64), 0, s, S, T, 0);
   hipLaunchKernelGGL(k_s_alloc, dim3((W + 15) / 16, (H + 15) / 16, nz), dim3(256), 0, s, S, T);
