@@ -30,6 +30,12 @@ __device__ inline void zrange_init_at(unsigned* zmin, unsigned* zmax, int n, int
 }
 
 // RayTraceEngine.cu:80-144 (createRenderingBlock) + 184-214 (fill).
+// kReadFirst: read a cell before each atomic and skip the atomic when the
+// read already shows it would change nothing (a cell's zmin only falls and
+// its zmax only rises, so a stale read errs towards doing the atomic).  That
+// pays where many maps update cells at once (the batched launches); alone,
+// the atomics without a return value are cheaper than the read.
+template <bool kReadFirst = false>
 __device__ inline void zrange_at(const MapView& m, const PosePair* pose, const Intr& K, int cols, int rows,
                                  unsigned* zmin, unsigned* zmax, int bx0, int nb) {
   const int nv = m.ctr[kCtrVisible];
@@ -67,8 +73,12 @@ __device__ inline void zrange_at(const MapView& m, const PosePair* pose, const I
     const unsigned a = __float_as_uint(zr0), b = __float_as_uint(zr1);
     for (int y = ul1; y <= lr1; ++y)
       for (int x = ul0; x <= lr0; ++x) {
-        atomicMin(&zmin[y * cols + x], a);
-        atomicMax(&zmax[y * cols + x], b);
+        unsigned* pn = &zmin[y * cols + x];
+        unsigned* pf = &zmax[y * cols + x];
+        if (!kReadFirst || a < __hip_atomic_load(pn, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP))
+          atomicMin(pn, a);
+        if (!kReadFirst || b > __hip_atomic_load(pf, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP))
+          atomicMax(pf, b);
       }
   }
 }

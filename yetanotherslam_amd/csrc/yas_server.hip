@@ -111,7 +111,7 @@ __global__ void __launch_bounds__(256) k_s_zrange_init(const StreamDev* streams,
 __global__ void __launch_bounds__(256) k_s_zrange(const StreamDev* streams, const Table* tab) {
   SLOT_OR_RETURN;
   if (sl.first || (int)blockIdx.x >= S.zr_groups) return;
-  zrange_at(S.m, S.pose, S.K, S.cols, S.rows, S.zmin, S.zmax, blockIdx.x, S.zr_groups);
+  zrange_at<true>(S.m, S.pose, S.K, S.cols, S.rows, S.zmin, S.zmax, blockIdx.x, S.zr_groups);
 }
 
 template <class VT>
