@@ -217,10 +217,13 @@ __device__ inline bool march_ray(const MapView& m, float& rx, float& ry, float& 
 template <class VT>
 __device__ inline void march_at(const MapView& m, const PosePair* pose, const Intr& K, int W, int H, int cols,
                                 const float* zmin, const float* zmax, float4* vmap, float vsinv, float step_scale, int b) {
+  // A 16x16 tile per workgroup; each wavefront takes one 8x8 z-range cell,
+  // so its rays share a depth range and run for similar numbers of steps.
   const int tiles_x = (W + 15) >> 4;
   const int tile = xcd_tile(b, tiles_x * ((H + 15) >> 4));
-  const int x = (tile % tiles_x) * 16 + (threadIdx.x & 15);
-  const int y = (tile / tiles_x) * 16 + (threadIdx.x >> 4);
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int x = (tile % tiles_x) * 16 + (wave & 1) * 8 + (lane & 7);
+  const int y = (tile / tiles_x) * 16 + (wave >> 1) * 8 + (lane >> 3);
   if (x >= W || y >= H) return;
   float4 out = make_float4(__int_as_float(0x7fffffff), 0.0f, 0.0f, -1.0f);
   const float zn = zmin[(y >> 3) * cols + (x >> 3)];
