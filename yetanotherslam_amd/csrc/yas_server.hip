@@ -201,7 +201,7 @@ __global__ void __launch_bounds__(256) k_s_pass1(const StreamDev* streams, const
 __global__ void __launch_bounds__(256) k_s_pass2(const StreamDev* streams, const Table* tab, int l, int slot0) {
   SLOT_FROM_OR_RETURN(slot0);
   if (sl.first || (int)blockIdx.x >= S.groups[l]) return;
-  pass2_at(S.lv[l], S.tb[l], S.st, S.part_res, S.groups[l], S.part_rgb, blockIdx.x, S.groups[l]);
+  pass2_at<true>(S.lv[l], S.tb[l], S.st, S.part_res, S.groups[l], S.part_rgb, blockIdx.x, S.groups[l]);
 }
 
 __global__ void __launch_bounds__(256) k_s_update(const StreamDev* streams, const Table* tab, int l, int icp, int rgb,
