@@ -73,7 +73,7 @@ def cpu_baseline(p, K, depth, rgb, budget_s=20.0, max_frames=40):
 # rocprofv3 FETCH_SIZE / WRITE_SIZE passes scaled by a plain copy of known
 # size (profiles/dense_640x480_5mm_hbm_traffic.md).  The profiler cannot run
 # inside this script, so the measured figures are carried here.
-TRAFFIC_BYTES = {"k_march": 55.8e6, "k_integrate": 111.7e6, "k_alloc": 4.5e6}
+TRAFFIC_BYTES = {"k_march": 55.5e6, "k_integrate": 111.7e6, "k_alloc": 4.5e6}
 
 
 def walk_steps(stats):
