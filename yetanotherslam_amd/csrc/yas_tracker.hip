@@ -74,6 +74,10 @@ __global__ void __launch_bounds__(256) k_normal(const float4* v, size_t sp, floa
   normal_at(v, sp, n, dp, W, H, blockIdx.x, blockIdx.y);
 }
 
+__global__ void __launch_bounds__(256) k_icp_point(const float4* v, IcpPoint* o, int W, int H) {
+  icp_point_at(v, (size_t)W * 16, o, W, H, blockIdx.x, blockIdx.y);
+}
+
 __global__ void __launch_bounds__(256) k_vmap_to_depth(const float4* v, size_t sp, float* d, size_t dp, int W, int H) {
   vmap_to_depth_at(v, sp, d, dp, W, H, blockIdx.x, blockIdx.y);
 }
@@ -116,8 +120,8 @@ struct FrameLevel {
   float4* ref_v;
   float* gx;
   float* gy;
-  float4* cur_n;
-  float4* ref_n;
+  IcpPoint* cur_p;
+  IcpPoint* ref_p;
 };
 __global__ void __launch_bounds__(256) k_frame_level(FrameLevel a) {
   const int x = blockIdx.x * 64 + (threadIdx.x & 63), y = blockIdx.y * 4 + (threadIdx.x >> 6);
@@ -147,8 +151,8 @@ __global__ void __launch_bounds__(256) k_frame_level_nb(FrameLevel a) {
   gradient_px(a.cur_int, (size_t)a.w * 4, a.w, a.h, x, y, gxv, gyv);
   a.gx[k] = gxv;
   a.gy[k] = gyv;
-  a.cur_n[k] = normal_px(a.cur_v, (size_t)a.w * 16, a.w, a.h, x, y);
-  a.ref_n[k] = normal_px(a.ref_v, (size_t)a.w * 16, a.w, a.h, x, y);
+  a.cur_p[k] = icp_point_px(a.cur_v, (size_t)a.w * 16, a.w, a.h, x, y);
+  a.ref_p[k] = icp_point_px(a.ref_v, (size_t)a.w * 16, a.w, a.h, x, y);
 }
 __global__ void k_track_init(TrackState* st, Mat44 T) { track_init_at(st, T.m); }
 __global__ void k_track_init_identity(TrackState* st) { track_init_identity_at(st); }
@@ -165,7 +169,9 @@ struct yas_tracker {
   Level lv[kMaxLevels];
   float *cur_int[kMaxLevels], *ref_int[kMaxLevels], *gx[kMaxLevels], *gy[kMaxLevels];
   float *cur_inv[kMaxLevels], *ref_inv[kMaxLevels];
-  float4 *cur_v[kMaxLevels], *cur_n[kMaxLevels], *ref_v[kMaxLevels], *ref_n[kMaxLevels];
+  float4 *cur_v[kMaxLevels], *ref_v[kMaxLevels];
+  IcpPoint *cur_p[kMaxLevels], *ref_p[kMaxLevels];  // what the ICP term reads
+  float4 *cur_n[kMaxLevels], *ref_n[kMaxLevels];    // normal maps, formed on request (yas_tracker_level)
   float4* resid[kMaxLevels];
   float* part_icp;  // [kRedGroups][32]
   float* part_res;  // [kRedGroups][4]
@@ -194,7 +200,7 @@ static int image_pyramid(yas_tracker* t, const float* g, size_t pitch, float** p
   return YAS_OK;
 }
 
-static int depth_pyramid(yas_tracker* t, const float* d, size_t pitch, float** inv, float4** v, float4** n) {
+static int depth_pyramid(yas_tracker* t, const float* d, size_t pitch, float** inv, float4** v, IcpPoint** pt) {
   hipStream_t s = t->ctx->stream;
   for (int l = 0; l < t->P.levels; ++l) {
     const Level& a = t->lv[l];
@@ -205,8 +211,7 @@ static int depth_pyramid(yas_tracker* t, const float* d, size_t pitch, float** i
                          inv[l], (size_t)a.w * 4, a.w, a.h);
     hipLaunchKernelGGL(k_vertex, grid64x4(a.w, a.h), dim3(256), 0, s, inv[l], (size_t)a.w * 4, v[l], (size_t)a.w * 16,
                        a.w, a.h, a.invfx, a.invfy, a.cx, a.cy, 3.0f);
-    hipLaunchKernelGGL(k_normal, grid64x4(a.w, a.h), dim3(256), 0, s, v[l], (size_t)a.w * 16, n[l], (size_t)a.w * 16,
-                       a.w, a.h);
+    hipLaunchKernelGGL(k_icp_point, grid64x4(a.w, a.h), dim3(256), 0, s, v[l], pt[l], a.w, a.h);
   }
   YAS_LAUNCH_CHECK();
   return YAS_OK;
@@ -219,8 +224,7 @@ static int model_pyramid(yas_tracker* t) {
     if (l > 0)
       hipLaunchKernelGGL(k_pyrdown_vmap, grid64x4(a.w, a.h), dim3(256), 0, s, t->ref_v[l - 1],
                          (size_t)t->lv[l - 1].w * 16, t->ref_v[l], (size_t)a.w * 16, a.w, a.h);
-    hipLaunchKernelGGL(k_normal, grid64x4(a.w, a.h), dim3(256), 0, s, t->ref_v[l], (size_t)a.w * 16, t->ref_n[l],
-                       (size_t)a.w * 16, a.w, a.h);
+    hipLaunchKernelGGL(k_icp_point, grid64x4(a.w, a.h), dim3(256), 0, s, t->ref_v[l], t->ref_p[l], a.w, a.h);
   }
   YAS_LAUNCH_CHECK();
   return YAS_OK;
@@ -233,7 +237,7 @@ static int groups_for(const Level& a) {
 
 static TrackBufs bufs(yas_tracker* t, int l) {
   TrackBufs B;
-  B.ref_v = t->ref_v[l]; B.ref_n = t->ref_n[l]; B.cur_v = t->cur_v[l]; B.cur_n = t->cur_n[l];
+  B.ref_v = t->ref_v[l]; B.ref_p = t->ref_p[l]; B.cur_v = t->cur_v[l]; B.cur_p = t->cur_p[l];
   B.ref_inv = t->ref_inv[l]; B.ref_int = t->ref_int[l]; B.cur_int = t->cur_int[l];
   B.gx = t->gx[l]; B.gy = t->gy[l]; B.resid = t->resid[l];
   return B;
@@ -311,7 +315,7 @@ int yas_tracker_set_tracking_image_dev(yas_tracker* t, const float* g, size_t pi
   return image_pyramid(t, g, pitch, t->cur_int, true);
 }
 int yas_tracker_set_tracking_depth_dev(yas_tracker* t, const float* d, size_t pitch) {
-  return depth_pyramid(t, d, pitch, t->cur_inv, t->cur_v, t->cur_n);
+  return depth_pyramid(t, d, pitch, t->cur_inv, t->cur_v, t->cur_p);
 }
 
 // The pipeline's per-frame tracker inputs: the model from a ray-cast vertex
@@ -338,7 +342,7 @@ int yas_tracker_prepare_frame_dev(yas_tracker* t, const float* vmap, size_t vpit
     f.inv_prev = l > 0 ? t->cur_inv[l - 1] : nullptr;
     f.refv_prev = l > 0 ? t->ref_v[l - 1] : nullptr;
     f.cur_int = t->cur_int[l]; f.cur_inv = t->cur_inv[l]; f.cur_v = t->cur_v[l]; f.ref_v = t->ref_v[l];
-    f.gx = t->gx[l]; f.gy = t->gy[l]; f.cur_n = t->cur_n[l]; f.ref_n = t->ref_n[l];
+    f.gx = t->gx[l]; f.gy = t->gy[l]; f.cur_p = t->cur_p[l]; f.ref_p = t->ref_p[l];
     hipLaunchKernelGGL(k_frame_level, grid64x4(v.w, v.h), dim3(256), 0, s, f);
   }
   for (int l = 0; l < t->P.levels; ++l)
@@ -391,6 +395,7 @@ int yas_tracker_create(yas_ctx* ctx, int W, int H, const double K[4], const yas_
       alloc((void**)p, n * sizeof(float));
     for (float4** p : {&t->cur_v[l], &t->cur_n[l], &t->ref_v[l], &t->ref_n[l], &t->resid[l]})
       alloc((void**)p, n * sizeof(float4));
+    for (IcpPoint** p : {&t->cur_p[l], &t->ref_p[l]}) alloc((void**)p, n * sizeof(IcpPoint));
   }
   alloc((void**)&t->part_icp, sizeof(float) * 32 * kRedGroups);
   alloc((void**)&t->part_res, sizeof(float) * 4 * kRedGroups);
@@ -409,6 +414,7 @@ int yas_tracker_create(yas_ctx* ctx, int W, int H, const double K[4], const yas_
       YAS_HIP(hipMemsetAsync(p, 0, n * sizeof(float), ctx->stream));
     for (float4* p : {t->cur_v[l], t->cur_n[l], t->ref_v[l], t->ref_n[l], t->resid[l]})
       YAS_HIP(hipMemsetAsync(p, 0, n * sizeof(float4), ctx->stream));
+    for (IcpPoint* p : {t->cur_p[l], t->ref_p[l]}) YAS_HIP(hipMemsetAsync(p, 0, n * sizeof(IcpPoint), ctx->stream));
   }
   YAS_HIP(hipMemsetAsync(t->st, 0, sizeof(TrackState), ctx->stream));
   int rc = yas_tracker_init_identity_async(t);
@@ -427,6 +433,7 @@ int yas_tracker_destroy(yas_tracker* t) {
   for (int l = 0; l < kMaxLevels; ++l) {
     for (float* p : {t->cur_int[l], t->ref_int[l], t->gx[l], t->gy[l], t->cur_inv[l], t->ref_inv[l]}) hipFree(p);
     for (float4* p : {t->cur_v[l], t->cur_n[l], t->ref_v[l], t->ref_n[l], t->resid[l]}) hipFree(p);
+    for (IcpPoint* p : {t->cur_p[l], t->ref_p[l]}) hipFree(p);
   }
   hipFree(t->part_icp);
   hipFree(t->part_res);
@@ -447,11 +454,11 @@ int yas_tracker_set_tracking_image(yas_tracker* t, const float* g, size_t pitch)
 }
 int yas_tracker_set_reference_depth(yas_tracker* t, const float* d, size_t pitch) {
   YAS_REQUIRE(t && d && pitch >= (size_t)t->W * 4, "yas_tracker_set_reference_depth: bad argument");
-  return depth_pyramid(t, d, pitch, t->ref_inv, t->ref_v, t->ref_n);
+  return depth_pyramid(t, d, pitch, t->ref_inv, t->ref_v, t->ref_p);
 }
 int yas_tracker_set_tracking_depth(yas_tracker* t, const float* d, size_t pitch) {
   YAS_REQUIRE(t && d && pitch >= (size_t)t->W * 4, "yas_tracker_set_tracking_depth: bad argument");
-  return depth_pyramid(t, d, pitch, t->cur_inv, t->cur_v, t->cur_n);
+  return depth_pyramid(t, d, pitch, t->cur_inv, t->cur_v, t->cur_p);
 }
 int yas_tracker_set_reference_model(yas_tracker* t, const float* vmap, size_t pitch) {
   YAS_REQUIRE(t && vmap && pitch >= (size_t)t->W * 16, "yas_tracker_set_reference_model: bad argument");
@@ -465,7 +472,7 @@ int yas_tracker_swap_frame_buffer(yas_tracker* t) {
   YAS_REQUIRE(t, "null tracker");
   for (int l = 0; l < kMaxLevels; ++l) {
     std::swap(t->ref_v[l], t->cur_v[l]);
-    std::swap(t->ref_n[l], t->cur_n[l]);
+    std::swap(t->ref_p[l], t->cur_p[l]);
     std::swap(t->ref_inv[l], t->cur_inv[l]);
     std::swap(t->ref_int[l], t->cur_int[l]);
   }
@@ -531,6 +538,16 @@ int yas_tracker_level(yas_tracker* t, int which, int level, void** ptr, size_t* 
   void* one[6] = {t->cur_int[level], t->ref_int[level], t->gx[level], t->gy[level], t->cur_inv[level], t->ref_inv[level]};
   void* four[5] = {t->cur_v[level], t->cur_n[level], t->ref_v[level], t->ref_n[level], t->resid[level]};
   YAS_REQUIRE(which >= 0 && which < 11, "bad buffer id");
+  if (which == 7 || which == 9) {
+    // The passes read vertex and normal as one IcpPoint; the float4 normal
+    // map is formed here from the vertex map, as the pyramid kernels would.
+    YAS_HIP(hipSetDevice(t->ctx->device));
+    hipLaunchKernelGGL(k_normal, grid64x4(a.w, a.h), dim3(256), 0, t->ctx->stream,
+                       which == 7 ? t->cur_v[level] : t->ref_v[level], (size_t)a.w * 16,
+                       which == 7 ? t->cur_n[level] : t->ref_n[level], (size_t)a.w * 16, a.w, a.h);
+    YAS_LAUNCH_CHECK();
+    YAS_HIP(hipStreamSynchronize(t->ctx->stream));
+  }
   *ptr = which < 6 ? one[which] : four[which - 6];
   if (pitch) *pitch = (size_t)a.w * (which < 6 ? 4 : 16);
   if (w) *w = a.w;

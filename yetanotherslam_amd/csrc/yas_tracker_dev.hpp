@@ -288,6 +288,30 @@ __device__ inline void normal_at(const float4* v, size_t sp, float4* n, size_t d
   ((float4*)((char*)n + (size_t)y * dp))[x] = normal_px(v, sp, W, H, x, y);
 }
 
+// What the ICP association reads of a pixel: its vertex and normal, 24
+// bytes instead of two float4.  A normal exists (w = 1) only where the
+// pixel's own vertex exists (normal_px tests it), and the association needs
+// both, so a pixel without a normal stores a NaN x instead of the two w
+// flags: the reference side skips it, and on the current side the NaN
+// distance fails the 1 cm gate, as the w tests would.
+struct __attribute__((aligned(8))) IcpPoint {
+  float vx, vy, vz, nx, ny, nz;
+};
+__device__ inline IcpPoint icp_point_px(const float4* v, size_t sp, int W, int H, int x, int y) {
+  const float4 a = ((const float4*)((const char*)v + (size_t)y * sp))[x];
+  const float4 n = normal_px(v, sp, W, H, x, y);
+  IcpPoint p;
+  p.vx = n.w > 0.0f ? a.x : __builtin_nanf("");
+  p.vy = a.y; p.vz = a.z;
+  p.nx = n.x; p.ny = n.y; p.nz = n.z;
+  return p;
+}
+__device__ inline void icp_point_at(const float4* v, size_t sp, IcpPoint* o, int W, int H, int bx, int by) {
+  int x = bx * 64 + (threadIdx.x & 63), y = by * 4 + (threadIdx.x >> 6);
+  if (x >= W || y >= H) return;
+  o[(size_t)y * W + x] = icp_point_px(v, sp, W, H, x, y);
+}
+
 // ImageProc.cu:262-275.
 __device__ inline void vmap_to_depth_at(const float4* v, size_t sp, float* d, size_t dp, int W, int H, int bx, int by) {
   int x = bx * 64 + (threadIdx.x & 63), y = by * 4 + (threadIdx.x >> 6);
@@ -299,9 +323,9 @@ __device__ inline void vmap_to_depth_at(const float4* v, size_t sp, float* d, si
 // ------------------------------------------------------------------ per-pixel terms
 struct TrackBufs {
   const float4* ref_v;
-  const float4* ref_n;
+  const IcpPoint* ref_p;
   const float4* cur_v;
-  const float4* cur_n;
+  const IcpPoint* cur_p;
   const float* ref_inv;
   const float* ref_int;
   const float* cur_int;
@@ -312,32 +336,31 @@ struct TrackBufs {
 
 // RGBDTracking.cu:316-372: projective association of reference pixel k into
 // the current frame, gated at sin(20 deg) normal angle and 1 cm; on a match
-// the point-to-plane row [n, v x n, n.(v_c - v)] is accumulated.
+// the point-to-plane row [n, v x n, n.(v_c - v)] is accumulated.  Only
+// pixels with a vertex and a normal on both sides can match (IcpPoint).
 __device__ inline void icp_pixel(const Level& L, const TrackBufs& B, const PoseF& T, float angle_th, float dist_th,
-                                 const float4& vl, const float4& nl, float (&acc)[32]) {
-  if (!(vl.w >= 0.0f)) return;
+                                 const IcpPoint& r, float (&acc)[32]) {
+  if (!(r.vx == r.vx)) return;
   float vx, vy, vz;
-  xform(T, vl.x, vl.y, vl.z, vx, vy, vz);
+  xform(T, r.vx, r.vy, r.vz, vx, vy, vz);
   const float invz = 1.0f / vz;
   const int u = floor_sat(L.fx * vx * invz + L.cx + 0.5f);
   const int v = floor_sat(L.fy * vy * invz + L.cy + 0.5f);
   if (u < 0 || v < 0 || u >= L.w || v >= L.h) return;
-  const float4 vc = B.cur_v[v * L.w + u];
-  const float4 nc = B.cur_n[v * L.w + u];  // read with the vertex: one round trip
-  if (vc.w < 0.0f) return;
+  const IcpPoint q = B.cur_p[v * L.w + u];
   float nx, ny, nz;
-  rot3(T, nl.x, nl.y, nl.z, nx, ny, nz);
-  const float dx = vx - vc.x, dy = vy - vc.y, dz = vz - vc.z;
+  rot3(T, r.nx, r.ny, r.nz, nx, ny, nz);
+  const float dx = vx - q.vx, dy = vy - q.vy, dz = vz - q.vz;
   const float dist = sqrtf(dx * dx + dy * dy + dz * dz);
-  const float c0 = nc.y * nz - nc.z * ny, c1 = nc.z * nx - nc.x * nz, c2 = nc.x * ny - nc.y * nx;
+  const float c0 = q.ny * nz - q.nz * ny, c1 = q.nz * nx - q.nx * nz, c2 = q.nx * ny - q.ny * nx;
   const float ang = sqrtf(c0 * c0 + c1 * c1 + c2 * c2);
-  if (!(ang < angle_th && dist <= dist_th && nl.w > 0.0f && nc.w > 0.0f)) return;
+  if (!(ang < angle_th && dist <= dist_th)) return;  // a NaN distance (no current point) fails here
   float row[7];
   row[0] = nx; row[1] = ny; row[2] = nz;
   row[3] = vy * nz - vz * ny;
   row[4] = vz * nx - vx * nz;
   row[5] = vx * ny - vy * nx;
-  row[6] = nx * (vc.x - vx) + ny * (vc.y - vy) + nz * (vc.z - vz);
+  row[6] = nx * (q.vx - vx) + ny * (q.vy - vy) + nz * (q.vz - vz);
   int c = 0;
 #pragma unroll
   for (int i = 0; i < 7; ++i)
@@ -415,16 +438,13 @@ __device__ inline void rgb_pixel(const Level& L, const TrackBufs& B, const float
 // Per-pixel reference values that pass 1 reads at pixel k before any
 // dependent (gathered) read.
 struct RefPixel {
-  float4 v, n;      // model vertex and normal (ICP)
+  IcpPoint p;       // model vertex and normal (ICP)
   float iz, inten;  // inverse depth and intensity (photometric)
 };
 template <bool ICP, bool RGB>
 __device__ inline RefPixel load_ref_pixel(const TrackBufs& B, int k) {
   RefPixel r;
-  if (ICP) {
-    r.v = B.ref_v[k];
-    r.n = B.ref_n[k];
-  }
+  if (ICP) r.p = B.ref_p[k];
   if (RGB) {
     r.iz = B.ref_inv[k];
     r.inten = B.ref_int[k];
@@ -468,7 +488,7 @@ __device__ inline void pass1_at(const Level& L, const TrackBufs& B, const TrackS
     } else {
       cur = load_ref_pixel<ICP, RGB>(B, k);
     }
-    if (ICP) icp_pixel(L, B, T, angle_th, dist_th, cur.v, cur.n, acc);
+    if (ICP) icp_pixel(L, B, T, angle_th, dist_th, cur.p, acc);
     if (RGB) rgb_pixel(L, B, wu, k, cur.iz, cur.inten, n, mean, m2);
   }
   if (ICP) {
