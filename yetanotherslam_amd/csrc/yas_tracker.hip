@@ -17,6 +17,9 @@
 //   update   one workgroup folds the partial rows in a fixed order, solves
 //            the 6x6 system in FP64 (the reference's Eigen LDLT, restated),
 //            applies T <- exp(xi) T in FP64 and keeps the best pose.
+// A full solve takes each update but the last at the start of the next
+// pass 1 instead: every workgroup of that pass folds the same rows and takes
+// the same step, one launch fewer per iteration with the same values.
 // Results are deterministic run to run.
 #include <cfloat>
 #include <cmath>
@@ -96,12 +99,23 @@ __global__ void __launch_bounds__(256) k_update(TrackState* st, const float* par
                                                 float wrh, float wrb, float wih, float wib) {
   update_at(st, part_icp, gi, part_rgb, gr, mode, first_of_level, do_step, record, wrh, wrb, wih, wib);
 }
+// The last step of a solve, from the state buffer the last pass used into the tracker's state.
+__global__ void __launch_bounds__(256) k_update_into(const TrackState* st_in, TrackState* st_out, StepArgs a) {
+  gn_update(st_in, st_out, true, a.part_icp, a.gi, a.part_rgb, a.gr, a.mode, a.first_of_level, 1, a.record, a.wrh,
+            a.wrb, a.wih, a.wib, nullptr);
+}
+// Step of the previous linearisation, then pass 1 at the new pose.
+template <bool ICP, bool RGB>
+__global__ void __launch_bounds__(256) k_step_pass1(Level L, TrackBufs B, StepArgs a, float* part_icp,
+                                                    float* part_res, float angle_th, float dist_th) {
+  pass1_at<ICP, RGB, true, true>(L, B, nullptr, part_icp, part_res, angle_th, dist_th, blockIdx.x, gridDim.x, &a);
+}
 // A frame's tracker inputs in two launches per level instead of seven:
 // first the pointwise products of level l from level l - 1 (or from the
 // frame at level 0): intensity, inverse depth and vertices of the current
 // frame, and the model's vertices; then the neighbourhood products of level
-// l: gradients and both normal maps.  Each pixel gets the value the separate
-// kernels give it.
+// l: gradients and both ICP point maps (vertex and normal).  Each pixel gets
+// the value the separate kernels give it.
 struct FrameLevel {
   int l, w, h, pw, ph;  // level, its size, the size of level l - 1
   float invfx, invfy, cx, cy;
@@ -174,9 +188,11 @@ struct yas_tracker {
   float4 *cur_n[kMaxLevels], *ref_n[kMaxLevels];    // normal maps, formed on request (yas_tracker_level)
   float4* resid[kMaxLevels];
   float* part_icp;  // [kRedGroups][32]
+  float* part_icp2;  // the other of the pair a solve alternates between
   float* part_res;  // [kRedGroups][4]
   float* part_rgb;  // [kRedGroups][32]
   TrackState* st;
+  TrackState* st2;  // a solve's steps alternate between st and st2
   TrackState* host_st;
   float angle_th, dist_th;
 };
@@ -268,14 +284,63 @@ static int launch_iteration(yas_tracker* t, int l, bool icp, bool rgb, bool do_s
   return YAS_OK;
 }
 
-// Coarse-to-fine Gauss-Newton from the pose already in the device state.
+// Coarse-to-fine Gauss-Newton from the pose already in the device state:
+// the same linearisations and steps as launch_iteration in sequence, with
+// each step (but the last) taken at the start of the next pass 1, which
+// saves one launch per iteration.  Step i reads the state buffer of step
+// i - 1 and writes the other one, and the ICP partial rows alternate the
+// same way, since that pass 1 reads the previous rows while writing its own.
 int yas_tracker_solve_async(yas_tracker* t) {
   const bool icp = t->P.mode != YAS_TRACK_RGB, rgb = t->P.mode != YAS_TRACK_DEPTH;
+  hipStream_t s = t->ctx->stream;
+  TrackState* S[2] = {t->st, t->st2};
+  float* PI[2] = {t->part_icp, t->part_icp2};
+  StepArgs a = {};
+  int i = 0;
   for (int l = t->P.levels - 1; l >= 0; --l)
-    for (int it = 0; it < t->P.iters[l]; ++it) {
-      int rc = launch_iteration(t, l, icp, rgb, true, it == 0);
-      if (rc) return rc;
+    for (int it = 0; it < t->P.iters[l]; ++it, ++i) {
+      const Level& lv = t->lv[l];
+      const int G = groups_for(lv);
+      const TrackBufs B = bufs(t, l);
+      TrackState* cur = S[i & 1];
+      float* pic = PI[i & 1];
+      if (i == 0) {
+        if (icp && rgb)
+          hipLaunchKernelGGL((k_pass1<true, true>), dim3(G), dim3(256), 0, s, lv, B, (const TrackState*)cur, pic,
+                             t->part_res, t->angle_th, t->dist_th);
+        else if (icp)
+          hipLaunchKernelGGL((k_pass1<true, false>), dim3(G), dim3(256), 0, s, lv, B, (const TrackState*)cur, pic,
+                             t->part_res, t->angle_th, t->dist_th);
+        else
+          hipLaunchKernelGGL((k_pass1<false, true>), dim3(G), dim3(256), 0, s, lv, B, (const TrackState*)cur, pic,
+                             t->part_res, t->angle_th, t->dist_th);
+      } else {
+        a.st_in = S[(i - 1) & 1];
+        a.st_out = cur;
+        if (icp && rgb)
+          hipLaunchKernelGGL((k_step_pass1<true, true>), dim3(G), dim3(256), 0, s, lv, B, a, pic, t->part_res,
+                             t->angle_th, t->dist_th);
+        else if (icp)
+          hipLaunchKernelGGL((k_step_pass1<true, false>), dim3(G), dim3(256), 0, s, lv, B, a, pic, t->part_res,
+                             t->angle_th, t->dist_th);
+        else
+          hipLaunchKernelGGL((k_step_pass1<false, true>), dim3(G), dim3(256), 0, s, lv, B, a, pic, t->part_res,
+                             t->angle_th, t->dist_th);
+      }
+      if (rgb)
+        hipLaunchKernelGGL(k_pass2, dim3(G), dim3(256), 0, s, lv, B, cur, (const float*)t->part_res, G, t->part_rgb);
+      // the step of this linearisation, taken by the next launch
+      a.part_icp = pic;
+      a.gi = icp ? G : 0;
+      a.part_rgb = t->part_rgb;
+      a.gr = rgb ? G : 0;
+      a.mode = t->P.mode;
+      a.first_of_level = it == 0 ? 1 : 0;
+      a.record = l == 0 ? 1 : 0;
+      a.wrh = t->P.w_rgb_h; a.wrb = t->P.w_rgb_b; a.wih = t->P.w_icp_h; a.wib = t->P.w_icp_b;
     }
+  if (i > 0) hipLaunchKernelGGL(k_update_into, dim3(1), dim3(256), 0, s, (const TrackState*)S[(i - 1) & 1], t->st, a);
+  YAS_LAUNCH_CHECK();
   return YAS_OK;
 }
 
@@ -398,9 +463,11 @@ int yas_tracker_create(yas_ctx* ctx, int W, int H, const double K[4], const yas_
     for (IcpPoint** p : {&t->cur_p[l], &t->ref_p[l]}) alloc((void**)p, n * sizeof(IcpPoint));
   }
   alloc((void**)&t->part_icp, sizeof(float) * 32 * kRedGroups);
+  alloc((void**)&t->part_icp2, sizeof(float) * 32 * kRedGroups);
   alloc((void**)&t->part_res, sizeof(float) * 4 * kRedGroups);
   alloc((void**)&t->part_rgb, sizeof(float) * 32 * kRedGroups);
   alloc((void**)&t->st, sizeof(TrackState));
+  alloc((void**)&t->st2, sizeof(TrackState));
   if (e == hipSuccess) e = hipHostMalloc((void**)&t->host_st, sizeof(TrackState), hipHostMallocDefault);
   if (e != hipSuccess) {
     yas_set_error("yas_tracker_create: %s", hipGetErrorString(e));
@@ -417,6 +484,7 @@ int yas_tracker_create(yas_ctx* ctx, int W, int H, const double K[4], const yas_
     for (IcpPoint* p : {t->cur_p[l], t->ref_p[l]}) YAS_HIP(hipMemsetAsync(p, 0, n * sizeof(IcpPoint), ctx->stream));
   }
   YAS_HIP(hipMemsetAsync(t->st, 0, sizeof(TrackState), ctx->stream));
+  YAS_HIP(hipMemsetAsync(t->st2, 0, sizeof(TrackState), ctx->stream));
   int rc = yas_tracker_init_identity_async(t);
   if (rc) {
     yas_tracker_destroy(t);
@@ -436,9 +504,11 @@ int yas_tracker_destroy(yas_tracker* t) {
     for (IcpPoint* p : {t->cur_p[l], t->ref_p[l]}) hipFree(p);
   }
   hipFree(t->part_icp);
+  hipFree(t->part_icp2);
   hipFree(t->part_res);
   hipFree(t->part_rgb);
   hipFree(t->st);
+  hipFree(t->st2);
   if (t->host_st) hipHostFree(t->host_st);
   delete t;
   return YAS_OK;

@@ -455,19 +455,51 @@ __device__ inline RefPixel load_ref_pixel(const TrackBufs& B, int k) {
 // PREFETCH: the next pixel's reference values are loaded before the
 // current pixel's gathers, so their latencies overlap (more registers; the
 // values and the order of the sums are the same).
-template <bool ICP, bool RGB, bool PREFETCH = false>
+// The Gauss-Newton step of the previous linearisation, taken by a pass 1
+// before it linearises (StepArgs): every workgroup folds the same partial
+// rows and takes the same step, so all get the same pose; workgroup 0 also
+// stores the new state.  The step reads st_in and writes st_out, a
+// different buffer, because other workgroups may still be reading st_in.
+struct StepArgs {
+  const TrackState* st_in;
+  TrackState* st_out;
+  const float* part_icp;
+  int gi;
+  const float* part_rgb;
+  int gr;
+  int mode, first_of_level, record;
+  float wrh, wrb, wih, wib;
+};
+__device__ __attribute__((always_inline)) inline void gn_update(const TrackState* st_in, TrackState* st_out,
+                                                                bool write, const float* part_icp, int gi,
+                                                                const float* part_rgb, int gr, int mode,
+                                                                int first_of_level, int do_step, int record, float wrh,
+                                                                float wrb, float wih, float wib, double* est_out);
+
+template <bool ICP, bool RGB, bool PREFETCH = false, bool STEP = false>
 __device__ inline void pass1_at(const Level& L, const TrackBufs& B, const TrackState* st, float* part_icp,
-                                float* part_res, float angle_th, float dist_th, int bx, int nb) {
+                                float* part_res, float angle_th, float dist_th, int bx, int nb,
+                                const StepArgs* sa = nullptr) {
   __shared__ float pose[12];
   __shared__ float wp[12];
   __shared__ float lds[4 * 32];
+  __shared__ double s_est[STEP ? 16 : 1];
   const int N = L.w * L.h;
   const int stride = nb * blockDim.x;
   int k = bx * blockDim.x + threadIdx.x;
   RefPixel nxt = {};
   if (PREFETCH && k < N) nxt = load_ref_pixel<ICP, RGB>(B, k);  // in flight while the pose loads
-  load_pose_f(st->est, pose);
-  if (RGB) load_warp(st->est, L.Kd, wp);
+  const double* est;
+  if (STEP) {
+    gn_update(sa->st_in, sa->st_out, bx == 0, sa->part_icp, sa->gi, sa->part_rgb, sa->gr, sa->mode,
+              sa->first_of_level, 1, sa->record, sa->wrh, sa->wrb, sa->wih, sa->wib, s_est);
+    __syncthreads();
+    est = s_est;
+  } else {
+    est = st->est;
+  }
+  load_pose_f(est, pose);
+  if (RGB) load_warp(est, L.Kd, wp);
   __syncthreads();
   PoseF T;
 #pragma unroll
@@ -721,56 +753,21 @@ __device__ inline void unpack29(const float* s, float* H, float* b) {
     }
 }
 
-// Fold the partial rows of one linearisation and, when `do_step`, take one
-// Gauss-Newton step of RGBDTracking.cu:161-204.  One workgroup.
-__device__ inline void update_at(TrackState* st, const float* part_icp, int gi, const float* part_rgb, int gr,
-                                 int mode, int first_of_level, int do_step, int record, float wrh, float wrb, float wih,
-                                 float wib) {
-  __shared__ float lds[4 * 32];
-  __shared__ float sums[2][32];
-  __shared__ double s_est[16];
-  __shared__ int s_good;
-  __shared__ float s_last_err;
-  // the state the step reads, loaded alongside the partial rows
-  if (threadIdx.x < 16) s_est[threadIdx.x] = st->est[threadIdx.x];
-  if (threadIdx.x == 16) s_good = st->good;
-  if (threadIdx.x == 17) s_last_err = st->last_err;
-  // both terms' rows are read in one pass, so their loads overlap
-  float vi[32], vr[32];
-#pragma unroll
-  for (int i = 0; i < 32; ++i) vi[i] = vr[i] = 0.0f;
-  const int g = gi > gr ? gi : gr;
-  for (int r = threadIdx.x; r < g; r += blockDim.x) {
-    if (r < gi) {
-#pragma unroll
-      for (int i = 0; i < 32; ++i) vi[i] += part_icp[r * 32 + i];
-    }
-    if (r < gr) {
-#pragma unroll
-      for (int i = 0; i < 32; ++i) vr[i] += part_rgb[r * 32 + i];
-    }
-  }
-  if (gi) {
-    const float t = block_sum32(vi, lds);
-    if (threadIdx.x < 32) sums[0][threadIdx.x] = t;
-  }
-  if (gr) {
-    const float t = block_sum32(vr, lds);
-    if (threadIdx.x < 32) sums[1][threadIdx.x] = t;
-  }
-  __syncthreads();
-  if (threadIdx.x < 29) {
-    if (gi) st->sums_icp[threadIdx.x] = sums[0][threadIdx.x];
-    if (gr) st->sums_rgb[threadIdx.x] = sums[1][threadIdx.x];
-  }
-  if (threadIdx.x != 0) return;
-  if (record && gi) st->icp_inliers = (int)sums[0][28];
-  if (record && gr) st->rgb_inliers = (int)sums[1][28];
+// Thread 0's part of gn_update: inlier counts and, when do_step and the
+// state is good, the step from the pose s_est.  Writes the state's fields to
+// st and the stepped pose to est_out (each when not null).
+__device__ __attribute__((always_inline)) inline void gn_step_lane0(const float (*sums)[32], const double* s_est,
+                                                                    int s_good, float s_last_err, TrackState* st,
+                                                                    double* est_out, int gi, int gr, int mode,
+                                                                    int first_of_level, int do_step, int record,
+                                                                    float wrh, float wrb, float wih, float wib) {
+  if (st && record && gi) st->icp_inliers = (int)sums[0][28];
+  if (st && record && gr) st->rgb_inliers = (int)sums[1][28];
   if (!do_step) return;
   float last_err = s_last_err;
   if (first_of_level) {
     last_err = FLT_MAX;
-    st->last_err = FLT_MAX;
+    if (st) st->last_err = FLT_MAX;
   }
   if (!s_good) return;
   float H[36], b[6];
@@ -811,7 +808,7 @@ __device__ inline void update_at(TrackState* st, const float* part_icp, int gi, 
 #pragma unroll
   for (int i = 0; i < 6; ++i) xi[i] = y[i];
   if (isnan(xi[0])) {
-    st->good = 0;
+    if (st) st->good = 0;
     return;
   }
   double E[16], est[16];
@@ -819,6 +816,11 @@ __device__ inline void update_at(TrackState* st, const float* part_icp, int gi, 
 #pragma unroll
   for (int i = 0; i < 16; ++i) est[i] = s_est[i];
   mul44(E, est, est);
+  if (est_out) {
+#pragma unroll
+    for (int i = 0; i < 16; ++i) est_out[i] = est[i];
+  }
+  if (!st) return;
 #pragma unroll
   for (int i = 0; i < 16; ++i) st->est[i] = est[i];
   if (err < last_err) {
@@ -826,6 +828,73 @@ __device__ inline void update_at(TrackState* st, const float* part_icp, int gi, 
     for (int i = 0; i < 16; ++i) st->last_ok[i] = est[i];
     st->last_err = err;
   }
+}
+
+// Fold the partial rows of one linearisation and, when `do_step`, take one
+// Gauss-Newton step of RGBDTracking.cu:161-204.  One workgroup.  The state
+// is read from st_in; when `write`, st_out receives it with the fold's and
+// the step's changes (st_out may be st_in).  When est_out (LDS) is given,
+// thread 0 leaves the pose after the step there.
+__device__ __attribute__((always_inline)) inline void gn_update(const TrackState* st_in, TrackState* st_out,
+                                                                bool write, const float* part_icp, int gi,
+                                                                const float* part_rgb, int gr, int mode,
+                                                                int first_of_level, int do_step, int record, float wrh,
+                                                                float wrb, float wih, float wib, double* est_out) {
+  __shared__ float lds[4 * 32];
+  __shared__ float sums[2][32];
+  __shared__ double s_est[16];
+  __shared__ int s_good;
+  __shared__ float s_last_err;
+  // the state the step reads, loaded alongside the partial rows
+  if (threadIdx.x < 16) s_est[threadIdx.x] = st_in->est[threadIdx.x];
+  if (threadIdx.x == 16) s_good = st_in->good;
+  if (threadIdx.x == 17) s_last_err = st_in->last_err;
+  // both terms' rows are read in one pass, so their loads overlap
+  float vi[32], vr[32];
+#pragma unroll
+  for (int i = 0; i < 32; ++i) vi[i] = vr[i] = 0.0f;
+  const int g = gi > gr ? gi : gr;
+  for (int r = threadIdx.x; r < g; r += blockDim.x) {
+    if (r < gi) {
+#pragma unroll
+      for (int i = 0; i < 32; ++i) vi[i] += part_icp[r * 32 + i];
+    }
+    if (r < gr) {
+#pragma unroll
+      for (int i = 0; i < 32; ++i) vr[i] += part_rgb[r * 32 + i];
+    }
+  }
+  if (gi) {
+    const float t = block_sum32(vi, lds);
+    if (threadIdx.x < 32) sums[0][threadIdx.x] = t;
+  }
+  if (gr) {
+    const float t = block_sum32(vr, lds);
+    if (threadIdx.x < 32) sums[1][threadIdx.x] = t;
+  }
+  if (write && st_out != st_in) {
+    // the fields this update leaves alone carry over
+    const int nw = (int)(sizeof(TrackState) / 4);
+    for (int w = threadIdx.x; w < nw; w += blockDim.x) ((int*)st_out)[w] = ((const int*)st_in)[w];
+  }
+  __syncthreads();
+  if (write && threadIdx.x < 29) {
+    if (gi) st_out->sums_icp[threadIdx.x] = sums[0][threadIdx.x];
+    if (gr) st_out->sums_rgb[threadIdx.x] = sums[1][threadIdx.x];
+  }
+  if (threadIdx.x != 0) return;
+  if (est_out) {  // the pose where no step is taken
+#pragma unroll
+    for (int i = 0; i < 16; ++i) est_out[i] = s_est[i];
+  }
+  gn_step_lane0(sums, s_est, s_good, s_last_err, write ? st_out : nullptr, est_out, gi, gr, mode, first_of_level,
+                do_step, record, wrh, wrb, wih, wib);
+}
+__device__ inline void update_at(TrackState* st, const float* part_icp, int gi, const float* part_rgb, int gr,
+                                 int mode, int first_of_level, int do_step, int record, float wrh, float wrb, float wih,
+                                 float wib) {
+  gn_update(st, st, true, part_icp, gi, part_rgb, gr, mode, first_of_level, do_step, record, wrh, wrb, wih, wib,
+            nullptr);
 }
 
 __device__ inline void track_init_at(TrackState* st, const double* T) {
