@@ -244,6 +244,11 @@ int yas_server_set_max_batch(yas_server* sv, int max_batch);
 /* Run the Gauss-Newton iterations on sub-ranges of `slots` batch slots in
  * turn (0, the default: the whole batch at once).  Results are unchanged. */
 int yas_server_set_gn_group(yas_server* sv, int slots);
+/* Capture now the batch graphs of every batch size up to the batch cap
+ * (one per power of two), which are otherwise captured when a batch of that
+ * size is first launched, so that no batch waits for a capture.  Call it
+ * after the last yas_server_set_gn_group; it returns once they are built. */
+int yas_server_prepare(yas_server* sv);
 /* Admit a stream (its volume, ray caster, tracker and pose chain are
  * allocated now); YAS_E_CAPACITY when its buffers would exceed the budget or
  * all stream slots are taken. Synchronous. */

@@ -44,11 +44,14 @@ def run_single(p, K, frames):
     return poses, pl.export_map_sorted(rf.VOXEL_F32_F32)
 
 
-def run_server(streams, n, interleave=True):
+def run_server(streams, n, interleave=True, prepare=False):
     """streams: list of (p, K, frames).  Submits frame t of every stream
-    before frame t+1 of any, pumping as it goes."""
+    before frame t+1 of any, pumping as it goes.  prepare: capture the batch
+    graphs before the first submit instead of at each size's first batch."""
     sv = rf.Server(320, 240, 8, tracker_params())
     ids = [sv.add_stream(p.width, p.height, K, map_params()) for p, K, _ in streams]
+    if prepare:
+        sv.prepare()
     torch.cuda.synchronize()
     results = {i: [] for i in ids}
     for t in range(n):
@@ -72,7 +75,7 @@ def test_stream_is_identical_alone_in_a_mix_and_in_the_pipeline():
     c = stream_frames(320, 240, 1.2, n)
     single_poses, single_map = run_single(*a)
     _, _, alone, alone_maps = run_server([a], n)
-    sv, ids, mix, mix_maps = run_server([b, a, c], n)
+    sv, ids, mix, mix_maps = run_server([b, a, c], n, prepare=True)
     ra_alone = alone[0]
     ra_mix = mix[ids[1]]
     assert len(ra_alone) == len(ra_mix) == n

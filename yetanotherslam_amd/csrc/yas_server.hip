@@ -581,6 +581,18 @@ int yas_server_set_gn_group(yas_server* sv, int slots) {
   return YAS_OK;
 }
 
+int yas_server_prepare(yas_server* sv) {
+  YAS_REQUIRE(sv, "null server");
+  YAS_HIP(hipSetDevice(sv->device));
+  for (int n = 1;; n <<= 1) {
+    hipGraphExec_t ex = nullptr;
+    int rc = graph_for(sv, n, &ex);
+    if (rc) return rc;
+    if (n >= sv->max_batch) break;
+  }
+  return YAS_OK;
+}
+
 int yas_server_stream_pipeline(yas_server* sv, int stream_id, yas_pipeline** pipe) {
   YAS_REQUIRE(sv && pipe && stream_id >= 0 && stream_id < sv->max_streams && sv->streams[stream_id].pipe,
               "yas_server_stream_pipeline: no such stream");

@@ -260,6 +260,10 @@ class Server:
     def set_max_batch(self, n):
         call("yas_server_set_max_batch", self.h, n)
 
+    def prepare(self):
+        """Capture every batch size's graph now instead of at its first batch."""
+        call("yas_server_prepare", self.h)
+
     def remove_stream(self, sid):
         call("yas_server_remove_stream", self.h, sid)
 
