@@ -162,12 +162,12 @@ __global__ void __launch_bounds__(256) k_s_frame_level_nb(const StreamDev* strea
   if (x >= a.w || y >= a.h) return;
   const size_t k = (size_t)y * a.w + x;
   const size_t p16 = (size_t)a.w * 16;
-  if (!sl.first) ((IcpPoint*)B.ref_p)[k] = icp_point_px(B.ref_v, p16, a.w, a.h, x, y);
+  if (!sl.first) store_icp_point((float4*)B.ref_p, a.w * a.h, (int)k, icp_point_px(B.ref_v, p16, a.w, a.h, x, y));
   float gxv, gyv;
   gradient_px(B.cur_int, (size_t)a.w * 4, a.w, a.h, x, y, gxv, gyv);
   ((float*)B.gx)[k] = gxv;
   ((float*)B.gy)[k] = gyv;
-  ((IcpPoint*)B.cur_p)[k] = icp_point_px(B.cur_v, p16, a.w, a.h, x, y);
+  store_icp_point((float4*)B.cur_p, a.w * a.h, (int)k, icp_point_px(B.cur_v, p16, a.w, a.h, x, y));
 }
 // After tracking: the current frame's intensity and inverse depth become the
 // reference of the next frame (copies; the single pipeline swaps pointers).

@@ -77,7 +77,7 @@ __global__ void __launch_bounds__(256) k_normal(const float4* v, size_t sp, floa
   normal_at(v, sp, n, dp, W, H, blockIdx.x, blockIdx.y);
 }
 
-__global__ void __launch_bounds__(256) k_icp_point(const float4* v, IcpPoint* o, int W, int H) {
+__global__ void __launch_bounds__(256) k_icp_point(const float4* v, float4* o, int W, int H) {
   icp_point_at(v, (size_t)W * 16, o, W, H, blockIdx.x, blockIdx.y);
 }
 
@@ -134,8 +134,8 @@ struct FrameLevel {
   float4* ref_v;
   float* gx;
   float* gy;
-  IcpPoint* cur_p;
-  IcpPoint* ref_p;
+  float4* cur_p;  // ICP point maps
+  float4* ref_p;
 };
 __global__ void __launch_bounds__(256) k_frame_level(FrameLevel a) {
   const int x = blockIdx.x * 64 + (threadIdx.x & 63), y = blockIdx.y * 4 + (threadIdx.x >> 6);
@@ -165,8 +165,8 @@ __global__ void __launch_bounds__(256) k_frame_level_nb(FrameLevel a) {
   gradient_px(a.cur_int, (size_t)a.w * 4, a.w, a.h, x, y, gxv, gyv);
   a.gx[k] = gxv;
   a.gy[k] = gyv;
-  a.cur_p[k] = icp_point_px(a.cur_v, (size_t)a.w * 16, a.w, a.h, x, y);
-  a.ref_p[k] = icp_point_px(a.ref_v, (size_t)a.w * 16, a.w, a.h, x, y);
+  store_icp_point(a.cur_p, a.w * a.h, (int)k, icp_point_px(a.cur_v, (size_t)a.w * 16, a.w, a.h, x, y));
+  store_icp_point(a.ref_p, a.w * a.h, (int)k, icp_point_px(a.ref_v, (size_t)a.w * 16, a.w, a.h, x, y));
 }
 __global__ void k_track_init(TrackState* st, Mat44 T) { track_init_at(st, T.m); }
 __global__ void k_track_init_identity(TrackState* st) { track_init_identity_at(st); }
@@ -184,7 +184,7 @@ struct yas_tracker {
   float *cur_int[kMaxLevels], *ref_int[kMaxLevels], *gx[kMaxLevels], *gy[kMaxLevels];
   float *cur_inv[kMaxLevels], *ref_inv[kMaxLevels];
   float4 *cur_v[kMaxLevels], *ref_v[kMaxLevels];
-  IcpPoint *cur_p[kMaxLevels], *ref_p[kMaxLevels];  // what the ICP term reads
+  float4 *cur_p[kMaxLevels], *ref_p[kMaxLevels];  // ICP point maps (what the ICP term reads)
   float4 *cur_n[kMaxLevels], *ref_n[kMaxLevels];    // normal maps, formed on request (yas_tracker_level)
   float4* resid[kMaxLevels];
   float* part_icp;  // [kRedGroups][32]
@@ -216,7 +216,7 @@ static int image_pyramid(yas_tracker* t, const float* g, size_t pitch, float** p
   return YAS_OK;
 }
 
-static int depth_pyramid(yas_tracker* t, const float* d, size_t pitch, float** inv, float4** v, IcpPoint** pt) {
+static int depth_pyramid(yas_tracker* t, const float* d, size_t pitch, float** inv, float4** v, float4** pt) {
   hipStream_t s = t->ctx->stream;
   for (int l = 0; l < t->P.levels; ++l) {
     const Level& a = t->lv[l];
@@ -460,7 +460,7 @@ int yas_tracker_create(yas_ctx* ctx, int W, int H, const double K[4], const yas_
       alloc((void**)p, n * sizeof(float));
     for (float4** p : {&t->cur_v[l], &t->cur_n[l], &t->ref_v[l], &t->ref_n[l], &t->resid[l]})
       alloc((void**)p, n * sizeof(float4));
-    for (IcpPoint** p : {&t->cur_p[l], &t->ref_p[l]}) alloc((void**)p, n * sizeof(IcpPoint));
+    for (float4** p : {&t->cur_p[l], &t->ref_p[l]}) alloc((void**)p, n * sizeof(IcpPoint));
   }
   alloc((void**)&t->part_icp, sizeof(float) * 32 * kRedGroups);
   alloc((void**)&t->part_icp2, sizeof(float) * 32 * kRedGroups);
@@ -481,7 +481,7 @@ int yas_tracker_create(yas_ctx* ctx, int W, int H, const double K[4], const yas_
       YAS_HIP(hipMemsetAsync(p, 0, n * sizeof(float), ctx->stream));
     for (float4* p : {t->cur_v[l], t->cur_n[l], t->ref_v[l], t->ref_n[l], t->resid[l]})
       YAS_HIP(hipMemsetAsync(p, 0, n * sizeof(float4), ctx->stream));
-    for (IcpPoint* p : {t->cur_p[l], t->ref_p[l]}) YAS_HIP(hipMemsetAsync(p, 0, n * sizeof(IcpPoint), ctx->stream));
+    for (float4* p : {t->cur_p[l], t->ref_p[l]}) YAS_HIP(hipMemsetAsync(p, 0, n * sizeof(IcpPoint), ctx->stream));
   }
   YAS_HIP(hipMemsetAsync(t->st, 0, sizeof(TrackState), ctx->stream));
   YAS_HIP(hipMemsetAsync(t->st2, 0, sizeof(TrackState), ctx->stream));
@@ -501,7 +501,7 @@ int yas_tracker_destroy(yas_tracker* t) {
   for (int l = 0; l < kMaxLevels; ++l) {
     for (float* p : {t->cur_int[l], t->ref_int[l], t->gx[l], t->gy[l], t->cur_inv[l], t->ref_inv[l]}) hipFree(p);
     for (float4* p : {t->cur_v[l], t->cur_n[l], t->ref_v[l], t->ref_n[l], t->resid[l]}) hipFree(p);
-    for (IcpPoint* p : {t->cur_p[l], t->ref_p[l]}) hipFree(p);
+    for (float4* p : {t->cur_p[l], t->ref_p[l]}) hipFree(p);
   }
   hipFree(t->part_icp);
   hipFree(t->part_icp2);
@@ -609,7 +609,7 @@ int yas_tracker_level(yas_tracker* t, int which, int level, void** ptr, size_t* 
   void* four[5] = {t->cur_v[level], t->cur_n[level], t->ref_v[level], t->ref_n[level], t->resid[level]};
   YAS_REQUIRE(which >= 0 && which < 11, "bad buffer id");
   if (which == 7 || which == 9) {
-    // The passes read vertex and normal as one IcpPoint; the float4 normal
+    // The passes read vertex and normal as ICP points; the float4 normal
     // map is formed here from the vertex map, as the pyramid kernels would.
     YAS_HIP(hipSetDevice(t->ctx->device));
     hipLaunchKernelGGL(k_normal, grid64x4(a.w, a.h), dim3(256), 0, t->ctx->stream,

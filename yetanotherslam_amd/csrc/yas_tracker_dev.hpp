@@ -294,9 +294,23 @@ __device__ inline void normal_at(const float4* v, size_t sp, float4* n, size_t d
 // both, so a pixel without a normal stores a NaN x instead of the two w
 // flags: the reference side skips it, and on the current side the NaN
 // distance fails the 1 cm gate, as the w tests would.
-struct __attribute__((aligned(8))) IcpPoint {
+struct IcpPoint {
   float vx, vy, vz, nx, ny, nz;
 };
+// A level's ICP point map (n pixels) is two planes, so every access is one
+// aligned 16-byte and one aligned 8-byte access: (vx, vy, vz, nx) as n
+// float4, then (ny, nz) as n float2.
+__device__ inline IcpPoint load_icp_point(const float4* m, int n, int k) {
+  const float4 a = m[k];
+  const float2 b = reinterpret_cast<const float2*>(m + n)[k];
+  IcpPoint p;
+  p.vx = a.x; p.vy = a.y; p.vz = a.z; p.nx = a.w; p.ny = b.x; p.nz = b.y;
+  return p;
+}
+__device__ inline void store_icp_point(float4* m, int n, int k, const IcpPoint& p) {
+  m[k] = make_float4(p.vx, p.vy, p.vz, p.nx);
+  reinterpret_cast<float2*>(m + n)[k] = make_float2(p.ny, p.nz);
+}
 __device__ inline IcpPoint icp_point_px(const float4* v, size_t sp, int W, int H, int x, int y) {
   const float4 a = ((const float4*)((const char*)v + (size_t)y * sp))[x];
   const float4 n = normal_px(v, sp, W, H, x, y);
@@ -306,10 +320,10 @@ __device__ inline IcpPoint icp_point_px(const float4* v, size_t sp, int W, int H
   p.nx = n.x; p.ny = n.y; p.nz = n.z;
   return p;
 }
-__device__ inline void icp_point_at(const float4* v, size_t sp, IcpPoint* o, int W, int H, int bx, int by) {
+__device__ inline void icp_point_at(const float4* v, size_t sp, float4* o, int W, int H, int bx, int by) {
   int x = bx * 64 + (threadIdx.x & 63), y = by * 4 + (threadIdx.x >> 6);
   if (x >= W || y >= H) return;
-  o[(size_t)y * W + x] = icp_point_px(v, sp, W, H, x, y);
+  store_icp_point(o, W * H, y * W + x, icp_point_px(v, sp, W, H, x, y));
 }
 
 // ImageProc.cu:262-275.
@@ -323,9 +337,9 @@ __device__ inline void vmap_to_depth_at(const float4* v, size_t sp, float* d, si
 // ------------------------------------------------------------------ per-pixel terms
 struct TrackBufs {
   const float4* ref_v;
-  const IcpPoint* ref_p;
+  const float4* ref_p;  // ICP point maps (two planes, load_icp_point)
   const float4* cur_v;
-  const IcpPoint* cur_p;
+  const float4* cur_p;
   const float* ref_inv;
   const float* ref_int;
   const float* cur_int;
@@ -347,7 +361,7 @@ __device__ inline void icp_pixel(const Level& L, const TrackBufs& B, const PoseF
   const int u = floor_sat(L.fx * vx * invz + L.cx + 0.5f);
   const int v = floor_sat(L.fy * vy * invz + L.cy + 0.5f);
   if (u < 0 || v < 0 || u >= L.w || v >= L.h) return;
-  const IcpPoint q = B.cur_p[v * L.w + u];
+  const IcpPoint q = load_icp_point(B.cur_p, L.w * L.h, v * L.w + u);
   float nx, ny, nz;
   rot3(T, r.nx, r.ny, r.nz, nx, ny, nz);
   const float dx = vx - q.vx, dy = vy - q.vy, dz = vz - q.vz;
@@ -442,9 +456,9 @@ struct RefPixel {
   float iz, inten;  // inverse depth and intensity (photometric)
 };
 template <bool ICP, bool RGB>
-__device__ inline RefPixel load_ref_pixel(const TrackBufs& B, int k) {
+__device__ inline RefPixel load_ref_pixel(const TrackBufs& B, int n, int k) {
   RefPixel r;
-  if (ICP) r.p = B.ref_p[k];
+  if (ICP) r.p = load_icp_point(B.ref_p, n, k);
   if (RGB) {
     r.iz = B.ref_inv[k];
     r.inten = B.ref_int[k];
@@ -488,7 +502,7 @@ __device__ inline void pass1_at(const Level& L, const TrackBufs& B, const TrackS
   const int stride = nb * blockDim.x;
   int k = bx * blockDim.x + threadIdx.x;
   RefPixel nxt = {};
-  if (PREFETCH && k < N) nxt = load_ref_pixel<ICP, RGB>(B, k);  // in flight while the pose loads
+  if (PREFETCH && k < N) nxt = load_ref_pixel<ICP, RGB>(B, N, k);  // in flight while the pose loads
   const double* est;
   if (STEP) {
     gn_update(sa->st_in, sa->st_out, bx == 0, sa->part_icp, sa->gi, sa->part_rgb, sa->gr, sa->mode,
@@ -516,9 +530,9 @@ __device__ inline void pass1_at(const Level& L, const TrackBufs& B, const TrackS
     RefPixel cur;
     if (PREFETCH) {
       cur = nxt;
-      if (k + stride < N) nxt = load_ref_pixel<ICP, RGB>(B, k + stride);
+      if (k + stride < N) nxt = load_ref_pixel<ICP, RGB>(B, N, k + stride);
     } else {
-      cur = load_ref_pixel<ICP, RGB>(B, k);
+      cur = load_ref_pixel<ICP, RGB>(B, N, k);
     }
     if (ICP) icp_pixel(L, B, T, angle_th, dist_th, cur.p, acc);
     if (RGB) rgb_pixel(L, B, wu, k, cur.iz, cur.inten, n, mean, m2);
