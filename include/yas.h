@@ -365,9 +365,9 @@ int yas_tracker_step_sums(yas_tracker* trk, int level, const double T[16], int k
  * which = 0 current intensity, 1 reference intensity, 2 gx, 3 gy,
  * 4 current inverse depth, 5 reference inverse depth (float);
  * 6 current vmap, 7 current nmap, 8 reference vmap, 9 reference nmap,
- * 10 photometric residual rows (r, gx, gy, valid) (float4).  The normal
- * maps are formed from the vertex maps by this call, which then waits for
- * the context's stream. */
+ * 10 photometric residual rows (r, gx, gy) (float3; r is NaN where the
+ * pixel has no residual).  The normal maps are formed from the vertex maps
+ * by this call, which then waits for the context's stream. */
 int yas_tracker_level(yas_tracker* trk, int which, int level, void** ptr, size_t* pitch, int* width,
                       int* height);
 

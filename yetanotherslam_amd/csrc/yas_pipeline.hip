@@ -136,6 +136,10 @@ int yas_pipeline_fill(yas_pipeline* p, StreamDev& d) {
   d.gate = p->gate;
   int rc = yas_raycaster_fill(p->rc, d.m, d);
   if (!rc) rc = yas_tracker_fill(p->trk, d);
+  // the batched tracker reads the level-0 model vertices straight from the
+  // ray cast instead of a copy (the single pipeline swaps its buffers, so it
+  // keeps the copy)
+  d.tb[0].ref_v = d.vmap;
   return rc;
 }
 yas_map* yas_pipeline_map_of(yas_pipeline* p) { return p->map; }

@@ -127,9 +127,9 @@ __global__ void __launch_bounds__(256) k_s_march(const StreamDev* streams, const
 // Tracker inputs, two launches per pyramid level (the per-pixel functions
 // of the separate image kernels, so the same values): the pointwise
 // products of level l from level l - 1 (or the frame at level 0) -- model
-// vertices, intensity, inverse depth, vertices -- then the neighbourhood
-// products of level l -- model normals, gradients, normals.  A stream's
-// first frame has no model.
+// vertices (above level 0), intensity, inverse depth, vertices -- then the
+// neighbourhood products of level l -- gradients and the model's and the
+// frame's ICP points.  A stream's first frame has no model.
 __global__ void __launch_bounds__(256) k_s_frame_level(const StreamDev* streams, const Table* tab, int l) {
   SLOT_OR_RETURN;
   if (l >= S.levels) return;
@@ -139,10 +139,9 @@ __global__ void __launch_bounds__(256) k_s_frame_level(const StreamDev* streams,
   if (x >= a.w || y >= a.h) return;
   const size_t k = (size_t)y * a.w + x;
   float in, iv;
-  if (l == 0) {
+  if (l == 0) {  // level 0's model vertices are the ray cast's own map
     in = ld(S.gray, (size_t)S.W * 4, x, y);
     iv = invdepth_of(ld(S.depth, (size_t)S.W * 4, x, y));
-    if (!sl.first) ((float4*)B.ref_v)[k] = ((const float4*)((const char*)S.vmap + (size_t)y * S.W * 16))[x];
   } else {
     const Level& p = S.lv[l - 1];
     in = pyrdown_gauss_px(S.tb[l - 1].cur_int, (size_t)p.w * 4, p.w, p.h, x, y);
@@ -190,8 +189,11 @@ __global__ void k_s_track_init(const StreamDev* streams, const Table* tab, int s
   track_init_identity_at(S.st);
 }
 
+// At least six wavefronts per SIMD: at the batch's full size this pass is
+// bound by HBM bandwidth, and the register cap keeps enough reads in flight.
 template <bool ICP, bool RGB>
-__global__ void __launch_bounds__(256) k_s_pass1(const StreamDev* streams, const Table* tab, int l, int slot0) {
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6)))
+k_s_pass1(const StreamDev* streams, const Table* tab, int l, int slot0) {
   SLOT_FROM_OR_RETURN(slot0);
   if (sl.first || (int)blockIdx.x >= S.groups[l]) return;
   pass1_at<ICP, RGB, true>(S.lv[l], S.tb[l], S.st, S.part_icp, S.part_res, S.angle_th, S.dist_th, blockIdx.x,

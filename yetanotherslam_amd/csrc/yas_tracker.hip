@@ -186,7 +186,7 @@ struct yas_tracker {
   float4 *cur_v[kMaxLevels], *ref_v[kMaxLevels];
   float4 *cur_p[kMaxLevels], *ref_p[kMaxLevels];  // ICP point maps (what the ICP term reads)
   float4 *cur_n[kMaxLevels], *ref_n[kMaxLevels];    // normal maps, formed on request (yas_tracker_level)
-  float4* resid[kMaxLevels];
+  float3* resid[kMaxLevels];
   float* part_icp;  // [kRedGroups][32]
   float* part_icp2;  // the other of the pair a solve alternates between
   float* part_res;  // [kRedGroups][4]
@@ -458,8 +458,8 @@ int yas_tracker_create(yas_ctx* ctx, int W, int H, const double K[4], const yas_
     size_t n = (size_t)a.w * a.h;
     for (float** p : {&t->cur_int[l], &t->ref_int[l], &t->gx[l], &t->gy[l], &t->cur_inv[l], &t->ref_inv[l]})
       alloc((void**)p, n * sizeof(float));
-    for (float4** p : {&t->cur_v[l], &t->cur_n[l], &t->ref_v[l], &t->ref_n[l], &t->resid[l]})
-      alloc((void**)p, n * sizeof(float4));
+    for (float4** p : {&t->cur_v[l], &t->cur_n[l], &t->ref_v[l], &t->ref_n[l]}) alloc((void**)p, n * sizeof(float4));
+    alloc((void**)&t->resid[l], n * sizeof(float3));
     for (float4** p : {&t->cur_p[l], &t->ref_p[l]}) alloc((void**)p, n * sizeof(IcpPoint));
   }
   alloc((void**)&t->part_icp, sizeof(float) * 32 * kRedGroups);
@@ -479,8 +479,9 @@ int yas_tracker_create(yas_ctx* ctx, int W, int H, const double K[4], const yas_
     size_t n = (size_t)t->lv[l].w * t->lv[l].h;
     for (float* p : {t->cur_int[l], t->ref_int[l], t->gx[l], t->gy[l], t->cur_inv[l], t->ref_inv[l]})
       YAS_HIP(hipMemsetAsync(p, 0, n * sizeof(float), ctx->stream));
-    for (float4* p : {t->cur_v[l], t->cur_n[l], t->ref_v[l], t->ref_n[l], t->resid[l]})
+    for (float4* p : {t->cur_v[l], t->cur_n[l], t->ref_v[l], t->ref_n[l]})
       YAS_HIP(hipMemsetAsync(p, 0, n * sizeof(float4), ctx->stream));
+    YAS_HIP(hipMemsetAsync(t->resid[l], 0, n * sizeof(float3), ctx->stream));
     for (float4* p : {t->cur_p[l], t->ref_p[l]}) YAS_HIP(hipMemsetAsync(p, 0, n * sizeof(IcpPoint), ctx->stream));
   }
   YAS_HIP(hipMemsetAsync(t->st, 0, sizeof(TrackState), ctx->stream));
@@ -500,7 +501,8 @@ int yas_tracker_destroy(yas_tracker* t) {
   hipStreamSynchronize(t->ctx->stream);
   for (int l = 0; l < kMaxLevels; ++l) {
     for (float* p : {t->cur_int[l], t->ref_int[l], t->gx[l], t->gy[l], t->cur_inv[l], t->ref_inv[l]}) hipFree(p);
-    for (float4* p : {t->cur_v[l], t->cur_n[l], t->ref_v[l], t->ref_n[l], t->resid[l]}) hipFree(p);
+    for (float4* p : {t->cur_v[l], t->cur_n[l], t->ref_v[l], t->ref_n[l]}) hipFree(p);
+    hipFree(t->resid[l]);
     for (float4* p : {t->cur_p[l], t->ref_p[l]}) hipFree(p);
   }
   hipFree(t->part_icp);
@@ -606,7 +608,7 @@ int yas_tracker_level(yas_tracker* t, int which, int level, void** ptr, size_t* 
   YAS_REQUIRE(level >= 0 && level < t->P.levels, "bad level");
   const Level& a = t->lv[level];
   void* one[6] = {t->cur_int[level], t->ref_int[level], t->gx[level], t->gy[level], t->cur_inv[level], t->ref_inv[level]};
-  void* four[5] = {t->cur_v[level], t->cur_n[level], t->ref_v[level], t->ref_n[level], t->resid[level]};
+  void* vec[5] = {t->cur_v[level], t->cur_n[level], t->ref_v[level], t->ref_n[level], t->resid[level]};
   YAS_REQUIRE(which >= 0 && which < 11, "bad buffer id");
   if (which == 7 || which == 9) {
     // The passes read vertex and normal as ICP points; the float4 normal
@@ -618,8 +620,8 @@ int yas_tracker_level(yas_tracker* t, int which, int level, void** ptr, size_t* 
     YAS_LAUNCH_CHECK();
     YAS_HIP(hipStreamSynchronize(t->ctx->stream));
   }
-  *ptr = which < 6 ? one[which] : four[which - 6];
-  if (pitch) *pitch = (size_t)a.w * (which < 6 ? 4 : 16);
+  *ptr = which < 6 ? one[which] : vec[which - 6];
+  if (pitch) *pitch = (size_t)a.w * (which < 6 ? 4 : which < 10 ? 16 : 12);
   if (w) *w = a.w;
   if (h) *h = a.h;
   return YAS_OK;

@@ -345,7 +345,7 @@ struct TrackBufs {
   const float* cur_int;
   const float* gx;
   const float* gy;
-  float4* resid;
+  float3* resid;  // photometric rows (r, gx, gy); r is NaN where a pixel has none
 };
 
 // RGBDTracking.cu:316-372: projective association of reference pixel k into
@@ -439,7 +439,7 @@ __device__ inline void rgb_pixel(const Level& L, const TrackBufs& B, const float
       ok = (gx * gx + gy * gy) > 0.0f && isfinite(r);
     }
   }
-  B.resid[k] = make_float4(r, gx, gy, ok ? 0.5f : -0.5f);
+  B.resid[k] = ok ? make_float3(r, gx, gy) : make_float3(__builtin_nanf(""), 0.0f, 0.0f);
   if (ok) {
     n += 1.0f;
     float d = r - mean;
@@ -564,7 +564,7 @@ __device__ inline void pass2_at(const Level& L, const TrackBufs& B, TrackState* 
   const int N = L.w * L.h;
   const int stride = nb * blockDim.x;
   int k = bx * blockDim.x + threadIdx.x;
-  float4 e_next = make_float4(0.f, 0.f, 0.f, 0.f);
+  float3 e_next = make_float3(0.f, 0.f, 0.f);
   float iz_next = 0.0f;
   if (PREFETCH && k < N) {
     e_next = B.resid[k];
@@ -590,7 +590,7 @@ __device__ inline void pass2_at(const Level& L, const TrackBufs& B, TrackState* 
 #pragma unroll
   for (int i = 0; i < 32; ++i) acc[i] = 0.0f;
   for (; k < N; k += stride) {
-    float4 e;
+    float3 e;
     float iz;
     if (PREFETCH) {
       e = e_next;
@@ -603,7 +603,7 @@ __device__ inline void pass2_at(const Level& L, const TrackBufs& B, TrackState* 
       e = B.resid[k];
       iz = B.ref_inv[k];
     }
-    if (!(e.w > 0.0f)) continue;
+    if (!(e.x == e.x)) continue;  // no residual at this pixel
     const float4 p = warp_ref(L, wu, k, iz);
     const float zinv = 1.0f / p.z;
     float wt = 1.0f;

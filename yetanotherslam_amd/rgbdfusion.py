@@ -578,7 +578,7 @@ class RGBDTracking:
              ctypes.byref(h))
         if which < 6:
             return download(p.value, (h.value, w.value), np.float32)
-        return download(p.value, (h.value, w.value, 4), np.float32)
+        return download(p.value, (h.value, w.value, 4 if which < 10 else 3), np.float32)
 
 
 class DensePipeline:
