@@ -121,7 +121,7 @@ __global__ void __launch_bounds__(256) k_s_march(const StreamDev* streams, const
   const int T = ((S.W + 15) >> 4) * ((S.H + 15) >> 4);
   if ((int)blockIdx.x >= T) return;
   march_at<VT>(S.m, S.pose, S.K, S.W, S.H, S.cols, (const float*)S.zmin, (const float*)S.zmax, S.vmap, S.vsinv,
-               S.step_scale, blockIdx.x);
+               S.step_scale, rotate_band(blockIdx.x, T, blockIdx.z));
 }
 
 // Tracker inputs, two launches per pyramid level (the per-pixel functions
@@ -331,7 +331,8 @@ static int launch_batch_kernels(yas_server* sv, hipStream_t s, int nz) {
   hipLaunchKernelGGL(k_s_visible, dim3(4096, 1, nz), dim3(256), 0, s, S, T, 0);
   hipLaunchKernelGGL(k_s_zrange_init, dim3(((W / 8) * (H / 8) + 255) / 256, 1, nz), dim3(256), 0, s, S, T);
   hipLaunchKernelGGL(k_s_zrange, dim3(2048, 1, nz), dim3(256), 0, s, S, T);
-  const dim3 mg(((W + 15) / 16) * ((H + 15) / 16), 1, nz);
+  // a multiple of 8 workgroups per image, so workgroup b of every image runs on XCD b % 8
+  const dim3 mg((((W + 15) / 16) * ((H + 15) / 16) + 7) & ~7, 1, nz);
   switch (sv->voxel_format) {
     case YAS_VOXEL_I16_U8: hipLaunchKernelGGL(k_s_march<VoxI16>, mg, dim3(256), 0, s, S, T); break;
     case YAS_VOXEL_F16_U16: hipLaunchKernelGGL(k_s_march<VoxF16>, mg, dim3(256), 0, s, S, T); break;
