@@ -374,15 +374,6 @@ __device__ inline int xcd_tile(int b, int T) {
   const int q = T >> 3, r = T & 7;
   return j * q + (j < r ? j : r) + k;
 }
-// For one image of a batch whose launches give every image the same number
-// of workgroups (a multiple of 8, so workgroup b of each image runs on XCD
-// b % 8): the launch order permuted within each full group of 8 so that
-// XCD j takes band (j + image) % 8 of xcd_tile.  Across the images of a
-// batch every XCD then gets every band alike, instead of the band with the
-// longest rays landing on the same XCD for all of them.
-__device__ inline int rotate_band(int b, int T, int image) {
-  return b < (T & ~7) ? ((b & ~7) | ((b + image) & 7)) : b;
-}
 
 // ------------------------------------------------------------------ wave64
 __device__ inline int lane_id() { return __lane_id(); }

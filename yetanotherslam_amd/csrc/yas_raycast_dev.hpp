@@ -213,14 +213,14 @@ __device__ inline bool march_ray(const MapView& m, float& rx, float& ry, float& 
 
 // RayTraceEngine.cu:331-410.  Departure: pixels the reference leaves
 // unwritten get w = -1 (x = NaN, as the reference writes first).
-// Workgroup b of the image's (W/16)*(H/16) tiles.
+// Tile `tile` (raster order) of the image's (W/16)*(H/16) tiles of 16x16.
 template <class VT>
-__device__ inline void march_at(const MapView& m, const PosePair* pose, const Intr& K, int W, int H, int cols,
-                                const float* zmin, const float* zmax, float4* vmap, float vsinv, float step_scale, int b) {
+__device__ inline void march_tile_at(const MapView& m, const PosePair* pose, const Intr& K, int W, int H, int cols,
+                                     const float* zmin, const float* zmax, float4* vmap, float vsinv,
+                                     float step_scale, int tile) {
   // A 16x16 tile per workgroup; each wavefront takes one 8x8 z-range cell,
   // so its rays share a depth range and run for similar numbers of steps.
   const int tiles_x = (W + 15) >> 4;
-  const int tile = xcd_tile(b, tiles_x * ((H + 15) >> 4));
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int x = (tile % tiles_x) * 16 + (wave & 1) * 8 + (lane & 7);
   const int y = (tile / tiles_x) * 16 + (wave >> 1) * 8 + (lane >> 3);
@@ -261,6 +261,14 @@ __device__ inline void march_at(const MapView& m, const PosePair* pose, const In
     }
   }
   vmap[(size_t)y * W + x] = out;
+}
+
+// Workgroup b in launch order: the tiles dealt to the XCDs in bands (xcd_tile).
+template <class VT>
+__device__ inline void march_at(const MapView& m, const PosePair* pose, const Intr& K, int W, int H, int cols,
+                                const float* zmin, const float* zmax, float4* vmap, float vsinv, float step_scale, int b) {
+  march_tile_at<VT>(m, pose, K, W, H, cols, zmin, zmax, vmap, vsinv, step_scale,
+                    xcd_tile(b, ((W + 15) >> 4) * ((H + 15) >> 4)));
 }
 
 }  // namespace yas

@@ -100,7 +100,20 @@ __global__ void __launch_bounds__(256) k_s_visible(const StreamDev* streams, con
   visible_scan(S.m, S.pose->Tinv, S.K, S.W, S.H, blockIdx.x * blockDim.x, S.vis_groups * blockDim.x);
 }
 
-__global__ void __launch_bounds__(256) k_s_zrange_init(const StreamDev* streams, const Table* tab) {
+// The batch's ray-march dispatch order: the z-range cells give each 16x16
+// tile of every image a class by the largest depth range of its 2x2 cells,
+// and the march takes the classes in turn, longest ranges first, so that
+// the longest rays do not start last.  The classes are lists of
+// (slot << 16 | tile); counts[c] is the length of class c.
+constexpr int kMarchClasses = 4;
+struct MarchOrder {
+  int* tiles;   // [kMarchClasses][cap]
+  int* counts;  // [kMarchClasses]
+  int cap;      // kMaxSlots * tiles of the largest image
+};
+
+__global__ void __launch_bounds__(256) k_s_zrange_init(const StreamDev* streams, const Table* tab, MarchOrder mo) {
+  if (blockIdx.x == 0 && blockIdx.z == 0 && threadIdx.x < kMarchClasses) mo.counts[threadIdx.x] = 0;
   SLOT_OR_RETURN;
   if (sl.first) return;
   const int n = S.cols * S.rows, g = (n + 255) / 256;
@@ -114,14 +127,62 @@ __global__ void __launch_bounds__(256) k_s_zrange(const StreamDev* streams, cons
   zrange_at<true>(S.m, S.pose, S.K, S.cols, S.rows, S.zmin, S.zmax, blockIdx.x, S.zr_groups);
 }
 
-template <class VT>
-__global__ void __launch_bounds__(256) k_s_march(const StreamDev* streams, const Table* tab) {
+__global__ void __launch_bounds__(256) k_s_march_order(const StreamDev* streams, const Table* tab, MarchOrder mo) {
   SLOT_OR_RETURN;
   if (sl.first) return;
-  const int T = ((S.W + 15) >> 4) * ((S.H + 15) >> 4);
-  if ((int)blockIdx.x >= T) return;
-  march_at<VT>(S.m, S.pose, S.K, S.W, S.H, S.cols, (const float*)S.zmin, (const float*)S.zmax, S.vmap, S.vsinv,
-               S.step_scale, rotate_band(blockIdx.x, T, blockIdx.z));
+  const int tx = (S.W + 15) >> 4, T = tx * ((S.H + 15) >> 4);
+  const int t = (int)blockIdx.x * 256 + (int)threadIdx.x;
+  if (t >= T) return;
+  const float* zmin = (const float*)S.zmin;
+  const float* zmax = (const float*)S.zmax;
+  const float eps = 1.1920928955078125e-07f;
+  float span = 0.0f;
+  for (int j = 0; j < 2; ++j)
+    for (int i = 0; i < 2; ++i) {
+      const int x = (t % tx) * 2 + i, y = (t / tx) * 2 + j;
+      if (x < S.cols && y < S.rows) {
+        const float zn = zmin[y * S.cols + x], zf = zmax[y * S.cols + x];
+        if (zn >= eps && zf > zn) span = fmaxf(span, zf - zn);
+      }
+    }
+  const float v = span * S.vsinv;  // voxels
+  const int c = v >= 96.0f ? 0 : v >= 32.0f ? 1 : v >= 8.0f ? 2 : 3;
+  // one atomic per class per wavefront
+  const unsigned long long below = lane_mask_lt();
+  int pos = 0;
+  for (int k = 0; k < kMarchClasses; ++k) {
+    const unsigned long long mask = __ballot(c == k);
+    if (mask == 0) continue;
+    const int leader = __ffsll((long long)mask) - 1;
+    int base = 0;
+    if (lane_id() == leader) base = atomicAdd(&mo.counts[k], __popcll(mask));
+    base = __shfl(base, leader, 64);
+    if (c == k) pos = base + __popcll(mask & below);
+  }
+  mo.tiles[c * mo.cap + pos] = ((int)blockIdx.z << 16) | t;
+}
+
+template <class VT>
+__global__ void __launch_bounds__(256) k_s_march(const StreamDev* streams, const Table* tab, MarchOrder mo) {
+  int g = (int)(blockIdx.x + blockIdx.z * gridDim.x);
+  int item = -1;
+  for (int c = 0; c < kMarchClasses; ++c) {
+    const int n = mo.counts[c];
+    if (g < n) {
+      // within a class, XCD bands again: consecutive entries (neighbouring
+      // tiles, as the order kernel appends them) share an XCD's L2
+      item = mo.tiles[c * mo.cap + xcd_tile(g, n)];
+      break;
+    }
+    g -= n;
+  }
+  if (item < 0) return;
+  const int slot = item >> 16, tile = item & 0xffff;
+  const YAS_CONSTANT Table& ctab = *(const YAS_CONSTANT Table*)tab;
+  const YAS_CONSTANT Slot& sl = ctab.slot[slot];
+  const YAS_CONSTANT StreamDev& S = ((const YAS_CONSTANT StreamDev*)streams)[sl.stream];
+  march_tile_at<VT>(S.m, S.pose, S.K, S.W, S.H, S.cols, (const float*)S.zmin, (const float*)S.zmax, S.vmap,
+                    S.vsinv, S.step_scale, tile);
 }
 
 // Tracker inputs, two launches per pyramid level (the per-pixel functions
@@ -306,6 +367,8 @@ struct yas_server {
   StreamDev* dev_streams;
   Table* dev_table;
   double* dev_results;  // [kInFlight][kMaxSlots * kResultStride]
+  int* march_tiles;     // ray-march dispatch order, [kMarchClasses][kMaxSlots * tiles]
+  int* march_counts;    // [kMarchClasses]
   Batch batches[kInFlight];
   int next_batch;
   std::map<int, hipGraphExec_t> graphs;  // batch-size class -> graph
@@ -329,14 +392,16 @@ static int launch_batch_kernels(yas_server* sv, hipStream_t s, int nz) {
   // ray cast at the last pose
   hipLaunchKernelGGL(k_s_pose, dim3(1, 1, nz), dim3(64), 0, s, S, T, 1);
   hipLaunchKernelGGL(k_s_visible, dim3(4096, 1, nz), dim3(256), 0, s, S, T, 0);
-  hipLaunchKernelGGL(k_s_zrange_init, dim3(((W / 8) * (H / 8) + 255) / 256, 1, nz), dim3(256), 0, s, S, T);
+  const int tiles = ((W + 15) / 16) * ((H + 15) / 16);
+  const MarchOrder mo{sv->march_tiles, sv->march_counts, kMaxSlots * tiles};
+  hipLaunchKernelGGL(k_s_zrange_init, dim3(((W / 8) * (H / 8) + 255) / 256, 1, nz), dim3(256), 0, s, S, T, mo);
   hipLaunchKernelGGL(k_s_zrange, dim3(2048, 1, nz), dim3(256), 0, s, S, T);
-  // a multiple of 8 workgroups per image, so workgroup b of every image runs on XCD b % 8
-  const dim3 mg((((W + 15) / 16) * ((H + 15) / 16) + 7) & ~7, 1, nz);
+  hipLaunchKernelGGL(k_s_march_order, dim3((tiles + 255) / 256, 1, nz), dim3(256), 0, s, S, T, mo);
+  const dim3 mg(tiles, 1, nz);  // one workgroup per tile of every image, in the order's sequence
   switch (sv->voxel_format) {
-    case YAS_VOXEL_I16_U8: hipLaunchKernelGGL(k_s_march<VoxI16>, mg, dim3(256), 0, s, S, T); break;
-    case YAS_VOXEL_F16_U16: hipLaunchKernelGGL(k_s_march<VoxF16>, mg, dim3(256), 0, s, S, T); break;
-    default: hipLaunchKernelGGL(k_s_march<VoxF32>, mg, dim3(256), 0, s, S, T);
+    case YAS_VOXEL_I16_U8: hipLaunchKernelGGL(k_s_march<VoxI16>, mg, dim3(256), 0, s, S, T, mo); break;
+    case YAS_VOXEL_F16_U16: hipLaunchKernelGGL(k_s_march<VoxF16>, mg, dim3(256), 0, s, S, T, mo); break;
+    default: hipLaunchKernelGGL(k_s_march<VoxF32>, mg, dim3(256), 0, s, S, T, mo);
   }
   // tracker inputs: reference model pyramid, current intensity and depth pyramids
   for (int l = 0; l < L; ++l) hipLaunchKernelGGL(k_s_frame_level, level_grid(l), dim3(256), 0, s, S, T, l);
@@ -438,6 +503,8 @@ int yas_server_create(yas_ctx* ctx, int max_width, int max_height, int max_strea
                       int voxel_format, size_t hbm_budget_bytes, yas_server** out) {
   YAS_REQUIRE(ctx && tp && out, "yas_server_create: null argument");
   YAS_REQUIRE(max_width >= 8 && max_height >= 8 && max_streams >= 1, "yas_server_create: bad sizes");
+  YAS_REQUIRE((long long)((max_width + 15) / 16) * ((max_height + 15) / 16) < 65536,
+              "yas_server_create: images of at most 65535 16x16 tiles");
   YAS_REQUIRE(tp->levels >= 1 && tp->levels <= kTrackMaxLevels, "yas_server_create: levels must be 1..5");
   YAS_REQUIRE(voxel_format >= 0 && voxel_format <= 2, "yas_server_create: unknown voxel format");
   yas_server* sv = new yas_server();
@@ -464,6 +531,9 @@ int yas_server_create(yas_ctx* ctx, int max_width, int max_height, int max_strea
   hipError_t e = hipMalloc((void**)&sv->dev_streams, sizeof(StreamDev) * max_streams);
   if (e == hipSuccess) e = hipMalloc((void**)&sv->dev_table, sizeof(Table));
   if (e == hipSuccess) e = hipMalloc((void**)&sv->dev_results, sizeof(double) * kInFlight * kMaxSlots * kResultStride);
+  const size_t max_tiles = (size_t)((max_width + 15) / 16) * ((max_height + 15) / 16);
+  if (e == hipSuccess) e = hipMalloc((void**)&sv->march_tiles, sizeof(int) * kMarchClasses * kMaxSlots * max_tiles);
+  if (e == hipSuccess) e = hipMalloc((void**)&sv->march_counts, sizeof(int) * kMarchClasses);
   for (int k = 0; e == hipSuccess && k < kInFlight; ++k) {
     e = hipHostMalloc((void**)&sv->batches[k].staged, sizeof(Table), hipHostMallocDefault);
     if (e == hipSuccess)
@@ -495,6 +565,8 @@ int yas_server_destroy(yas_server* sv) {
   hipFree(sv->dev_streams);
   hipFree(sv->dev_table);
   hipFree(sv->dev_results);
+  hipFree(sv->march_tiles);
+  hipFree(sv->march_counts);
   if (sv->ctx) yas_ctx_destroy(sv->ctx);
   delete sv;
   return YAS_OK;
