@@ -628,100 +628,110 @@ __device__ inline void pass2_at(const Level& L, const TrackBufs& B, TrackState* 
 }
 
 // ------------------------------------------------------------------ GN update
-// Eigen 3.3 LDLT restated (diagonal pivoting, pseudo-inverse of D), solving
-// A x = y in place.  Fully unrolled on register arrays: the pivot row is a
-// run-time value, so swaps are predicated over the candidate rows, keeping
-// every index static; the arithmetic is the textbook sequence in order.
-template <class T>
-__device__ inline void swap_d(T& a, T& b) {
-  const T t = a;
-  a = b;
-  b = t;
-}
 // Swap by selects on both values, so the run-time pivot never becomes an
 // array index (which would move the array to scratch memory).
-__device__ inline void swap_if(bool p, double& a, double& b) {
+__host__ __device__ inline void swap_if(bool p, double& a, double& b) {
   const double x = a, z = b;
   a = p ? z : x;
   b = p ? x : z;
 }
-__device__ inline void ldlt_solve6(double (&A)[36], double (&y)[6]) {
+// Eigen 3.3's LDLT restated (Eigen/src/Cholesky/LDLT.h, unblocked: diagonal
+// pivoting, left-looking, pseudo-inverse of D), solving H x = b for the
+// symmetric 6x6 H, with its pivots found first.  Eigen's pivot k is the
+// largest |diagonal| among positions k..5 of the matrix as permuted so far
+// (the first such position on ties), and when step k searches them those
+// diagonal entries are still the input's: the factorisation changes A(k,k)
+// and the column below it at step k only.  So the transpositions follow
+// from the input diagonal alone, and the factorisation is the unpivoted one
+// of P H P^T, the same operations on the same values as Eigen's.  H and b
+// are read through the permutation, and `scratch` (6 doubles) undoes it; on
+// the device all three live in LDS.  The unrolled factorisation and solves
+// then use static register indices only.
+__host__ __device__ inline void ldlt_solve6_presorted(const float* H, const float* b, double* scratch,
+                                                      double (&x)[6]) {
   constexpr int n = 6;
-  int tr[n];
+  double d[n];
+  int p[n];
+#pragma unroll
+  for (int i = 0; i < n; ++i) {
+    d[i] = (double)H[i * n + i];
+    p[i] = i;
+  }
   bool stop = false;
 #pragma unroll
   for (int k = 0; k < n; ++k) {
-    if (stop) continue;
     int big = k;
-    double bv = fabs(A[k * n + k]);
+    double bv = fabs(d[k]);
 #pragma unroll
     for (int i = k + 1; i < n; ++i)
-      if (fabs(A[i * n + i]) > bv) {
-        bv = fabs(A[i * n + i]);
+      if (fabs(d[i]) > bv) {
+        bv = fabs(d[i]);
         big = i;
       }
-    tr[k] = big;
 #pragma unroll
-    for (int b = k + 1; b < n; ++b) {
-      if (b != big) continue;
-#pragma unroll
-      for (int j = 0; j < k; ++j) swap_d(A[k * n + j], A[b * n + j]);
-#pragma unroll
-      for (int i = b + 1; i < n; ++i) swap_d(A[i * n + k], A[i * n + b]);
-      swap_d(A[k * n + k], A[b * n + b]);
-#pragma unroll
-      for (int i = k + 1; i < b; ++i) swap_d(A[i * n + k], A[b * n + i]);
+    for (int i = k + 1; i < n; ++i) {
+      const bool sw = i == big;
+      swap_if(sw, d[k], d[i]);
+      const int pk = p[k], pi = p[i];
+      p[k] = sw ? pi : pk;
+      p[i] = sw ? pk : pi;
     }
-    if (k > 0) {
-      double temp[n];
+    if (k == 0 && !(fabs(d[0]) > 0.0)) stop = true;  // no pivot at all (then big was 0: nothing moved)
+  }
+  if (stop) {
 #pragma unroll
-      for (int j = 0; j < k; ++j) temp[j] = A[j * n + j] * A[k * n + j];
-      double s = 0.0;
+    for (int i = 0; i < n; ++i) p[i] = i;
+  }
+  double A[n * n], y[n];
 #pragma unroll
-      for (int j = 0; j < k; ++j) s += A[k * n + j] * temp[j];
-      A[k * n + k] -= s;
+  for (int i = 0; i < n; ++i) {
 #pragma unroll
-      for (int i = k + 1; i < n; ++i) {
-        double r = 0.0;
+    for (int j = 0; j <= i; ++j) A[i * n + j] = (double)H[p[i] * n + p[j]];
+    y[i] = (double)b[p[i]];
+  }
+  if (!stop) {
 #pragma unroll
-        for (int j = 0; j < k; ++j) r += A[i * n + j] * temp[j];
-        A[i * n + k] -= r;
+    for (int k = 0; k < n; ++k) {
+      if (k > 0) {
+        double temp[n];
+#pragma unroll
+        for (int j = 0; j < k; ++j) temp[j] = A[j * n + j] * A[k * n + j];
+        double s = 0.0;
+#pragma unroll
+        for (int j = 0; j < k; ++j) s += A[k * n + j] * temp[j];
+        A[k * n + k] -= s;
+#pragma unroll
+        for (int i = k + 1; i < n; ++i) {
+          double r = 0.0;
+#pragma unroll
+          for (int j = 0; j < k; ++j) r += A[i * n + j] * temp[j];
+          A[i * n + k] -= r;
+        }
+      }
+      const double akk = A[k * n + k];
+      if (fabs(akk) > 0.0) {
+#pragma unroll
+        for (int i = k + 1; i < n; ++i) A[i * n + k] /= akk;
       }
     }
-    const double akk = A[k * n + k];
-    const bool valid = fabs(akk) > 0.0;
-    if (k == 0 && !valid) {
-#pragma unroll
-      for (int j = 0; j < n; ++j) tr[j] = j;
-      stop = true;
-      continue;
-    }
-    if (valid) {
-#pragma unroll
-      for (int i = k + 1; i < n; ++i) A[i * n + k] /= akk;
-    }
   }
-#pragma unroll
-  for (int k = 0; k < n; ++k)
-#pragma unroll
-    for (int m = k + 1; m < n; ++m) swap_if(m == tr[k], y[k], y[m]);
 #pragma unroll
   for (int i = 0; i < n; ++i)
 #pragma unroll
     for (int j = 0; j < i; ++j) y[i] -= A[i * n + j] * y[j];
 #pragma unroll
   for (int i = 0; i < n; ++i) {
-    const double d = A[i * n + i];
-    y[i] = fabs(d) > DBL_MIN ? y[i] / d : 0.0;
+    const double dd = A[i * n + i];
+    y[i] = fabs(dd) > DBL_MIN ? y[i] / dd : 0.0;
   }
 #pragma unroll
   for (int i = n - 1; i >= 0; --i)
 #pragma unroll
     for (int j = i + 1; j < n; ++j) y[i] -= A[j * n + i] * y[j];
 #pragma unroll
-  for (int k = n - 1; k >= 0; --k)
+  for (int i = 0; i < n; ++i) scratch[p[i]] = y[i];
 #pragma unroll
-    for (int m = k + 1; m < n; ++m) swap_if(m == tr[k], y[k], y[m]);
+  for (int i = 0; i < n; ++i) x[i] = scratch[i];
 }
 
 // SE(3) exponential, twist (rho, omega), Rodrigues' formula.
@@ -774,7 +784,8 @@ __device__ __attribute__((always_inline)) inline void gn_step_lane0(const float 
                                                                     int s_good, float s_last_err, TrackState* st,
                                                                     double* est_out, int gi, int gr, int mode,
                                                                     int first_of_level, int do_step, int record,
-                                                                    float wrh, float wrb, float wih, float wib) {
+                                                                    float wrh, float wrb, float wih, float wib,
+                                                                    float* lds_H, float* lds_b, double* lds_x) {
   if (st && record && gi) st->icp_inliers = (int)sums[0][28];
   if (st && record && gr) st->rgb_inliers = (int)sums[1][28];
   if (!do_step) return;
@@ -812,15 +823,13 @@ __device__ __attribute__((always_inline)) inline void gn_step_lane0(const float 
     rsum = sums[0][27];
   }
   const float err = sqrtf(rsum) / (nres + 6.0f);
-  double A[36], y[6];
+  // the solve reads H and b through its pivot permutation, so from LDS
 #pragma unroll
-  for (int i = 0; i < 36; ++i) A[i] = (double)H[i];
+  for (int i = 0; i < 36; ++i) lds_H[i] = H[i];
 #pragma unroll
-  for (int i = 0; i < 6; ++i) y[i] = (double)b[i];
-  ldlt_solve6(A, y);
+  for (int i = 0; i < 6; ++i) lds_b[i] = b[i];
   double xi[6];
-#pragma unroll
-  for (int i = 0; i < 6; ++i) xi[i] = y[i];
+  ldlt_solve6_presorted(lds_H, lds_b, lds_x, xi);
   if (isnan(xi[0])) {
     if (st) st->good = 0;
     return;
@@ -859,6 +868,8 @@ __device__ __attribute__((always_inline)) inline void gn_update(const TrackState
   __shared__ double s_est[16];
   __shared__ int s_good;
   __shared__ float s_last_err;
+  __shared__ float s_H[36], s_b[6];  // thread 0's solve
+  __shared__ double s_x[6];
   // the state the step reads, loaded alongside the partial rows
   if (threadIdx.x < 16) s_est[threadIdx.x] = st_in->est[threadIdx.x];
   if (threadIdx.x == 16) s_good = st_in->good;
@@ -902,7 +913,7 @@ __device__ __attribute__((always_inline)) inline void gn_update(const TrackState
     for (int i = 0; i < 16; ++i) est_out[i] = s_est[i];
   }
   gn_step_lane0(sums, s_est, s_good, s_last_err, write ? st_out : nullptr, est_out, gi, gr, mode, first_of_level,
-                do_step, record, wrh, wrb, wih, wib);
+                do_step, record, wrh, wrb, wih, wib, s_H, s_b, s_x);
 }
 __device__ inline void update_at(TrackState* st, const float* part_icp, int gi, const float* part_rgb, int gr,
                                  int mode, int first_of_level, int do_step, int record, float wrh, float wrb, float wih,
