@@ -91,8 +91,8 @@ __device__ inline void chan_merge(float& n, float& mean, float& m2, float nb, fl
   n = nn;
 }
 
-// Workgroup merge of per-thread statistics; the result is broadcast to all threads.
-__device__ inline void block_stats(float& n, float& mean, float& m2, float* lds /* [3 * nwaves] */) {
+// Wavefront merge of per-thread statistics: every lane gets the wavefront's.
+__device__ inline void wave_stats(float& n, float& mean, float& m2) {
 #pragma unroll
   for (int o = 1; o < 64; o <<= 1) {
     float nb = __shfl_xor(n, o, 64), mb = __shfl_xor(mean, o, 64), qb = __shfl_xor(m2, o, 64);
@@ -105,6 +105,11 @@ __device__ inline void block_stats(float& n, float& mean, float& m2, float* lds 
       n = n0; mean = m0; m2 = q0;
     }
   }
+}
+
+// Workgroup merge of per-thread statistics; the result is broadcast to all threads.
+__device__ inline void block_stats(float& n, float& mean, float& m2, float* lds /* [3 * nwaves] */) {
+  wave_stats(n, mean, m2);
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
   if (lane == 0) {
     lds[3 * wid] = n; lds[3 * wid + 1] = mean; lds[3 * wid + 2] = m2;
@@ -119,6 +124,56 @@ __device__ inline void block_stats(float& n, float& mean, float& m2, float* lds 
   __syncthreads();
   n = lds[0]; mean = lds[1]; m2 = lds[2];
   __syncthreads();
+}
+
+// block_sum32 of two arrays in one LDS round: their wavefront sums overlap
+// and one pair of barriers serves both.  Thread t < 32 returns value t's
+// total of `a`, thread 32 + t that of `b`, each summed as block_sum32 does.
+__device__ inline float block_sum32x2(float (&a)[32], float (&b)[32], float* lds /* [2 * nwaves * 32] */) {
+  const float wa = wave_sum32(a);
+  const float wb = wave_sum32(b);
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  if ((lane & 1) == 0) {
+    lds[wid * 32 + (lane >> 1)] = wa;
+    lds[(nw + wid) * 32 + (lane >> 1)] = wb;
+  }
+  __syncthreads();
+  float t = 0.0f;
+  if (threadIdx.x < 64) {
+    const int base = threadIdx.x < 32 ? 0 : nw * 32, i = threadIdx.x & 31;
+    t = lds[base + i];
+    for (int k = 1; k < nw; ++k) t += lds[base + k * 32 + i];
+  }
+  __syncthreads();
+  return t;
+}
+
+// The end of a pass 1 with both terms: block_sum32 of the ICP values and
+// block_stats of the residual statistics in one LDS round.  Thread t < 32
+// returns ICP value t's total; thread 32 gets the merged statistics in (n,
+// mean, m2).  The same orders as the two separate reductions.  No barrier
+// follows, so the caller must not use `lds` again.
+__device__ inline float block_end_pass1(float (&acc)[32], float& n, float& mean, float& m2,
+                                        float* lds /* [nwaves * 35] */) {
+  const float w = wave_sum32(acc);
+  wave_stats(n, mean, m2);
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  float* st = lds + nw * 32;
+  if ((lane & 1) == 0) lds[wid * 32 + (lane >> 1)] = w;
+  if (lane == 0) {
+    st[3 * wid] = n; st[3 * wid + 1] = mean; st[3 * wid + 2] = m2;
+  }
+  __syncthreads();
+  float t = 0.0f;
+  if (threadIdx.x < 32) {
+    t = lds[threadIdx.x];
+    for (int k = 1; k < nw; ++k) t += lds[k * 32 + threadIdx.x];
+  } else if (threadIdx.x == 32) {
+    float a = st[0], b = st[1], c = st[2];
+    for (int k = 1; k < nw; ++k) chan_merge(a, b, c, st[3 * k], st[3 * k + 1], st[3 * k + 2]);
+    n = a; mean = b; m2 = c;
+  }
+  return t;
 }
 
 // A value every lane holds alike (one LDS word read by all), moved into a
@@ -496,7 +551,7 @@ __device__ inline void pass1_at(const Level& L, const TrackBufs& B, const TrackS
                                 const StepArgs* sa = nullptr) {
   __shared__ float pose[12];
   __shared__ float wp[12];
-  __shared__ float lds[4 * 32];
+  __shared__ float lds[4 * 35];
   __shared__ double s_est[STEP ? 16 : 1];
   const int N = L.w * L.h;
   const int stride = nb * blockDim.x;
@@ -537,11 +592,18 @@ __device__ inline void pass1_at(const Level& L, const TrackBufs& B, const TrackS
     if (ICP) icp_pixel(L, B, T, angle_th, dist_th, cur.p, acc);
     if (RGB) rgb_pixel(L, B, wu, k, cur.iz, cur.inten, n, mean, m2);
   }
-  if (ICP) {
+  if (ICP && RGB) {
+    const float t = block_end_pass1(acc, n, mean, m2, lds);
+    if (threadIdx.x < 32) part_icp[bx * 32 + threadIdx.x] = t;
+    if (threadIdx.x == 32) {
+      part_res[bx * 4] = n;
+      part_res[bx * 4 + 1] = mean;
+      part_res[bx * 4 + 2] = m2;
+    }
+  } else if (ICP) {
     float t = block_sum32(acc, lds);
     if (threadIdx.x < 32) part_icp[bx * 32 + threadIdx.x] = t;
-  }
-  if (RGB) {
+  } else {
     block_stats(n, mean, m2, lds);
     if (threadIdx.x == 0) {
       part_res[bx * 4] = n;
@@ -863,7 +925,7 @@ __device__ __attribute__((always_inline)) inline void gn_update(const TrackState
                                                                 const float* part_rgb, int gr, int mode,
                                                                 int first_of_level, int do_step, int record, float wrh,
                                                                 float wrb, float wih, float wib, double* est_out) {
-  __shared__ float lds[4 * 32];
+  __shared__ float lds[2 * 4 * 32];
   __shared__ float sums[2][32];
   __shared__ double s_est[16];
   __shared__ int s_good;
@@ -889,11 +951,13 @@ __device__ __attribute__((always_inline)) inline void gn_update(const TrackState
       for (int i = 0; i < 32; ++i) vr[i] += part_rgb[r * 32 + i];
     }
   }
-  if (gi) {
+  if (gi && gr) {
+    const float t = block_sum32x2(vi, vr, lds);
+    if (threadIdx.x < 64) sums[threadIdx.x >> 5][threadIdx.x & 31] = t;
+  } else if (gi) {
     const float t = block_sum32(vi, lds);
     if (threadIdx.x < 32) sums[0][threadIdx.x] = t;
-  }
-  if (gr) {
+  } else if (gr) {
     const float t = block_sum32(vr, lds);
     if (threadIdx.x < 32) sums[1][threadIdx.x] = t;
   }
