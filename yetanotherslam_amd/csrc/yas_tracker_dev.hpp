@@ -107,23 +107,19 @@ __device__ inline void wave_stats(float& n, float& mean, float& m2) {
   }
 }
 
-// Workgroup merge of per-thread statistics; the result is broadcast to all threads.
-__device__ inline void block_stats(float& n, float& mean, float& m2, float* lds /* [3 * nwaves] */) {
+// Workgroup merge of per-thread statistics; every thread gets the result,
+// merging the wavefronts' in the same order itself (one barrier).  `st` is
+// read after the barrier, so nothing in the kernel may write it afterwards.
+__device__ inline void block_stats(float& n, float& mean, float& m2, float* st /* [3 * nwaves] */) {
   wave_stats(n, mean, m2);
-  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
   if (lane == 0) {
-    lds[3 * wid] = n; lds[3 * wid + 1] = mean; lds[3 * wid + 2] = m2;
+    st[3 * wid] = n; st[3 * wid + 1] = mean; st[3 * wid + 2] = m2;
   }
   __syncthreads();
-  if (threadIdx.x == 0) {
-    const int nw = blockDim.x >> 6;
-    float a = lds[0], b = lds[1], c = lds[2];
-    for (int k = 1; k < nw; ++k) chan_merge(a, b, c, lds[3 * k], lds[3 * k + 1], lds[3 * k + 2]);
-    lds[0] = a; lds[1] = b; lds[2] = c;
-  }
-  __syncthreads();
-  n = lds[0]; mean = lds[1]; m2 = lds[2];
-  __syncthreads();
+  float a = st[0], b = st[1], c = st[2];
+  for (int k = 1; k < nw; ++k) chan_merge(a, b, c, st[3 * k], st[3 * k + 1], st[3 * k + 2]);
+  n = a; mean = b; m2 = c;
 }
 
 // block_sum32 of two arrays in one LDS round: their wavefront sums overlap
@@ -604,7 +600,7 @@ __device__ inline void pass1_at(const Level& L, const TrackBufs& B, const TrackS
     float t = block_sum32(acc, lds);
     if (threadIdx.x < 32) part_icp[bx * 32 + threadIdx.x] = t;
   } else {
-    block_stats(n, mean, m2, lds);
+    block_stats(n, mean, m2, lds + 4 * 32);
     if (threadIdx.x == 0) {
       part_res[bx * 4] = n;
       part_res[bx * 4 + 1] = mean;
@@ -621,7 +617,7 @@ __device__ inline void pass1_at(const Level& L, const TrackBufs& B, const TrackS
 template <bool PREFETCH = false>
 __device__ inline void pass2_at(const Level& L, const TrackBufs& B, TrackState* st, const float* part_res, int g1,
                                 float* part_rgb, int bx, int nb) {
-  __shared__ float lds[4 * 32];
+  __shared__ float lds[4 * 35];
   __shared__ float wp[12];
   const int N = L.w * L.h;
   const int stride = nb * blockDim.x;
@@ -636,7 +632,7 @@ __device__ inline void pass2_at(const Level& L, const TrackBufs& B, TrackState* 
   float n = 0.0f, mean = 0.0f, m2 = 0.0f;
   for (int g = threadIdx.x; g < g1; g += blockDim.x)
     chan_merge(n, mean, m2, part_res[g * 4], part_res[g * 4 + 1], part_res[g * 4 + 2]);
-  block_stats(n, mean, m2, lds);
+  block_stats(n, mean, m2, lds + 4 * 32);  // apart from the region block_sum32 uses below
   const float sigma = sqrtf(m2 / (n - 1.0f));
   const float th = (float)(4.685 * (double)sigma);
   if (bx == 0 && threadIdx.x == 0) {
