@@ -455,6 +455,10 @@ int yas_tracker_create(yas_ctx* ctx, int W, int H, const double K[4], const yas_
     a.fx = (float)a.Kd[0]; a.fy = (float)a.Kd[1]; a.cx = (float)a.Kd[2]; a.cy = (float)a.Kd[3];
     a.invfx = (float)(1.0 / a.Kd[0]);
     a.invfy = (float)(1.0 / a.Kd[1]);
+    a.Kinv[0] = 1.0 / a.Kd[0];
+    a.Kinv[1] = 1.0 / a.Kd[1];
+    a.Kinv[2] = -a.Kd[2] / a.Kd[0];
+    a.Kinv[3] = -a.Kd[3] / a.Kd[1];
     size_t n = (size_t)a.w * a.h;
     for (float** p : {&t->cur_int[l], &t->ref_int[l], &t->gx[l], &t->gy[l], &t->cur_inv[l], &t->ref_inv[l]})
       alloc((void**)p, n * sizeof(float));

@@ -21,6 +21,7 @@ struct Level {
   int w, h;
   float fx, fy, cx, cy, invfx, invfy;
   double Kd[4];
+  double Kinv[4];  // K^-1 terms in double: 1/fx, 1/fy, -cx/fx, -cy/fy (computed on the host)
 };
 
 struct TrackState {
@@ -435,12 +436,13 @@ __device__ inline void icp_pixel(const Level& L, const TrackBufs& B, const PoseF
 }
 
 // Warp parameters of ImageProc.cu:42-78: RK = R K^-1 (formed in double), t.
-__device__ inline void load_warp(const double* T, const double* Kd, float* lds /* 12 */) {
+// Kinv holds K^-1's terms (Level::Kinv), so no division is on this path.
+__device__ inline void load_warp(const double* T, const double* Kinv, float* lds /* 12 */) {
   if (threadIdx.x >= 16 && threadIdx.x < 28) {
     int q = threadIdx.x - 16;
     int i = q / 4, j = q % 4;
     if (j < 3) {
-      double Ki[9] = {1.0 / Kd[0], 0.0, -Kd[2] / Kd[0], 0.0, 1.0 / Kd[1], -Kd[3] / Kd[1], 0.0, 0.0, 1.0};
+      double Ki[9] = {Kinv[0], 0.0, Kinv[2], 0.0, Kinv[1], Kinv[3], 0.0, 0.0, 1.0};
       lds[3 * i + j] = (float)(T[4 * i] * Ki[j] + T[4 * i + 1] * Ki[3 + j] + T[4 * i + 2] * Ki[6 + j]);
     } else {
       lds[9 + i] = (float)T[4 * i + 3];
@@ -564,7 +566,7 @@ __device__ inline void pass1_at(const Level& L, const TrackBufs& B, const TrackS
     est = st->est;
   }
   load_pose_f(est, pose);
-  if (RGB) load_warp(est, L.Kd, wp);
+  if (RGB) load_warp(est, L.Kinv, wp);
   __syncthreads();
   PoseF T;
 #pragma unroll
@@ -628,7 +630,7 @@ __device__ inline void pass2_at(const Level& L, const TrackBufs& B, TrackState* 
     e_next = B.resid[k];
     iz_next = B.ref_inv[k];
   }
-  load_warp(st->est, L.Kd, wp);
+  load_warp(st->est, L.Kinv, wp);
   float n = 0.0f, mean = 0.0f, m2 = 0.0f;
   for (int g = threadIdx.x; g < g1; g += blockDim.x)
     chan_merge(n, mean, m2, part_res[g * 4], part_res[g * 4 + 1], part_res[g * 4 + 2]);
