@@ -14,6 +14,7 @@ Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline] [
 Prints one JSON line on stdout; progress goes to stderr.
 """
 import argparse
+import gc
 import json
 import math
 import os
@@ -348,6 +349,12 @@ def copy_probe():
 
 
 def main():
+    # Python's cyclic collector stays off for the run: with this process's
+    # heap a full collection takes tens of milliseconds, and one landing in a
+    # timed region (a serving run's submission loop, say) would be measured
+    # as device time.  Reference counting still frees everything this
+    # script allocates.
+    gc.disable()
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=290)
