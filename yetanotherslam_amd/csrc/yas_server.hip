@@ -261,10 +261,20 @@ k_s_pass1(const StreamDev* streams, const Table* tab, int l, int slot0) {
                            S.groups[l]);
 }
 
+// The Huber threshold of a linearisation, once per stream (one workgroup
+// each) rather than in every workgroup of pass 2: at the batch's size that
+// fold was about a third of pass 2's instructions.
+__global__ void __launch_bounds__(256) k_s_threshold(const StreamDev* streams, const Table* tab, int l, int slot0) {
+  SLOT_FROM_OR_RETURN(slot0);
+  if (sl.first) return;
+  __shared__ float st_lds[3 * 4];
+  huber_threshold(S.st, S.part_res, S.groups[l], st_lds, true);
+}
+
 __global__ void __launch_bounds__(256) k_s_pass2(const StreamDev* streams, const Table* tab, int l, int slot0) {
   SLOT_FROM_OR_RETURN(slot0);
   if (sl.first || (int)blockIdx.x >= S.groups[l]) return;
-  pass2_at<true>(S.lv[l], S.tb[l], S.st, S.part_res, S.groups[l], S.part_rgb, blockIdx.x, S.groups[l]);
+  pass2_at<true, true>(S.lv[l], S.tb[l], S.st, S.part_res, S.groups[l], S.part_rgb, blockIdx.x, S.groups[l]);
 }
 
 __global__ void __launch_bounds__(256) k_s_update(const StreamDev* streams, const Table* tab, int l, int icp, int rgb,
@@ -419,7 +429,10 @@ static int launch_batch_kernels(yas_server* sv, hipStream_t s, int nz) {
         if (icp && rgb) hipLaunchKernelGGL((k_s_pass1<true, true>), dim3(G, 1, gz), dim3(256), 0, s, S, T, l, s0);
         else if (icp) hipLaunchKernelGGL((k_s_pass1<true, false>), dim3(G, 1, gz), dim3(256), 0, s, S, T, l, s0);
         else hipLaunchKernelGGL((k_s_pass1<false, true>), dim3(G, 1, gz), dim3(256), 0, s, S, T, l, s0);
-        if (rgb) hipLaunchKernelGGL(k_s_pass2, dim3(G, 1, gz), dim3(256), 0, s, S, T, l, s0);
+        if (rgb) {
+          hipLaunchKernelGGL(k_s_threshold, dim3(1, 1, gz), dim3(256), 0, s, S, T, l, s0);
+          hipLaunchKernelGGL(k_s_pass2, dim3(G, 1, gz), dim3(256), 0, s, S, T, l, s0);
+        }
         hipLaunchKernelGGL(k_s_update, dim3(1, 1, gz), dim3(256), 0, s, S, T, l, icp ? 1 : 0, rgb ? 1 : 0,
                            sv->tp.mode, it == 0 ? 1 : 0, sv->tp.w_rgb_h, sv->tp.w_rgb_b, sv->tp.w_icp_h,
                            sv->tp.w_icp_b, s0);

@@ -616,7 +616,29 @@ __device__ inline void pass1_at(const Level& L, const TrackBufs& B, const TrackS
 // sigma and the Huber threshold 4.685 sigma, and reduce the weighted rows.
 // PREFETCH: each pixel's residual and inverse depth are loaded one pixel
 // ahead (the first pixel's while the statistics are folded).
-template <bool PREFETCH = false>
+// The residual statistics of pass 1 folded in a fixed order (every thread
+// gets the same values), with sigma's Huber threshold 4.685 sigma; thread 0
+// of workgroup 0 records {sum r, count, mean, threshold} when `record`.
+__device__ inline float huber_threshold(TrackState* st, const float* part_res, int g1, float* st_lds, bool record) {
+  float n = 0.0f, mean = 0.0f, m2 = 0.0f;
+  for (int g = threadIdx.x; g < g1; g += blockDim.x)
+    chan_merge(n, mean, m2, part_res[g * 4], part_res[g * 4 + 1], part_res[g * 4 + 2]);
+  block_stats(n, mean, m2, st_lds);
+  const float sigma = sqrtf(m2 / (n - 1.0f));
+  const float th = (float)(4.685 * (double)sigma);
+  if (record && threadIdx.x == 0) {
+    st->rgb_stats[0] = n * mean;
+    st->rgb_stats[1] = n;
+    st->rgb_stats[2] = mean;
+    st->rgb_stats[3] = th;
+  }
+  return th;
+}
+
+// PRESTATS: the threshold was folded by a launch of its own (one workgroup
+// per tracker, huber_threshold) and is read from the state, instead of
+// every workgroup folding it again.
+template <bool PREFETCH = false, bool PRESTATS = false>
 __device__ inline void pass2_at(const Level& L, const TrackBufs& B, TrackState* st, const float* part_res, int g1,
                                 float* part_rgb, int bx, int nb) {
   __shared__ float lds[4 * 35];
@@ -631,17 +653,13 @@ __device__ inline void pass2_at(const Level& L, const TrackBufs& B, TrackState* 
     iz_next = B.ref_inv[k];
   }
   load_warp(st->est, L.Kinv, wp);
-  float n = 0.0f, mean = 0.0f, m2 = 0.0f;
-  for (int g = threadIdx.x; g < g1; g += blockDim.x)
-    chan_merge(n, mean, m2, part_res[g * 4], part_res[g * 4 + 1], part_res[g * 4 + 2]);
-  block_stats(n, mean, m2, lds + 4 * 32);  // apart from the region block_sum32 uses below
-  const float sigma = sqrtf(m2 / (n - 1.0f));
-  const float th = (float)(4.685 * (double)sigma);
-  if (bx == 0 && threadIdx.x == 0) {
-    st->rgb_stats[0] = n * mean;
-    st->rgb_stats[1] = n;
-    st->rgb_stats[2] = mean;
-    st->rgb_stats[3] = th;
+  float th;
+  if (PRESTATS) {
+    th = st->rgb_stats[3];
+    __syncthreads();  // the warp parameters in LDS
+  } else {
+    // apart from the region block_sum32 uses below
+    th = huber_threshold(st, part_res, g1, lds + 4 * 32, bx == 0);
   }
   float wu[12];
 #pragma unroll
