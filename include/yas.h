@@ -98,8 +98,16 @@ const char* yas_version(void);
 const char* yas_last_error(void);
 
 /* Context: device + HIP stream.  stream == NULL creates a library-owned
- * stream; otherwise the caller's hipStream_t is used (not destroyed). */
+ * non-blocking stream; otherwise the caller's hipStream_t is used (not
+ * destroyed).  The reference launches every kernel on the legacy default
+ * stream (no stream argument in RGBDFusion/src/*.cu), which
+ * yas_ctx_create_null_stream binds to: work is then ordered with every other
+ * user of the device's null stream (e.g. torch's default stream).
+ * Objects built on a context hold a reference to it: yas_ctx_destroy drops
+ * the creator's reference, and the context is freed after its last object,
+ * so destroy order does not matter. */
 int yas_ctx_create(int device, void* stream, yas_ctx** out);
+int yas_ctx_create_null_stream(int device, yas_ctx** out);
 int yas_ctx_destroy(yas_ctx* ctx);
 int yas_ctx_synchronize(yas_ctx* ctx);
 void* yas_ctx_stream(yas_ctx* ctx);

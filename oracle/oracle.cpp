@@ -1426,7 +1426,9 @@ template <class R> struct Tracker : TrackerBase {
         }
       }
     }
-    if (swap) swap_buffers();
+    // RGBDTracking.cu:187-191: a NaN update returns the identity before the
+    // buffer swap
+    if (swap && good) swap_buffers();
     if (!good) identity_d(last_ok);
     std::memcpy(out, last_ok, sizeof(last_ok));
     return good;
@@ -1629,8 +1631,8 @@ void orc_pipe_free(void* p) { delete (Pipeline*)p; }
 void* orc_pipe_map(void* p) { return ((Pipeline*)p)->map.get(); }
 void* orc_pipe_tracker(void* p) { return ((Pipeline*)p)->trk.get(); }
 
-// One frame; returns 1 if tracked and fused, 0 if rejected (|t| > 0.1 m or
-// a NaN update).  T_out = camera-to-map pose of this frame.
+// One frame; returns 1 if tracked and fused, 0 if rejected (|t| > 0.1 m).
+// T_out = camera-to-map pose of this frame.
 int orc_pipe_step(void* pp, const uint16_t* depth, const uint8_t* rgb, double T_out[16], double times_ms[4]) {
   Pipeline* p = (Pipeline*)pp;
   typedef std::chrono::steady_clock clk;
@@ -1659,8 +1661,11 @@ int orc_pipe_step(void* pp, const uint16_t* depth, const uint8_t* rgb, double T_
     identity_d(I);
     int good = p->trk->get_transform(I, 1, DT);
     auto c = clk::now();
+    // Tracking.cpp:178-187 gates on |t| alone: the identity that a NaN
+    // update returns is accepted and fused at T_last.
+    (void)good;
     double tn = std::sqrt(DT[3] * DT[3] + DT[7] * DT[7] + DT[11] * DT[11]);
-    if (!good || tn > 0.1) {
+    if (tn > 0.1) {
       ok = 0;
     } else {
       double Di[16];

@@ -94,8 +94,9 @@ class OracleMap:
         self.precision = precision
 
     def __del__(self):
-        if getattr(self, "h", None):
-            lib().orc_map_free(self.h)
+        # at interpreter exit the module's globals may already be gone
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.orc_map_free(self.h)
             self.h = None
 
     def fuse(self, depth_f32, K, T):
@@ -242,8 +243,9 @@ class OracleTracker:
         self.h = lib().orc_trk_new(precision, W, H, _p(_f64(K)), levels, _p(it), mode, _p(w))
 
     def __del__(self):
-        if getattr(self, "h", None):
-            lib().orc_trk_free(self.h)
+        # at interpreter exit the module's globals may already be gone
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.orc_trk_free(self.h)
             self.h = None
 
     def _f32(self, a):
@@ -299,8 +301,9 @@ class OraclePipeline:
         self.W, self.H = W, H
 
     def __del__(self):
-        if getattr(self, "h", None):
-            lib().orc_pipe_free(self.h)
+        # at interpreter exit the module's globals may already be gone
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.orc_pipe_free(self.h)
             self.h = None
 
     def step(self, depth_u16, rgb):

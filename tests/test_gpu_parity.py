@@ -95,9 +95,13 @@ def test_image_ops_bit_exact():
     dc = to_dev(rgb)
     lib.call("yas_depth_u16_to_f32", ctx.h, rf._dev(du), W * 2, rf._dev(dz), W * 4, W, H, 1000.0)
     lib.call("yas_rgb_to_intensity", ctx.h, rf._dev(dc), W * 3, rf._dev(dg), W * 4, W, H, 0)
+    # OpenCV's 8-bit fixed-point gray (System.cpp:47-49)
+    dg8 = torch.empty((H, W), dtype=torch.float32, device="cuda")
+    lib.call("yas_rgb_to_intensity", ctx.h, rf._dev(dc), W * 3, rf._dev(dg8), W * 4, W, H, 1)
     torch.cuda.synchronize()
     assert_bits_equal(z, dz.cpu().numpy(), "depth u16->f32")
     assert_bits_equal(g, dg.cpu().numpy(), "intensity")
+    assert_bits_equal(po.rgb_to_intensity(rgb, mode=1), dg8.cpu().numpy(), "8-bit gray")
     # Gaussian pyramid level
     out = torch.empty((H // 2, W // 2), dtype=torch.float32, device="cuda")
     lib.call("yas_pyrdown_gauss", ctx.h, rf._dev(dg), W * 4, W, H, rf._dev(out), (W // 2) * 4)

@@ -43,6 +43,8 @@ void yas_synth_default_params(yas_synth_params* p);
 int yas_synth_frame(const yas_synth_params* p, int frame, uint16_t* depth,
                     uint8_t* rgb, float* clean_depth, double T_world_cam[16]);
 int yas_synth_pose(const yas_synth_params* p, int frame, double T_world_cam[16]);
+int yas_synth_render(const yas_synth_params* p, const double T_world_cam[16], int frame, uint16_t* depth,
+                     uint8_t* rgb, float* clean_depth);
 int yas_synth_spheres(const yas_synth_params* p, double out[12]);
 }
 
@@ -254,14 +256,15 @@ int yas_synth_spheres(const yas_synth_params* p, double out[12]) {
   return 0;
 }
 
-int yas_synth_frame(const yas_synth_params* p, int frame, uint16_t* depth, uint8_t* rgb,
-                    float* clean_depth, double T_world_cam[16]) {
-  if (!p || p->width <= 0 || p->height <= 0) return 1;
+// The scene of `p` seen from an arbitrary camera pose T (camera -> world);
+// `frame` selects the noise and dropout draws.
+int yas_synth_render(const yas_synth_params* p, const double T_in[16], int frame, uint16_t* depth, uint8_t* rgb,
+                     float* clean_depth) {
+  if (!p || !T_in || p->width <= 0 || p->height <= 0) return 1;
   const yas_synth_params& P = *p;
   Scene sc(P);
   double T[16];
-  pose_of(P, frame, T);
-  if (T_world_cam) std::memcpy(T_world_cam, T, sizeof(T));
+  std::memcpy(T, T_in, sizeof(T));
   const double light[3] = {0.3, 2.8, -0.2};
   const int W = P.width, H = P.height;
 #pragma omp parallel for schedule(static)
@@ -301,6 +304,15 @@ int yas_synth_frame(const yas_synth_params* p, int frame, uint16_t* depth, uint8
     }
   }
   return 0;
+}
+
+int yas_synth_frame(const yas_synth_params* p, int frame, uint16_t* depth, uint8_t* rgb,
+                    float* clean_depth, double T_world_cam[16]) {
+  if (!p) return 1;
+  double T[16];
+  pose_of(*p, frame, T);
+  if (T_world_cam) std::memcpy(T_world_cam, T, sizeof(T));
+  return yas_synth_render(p, T, frame, depth, rgb, clean_depth);
 }
 
 }  // extern "C"

@@ -251,7 +251,7 @@ int yas_batch_create(yas_ctx* ctx, int num_maps, yas_map* const* maps, yas_batch
   }
   YAS_HIP(hipSetDevice(ctx->device));
   yas_batch* b = new yas_batch();
-  b->ctx = ctx;
+  b->ctx = yas_ctx_retain(ctx);
   b->n = num_maps;
   b->maps.assign(maps, maps + num_maps);
   b->format = fmt;
@@ -287,6 +287,7 @@ int yas_batch_destroy(yas_batch* b) {
   if (b->t0) hipEventDestroy(b->t0);
   if (b->t1) hipEventDestroy(b->t1);
   if (b->stream) hipStreamDestroy(b->stream);
+  yas_ctx_release(b->ctx);
   delete b;
   return YAS_OK;
 }

@@ -11,6 +11,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <atomic>
 #include <cstdint>
 #include <cstdio>
 #include <mutex>
@@ -61,7 +62,13 @@ struct yas_ctx {
   int device;
   hipStream_t stream;
   bool own_stream;
+  std::atomic<int> refs;  // the creator's reference plus one per object built on it
 };
+
+// Every object built on a context holds a reference to it, so the context
+// (and its stream) outlives them whatever order they are destroyed in.
+yas_ctx* yas_ctx_retain(yas_ctx* c);
+void yas_ctx_release(yas_ctx* c);
 
 namespace yas {
 
@@ -343,14 +350,18 @@ __device__ inline int find_block_xyz(const MapView& m, int bx, int by, int bz, i
 }
 
 // ------------------------------------------------------------------ frame outcome
-// Tracking.cpp:178-187: reject |t| > 0.1 m (or a NaN update), otherwise
-// T_last <- T_last * DT^-1.  Records the frame's pose and outcome and sets
-// the gate that lets the fusion kernels run.  One thread.
+// Tracking.cpp:178-187: reject |t| > 0.1 m, otherwise
+// T_last <- T_last * DT^-1.  After a NaN update the reference's GetTransform
+// returns the identity (RGBDTracking.cu:187-191), which passes the gate:
+// the frame is accepted and fused at T_last * I^-1 = T_last.  Records the
+// frame's pose and outcome and sets the gate that lets the fusion kernels
+// run.  One thread.
 __device__ inline void post_track_at(const double* DT, const int* good, double* T_last, double* pose_out,
                                      int* tracked, int* gate) {
-  double tn = sqrt(DT[3] * DT[3] + DT[7] * DT[7] + DT[11] * DT[11]);
-  int ok = (*good != 0) && !(tn > 0.1);
-  if (ok) {
+  const bool converged = *good != 0;
+  double tn = converged ? sqrt(DT[3] * DT[3] + DT[7] * DT[7] + DT[11] * DT[11]) : 0.0;
+  int ok = !(tn > 0.1);
+  if (ok && converged) {
     double Di[16];
     inv44(DT, Di);
     mul44(T_last, Di, T_last);
@@ -359,6 +370,18 @@ __device__ inline void post_track_at(const double* DT, const int* good, double* 
   *tracked = ok;
   *gate = ok;
 }
+// Copies that undo SwapFrameBuffer's exchange of the reference and current
+// intensity and inverse-depth pyramids (dst <- src, n floats each), for a
+// frame whose update was NaN: the reference returns before the swap
+// (RGBDTracking.cu:187-191), and the asynchronous pipeline, which swaps
+// before the outcome is known, puts them back on the device.
+struct RefRestore {
+  float* dst[10];
+  const float* src[10];
+  int n[10];
+  int count;
+};
+
 // StereoInitialization (Tracking.cpp:139-155): the first frame keeps T_last.
 __device__ inline void record_first_at(const double* T_last, double* pose_out, int* tracked) {
   for (int i = 0; i < 16; ++i) pose_out[i] = T_last[i];

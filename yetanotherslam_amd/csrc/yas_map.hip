@@ -53,6 +53,7 @@ int yas_ctx_create(int device, void* stream, yas_ctx** out) {
   YAS_HIP(hipSetDevice(device));
   yas_ctx* c = new yas_ctx();
   c->device = device;
+  c->refs.store(1);
   if (stream) {
     c->stream = (hipStream_t)stream;
     c->own_stream = false;
@@ -69,12 +70,25 @@ int yas_ctx_create(int device, void* stream, yas_ctx** out) {
   return YAS_OK;
 }
 
+int yas_ctx_create_null_stream(int device, yas_ctx** out) {
+  YAS_REQUIRE(out != nullptr, "yas_ctx_create_null_stream: out is NULL");
+  int n = 0;
+  YAS_HIP(hipGetDeviceCount(&n));
+  YAS_REQUIRE(device >= 0 && device < n, "yas_ctx_create_null_stream: no such device");
+  YAS_HIP(hipSetDevice(device));
+  yas_ctx* c = new yas_ctx();
+  c->device = device;
+  c->refs.store(1);
+  c->stream = nullptr;  // the device's null stream
+  c->own_stream = false;
+  *out = c;
+  return YAS_OK;
+}
+
+// Drops the creator's reference; the context is freed once the objects
+// built on it are destroyed too.
 int yas_ctx_destroy(yas_ctx* ctx) {
-  if (!ctx) return YAS_OK;
-  hipSetDevice(ctx->device);
-  hipStreamSynchronize(ctx->stream);
-  if (ctx->own_stream) hipStreamDestroy(ctx->stream);
-  delete ctx;
+  yas_ctx_release(ctx);
   return YAS_OK;
 }
 
@@ -87,6 +101,19 @@ int yas_ctx_synchronize(yas_ctx* ctx) {
 void* yas_ctx_stream(yas_ctx* ctx) { return ctx ? (void*)ctx->stream : nullptr; }
 
 }  // extern "C"
+
+yas_ctx* yas_ctx_retain(yas_ctx* c) {
+  if (c) c->refs.fetch_add(1);
+  return c;
+}
+
+void yas_ctx_release(yas_ctx* c) {
+  if (!c || c->refs.fetch_sub(1) != 1) return;
+  hipSetDevice(c->device);
+  hipStreamSynchronize(c->stream);
+  if (c->own_stream) hipStreamDestroy(c->stream);
+  delete c;
+}
 
 // ------------------------------------------------------------------ kernels
 
@@ -275,7 +302,7 @@ int yas_map_create(yas_ctx* ctx, const yas_map_params* P, const float K[4], yas_
               "yas_map_create: max_weight out of range for the voxel format");
   YAS_HIP(hipSetDevice(ctx->device));
   yas_map* m = new yas_map();
-  m->ctx = ctx;
+  m->ctx = yas_ctx_retain(ctx);
   m->P = *P;
   m->K = make_intr(K);
   for (int i = 0; i < 16; ++i) m->pose[i] = (i % 5 == 0) ? 1.0 : 0.0;
@@ -347,6 +374,7 @@ int yas_map_destroy(yas_map* m) {
   if (m->hib_keys) hipHostFree(m->hib_keys);
   if (m->hib_flags) hipHostFree(m->hib_flags);
   if (m->hib_voxels) hipHostFree(m->hib_voxels);
+  yas_ctx_release(m->ctx);
   delete m;
   return YAS_OK;
 }

@@ -635,10 +635,14 @@ int yas_map_load(yas_ctx* ctx, const char* path, yas_map** out) {
   int* fd = nullptr;
   hipStream_t s = ctx->stream;
   hipError_t e = hipMalloc((void**)&fd, sizeof(int) * (size_t)(n > 0 ? n : 1));
+  // The uploads go on the map's stream, after the clearing kernels that
+  // yas_map_create queued there (a copy on the null stream is not ordered
+  // with a non-blocking stream and could be overwritten by them).  The host
+  // vectors stay alive until the synchronisation below.
   if (e == hipSuccess && n > 0) {
-    e = hipMemcpy(m->v.block_keys, keys.data(), sizeof(unsigned long long) * n, hipMemcpyHostToDevice);
-    if (e == hipSuccess) e = hipMemcpy(fd, flags.data(), sizeof(int) * n, hipMemcpyHostToDevice);
-    if (e == hipSuccess) e = hipMemcpy(m->v.voxels, vox.data(), vox.size(), hipMemcpyHostToDevice);
+    e = hipMemcpyAsync(m->v.block_keys, keys.data(), sizeof(unsigned long long) * n, hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(fd, flags.data(), sizeof(int) * n, hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(m->v.voxels, vox.data(), vox.size(), hipMemcpyHostToDevice, s);
   }
   if (e != hipSuccess) {
     hipFree(fd);

@@ -438,7 +438,7 @@ int yas_mesher_create(yas_ctx* ctx, int max_triangles, yas_mesher** out) {
   YAS_HIP(hipSetDevice(ctx->device));
   yas_mesher* me = new yas_mesher();
   std::memset(me, 0, sizeof(*me));
-  me->ctx = ctx;
+  me->ctx = yas_ctx_retain(ctx);
   me->max_tri = max_triangles;
   hipError_t e = hipMalloc((void**)&me->tris, sizeof(float) * 9 * (size_t)max_triangles);
   if (e == hipSuccess) e = hipMalloc((void**)&me->normals, sizeof(float) * 9 * (size_t)max_triangles);
@@ -464,6 +464,7 @@ int yas_mesher_destroy(yas_mesher* me) {
   hipFree(me->total);
   hipFree(me->cases);
   hipFree(me->vinfo);
+  yas_ctx_release(me->ctx);
   delete me;
   return YAS_OK;
 }

@@ -31,15 +31,24 @@ int yas_tracker_set_tracking_image_dev(yas_tracker* t, const float* g, size_t pi
 int yas_tracker_prepare_frame_dev(yas_tracker* t, const float* vmap, size_t vpitch, const float* g, size_t gpitch,
                                   const float* d, size_t dpitch);
 int yas_tracker_set_tracking_depth_dev(yas_tracker* t, const float* d, size_t pitch);
+int yas_tracker_unswap_copies(yas_tracker* t, RefRestore& r);
 int yas_rgb_to_gray_async(yas_ctx* ctx, const uint8_t* rgb, size_t sp, float* dst, size_t dp, int W, int H, int mode);
 
 namespace {
 
-__global__ void k_post_track(const double* DT, const int* good, double* T_last, double* pose_out, int* tracked,
-                             int* gate, const int* ray_visible, int* ray_visible_out) {
-  if (threadIdx.x != 0) return;
-  *ray_visible_out = *ray_visible;
-  post_track_at(DT, good, T_last, pose_out, tracked, gate);
+// The frame's outcome (one thread), and after a NaN update the reference
+// pyramids put back as they were before the host-side swap (all threads).
+__global__ void __launch_bounds__(256) k_post_track(const double* DT, const int* good, double* T_last,
+                                                    double* pose_out, int* tracked, int* gate, const int* ray_visible,
+                                                    int* ray_visible_out, RefRestore rr) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    *ray_visible_out = *ray_visible;
+    post_track_at(DT, good, T_last, pose_out, tracked, gate);
+  }
+  if (*good) return;
+  const int stride = gridDim.x * blockDim.x;
+  for (int c = 0; c < rr.count; ++c)
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < rr.n[c]; i += stride) rr.dst[c][i] = rr.src[c][i];
 }
 
 // Per-frame counters of the map: allocated, visible, flags, walk steps.
@@ -153,7 +162,7 @@ int yas_pipeline_create(yas_ctx* ctx, int W, int H, const float K[4], const yas_
   YAS_REQUIRE(depth_scale > 0.f, "yas_pipeline_create: depth_scale must be > 0");
   YAS_HIP(hipSetDevice(ctx->device));
   yas_pipeline* p = new yas_pipeline();
-  p->ctx = ctx;
+  p->ctx = yas_ctx_retain(ctx);
   p->timing = false;
   std::memset(p->stage_ms, 0, sizeof(p->stage_ms));
   p->map = nullptr; p->rc = nullptr; p->trk = nullptr;
@@ -209,6 +218,7 @@ int yas_pipeline_destroy(yas_pipeline* p) {
   for (auto* v : {&p->pending, &p->spare})
     for (auto& ev : *v)
       for (hipEvent_t e : ev) hipEventDestroy(e);
+  yas_ctx_release(p->ctx);
   delete p;
   return YAS_OK;
 }
@@ -265,9 +275,12 @@ int yas_pipeline_step(yas_pipeline* p, const uint16_t* depth, size_t dpitch, con
     if ((rc = yas_tracker_init_identity_async(p->trk))) return rc;
     if ((rc = yas_tracker_solve_async(p->trk))) return rc;
     yas_tracker_swap_frame_buffer(p->trk);
-    hipLaunchKernelGGL(k_post_track, dim3(1), dim3(64), 0, s, yas_tracker_result_dev(p->trk), yas_tracker_good_dev(p->trk),
-                       p->T_last, pose_out, tracked_out, p->gate, (const int*)yas_map_view(p->map).ctr + kCtrVisible,
-                       p->frame_stats + 8 * (size_t)p->frames + 4);
+    RefRestore rr;
+    yas_tracker_unswap_copies(p->trk, rr);
+    hipLaunchKernelGGL(k_post_track, dim3(32), dim3(256), 0, s, yas_tracker_result_dev(p->trk),
+                       yas_tracker_good_dev(p->trk), p->T_last, pose_out, tracked_out, p->gate,
+                       (const int*)yas_map_view(p->map).ctr + kCtrVisible, p->frame_stats + 8 * (size_t)p->frames + 4,
+                       rr);
     YAS_LAUNCH_CHECK();
     if ((rc = yas_map_prepare_pose_dev(p->map, p->T_last))) return rc;
     gate = p->gate;

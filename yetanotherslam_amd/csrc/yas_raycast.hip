@@ -107,7 +107,7 @@ int yas_raycaster_create(yas_ctx* ctx, int W, int H, const float K[4], yas_rayca
   YAS_REQUIRE(W % 8 == 0 && H % 8 == 0, "yas_raycaster_create: width and height must be multiples of 8");
   YAS_HIP(hipSetDevice(ctx->device));
   yas_raycaster* rc = new yas_raycaster();
-  rc->ctx = ctx;
+  rc->ctx = yas_ctx_retain(ctx);
   rc->W = W;
   rc->H = H;
   rc->cols = W / 8;
@@ -136,6 +136,7 @@ int yas_raycaster_destroy(yas_raycaster* rc) {
   hipFree(rc->zmin);
   hipFree(rc->zmax);
   if (rc->host_ctr) hipHostFree(rc->host_ctr);
+  yas_ctx_release(rc->ctx);
   delete rc;
   return YAS_OK;
 }

@@ -231,9 +231,13 @@ __global__ void __launch_bounds__(256) k_s_frame_level_nb(const StreamDev* strea
 }
 // After tracking: the current frame's intensity and inverse depth become the
 // reference of the next frame (copies; the single pipeline swaps pointers).
+// The current frame's intensity and inverse-depth pyramids become the
+// reference ones (SwapFrameBuffer).  After a NaN update the reference keeps
+// its buffers (RGBDTracking.cu:187-191 returns before the swap).
 __global__ void __launch_bounds__(256) k_s_keep(const StreamDev* streams, const Table* tab, int l) {
   SLOT_OR_RETURN;
   if (l >= S.levels) return;
+  if (!sl.first && !S.st->good) return;
   const Level& a = S.lv[l];
   const TrackBufs& B = S.tb[l];
   const int x = blockIdx.x * 64 + (threadIdx.x & 63), y = blockIdx.y * 4 + (threadIdx.x >> 6);

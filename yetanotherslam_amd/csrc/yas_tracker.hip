@@ -435,7 +435,7 @@ int yas_tracker_create(yas_ctx* ctx, int W, int H, const double K[4], const yas_
   YAS_HIP(hipSetDevice(ctx->device));
   yas_tracker* t = new yas_tracker();
   std::memset(t, 0, sizeof(*t));
-  t->ctx = ctx;
+  t->ctx = yas_ctx_retain(ctx);
   t->W = W;
   t->H = H;
   t->P = *P;
@@ -516,6 +516,7 @@ int yas_tracker_destroy(yas_tracker* t) {
   hipFree(t->st);
   hipFree(t->st2);
   if (t->host_st) hipHostFree(t->host_st);
+  yas_ctx_release(t->ctx);
   delete t;
   return YAS_OK;
 }
@@ -544,6 +545,22 @@ int yas_tracker_set_reference_model(yas_tracker* t, const float* vmap, size_t pi
   return model_pyramid(t);
 }
 
+}  // extern "C"
+
+// After yas_tracker_swap_frame_buffer: the copies that restore the previous
+// reference intensity and inverse depth (the buffers now named current).
+int yas_tracker_unswap_copies(yas_tracker* t, RefRestore& r) {
+  r.count = 0;
+  for (int l = 0; l < t->P.levels; ++l) {
+    const int n = t->lv[l].w * t->lv[l].h;
+    r.dst[r.count] = t->ref_int[l]; r.src[r.count] = t->cur_int[l]; r.n[r.count++] = n;
+    r.dst[r.count] = t->ref_inv[l]; r.src[r.count] = t->cur_inv[l]; r.n[r.count++] = n;
+  }
+  return YAS_OK;
+}
+
+extern "C" {
+
 int yas_tracker_swap_frame_buffer(yas_tracker* t) {
   YAS_REQUIRE(t, "null tracker");
   for (int l = 0; l < kMaxLevels; ++l) {
@@ -568,8 +585,9 @@ int yas_tracker_get_transform(yas_tracker* t, const double init[16], int swap, d
   if (rc) return rc;
   YAS_HIP(hipMemcpyAsync(t->host_st, t->st, sizeof(TrackState), hipMemcpyDeviceToHost, t->ctx->stream));
   YAS_HIP(hipStreamSynchronize(t->ctx->stream));
-  if (swap) yas_tracker_swap_frame_buffer(t);
   const TrackState& h = *t->host_st;
+  // RGBDTracking.cu:187-191: a NaN update returns before the buffer swap
+  if (swap && h.good) yas_tracker_swap_frame_buffer(t);
   if (h.good) std::memcpy(out, h.last_ok, sizeof(h.last_ok));
   else
     for (int i = 0; i < 16; ++i) out[i] = (i % 5 == 0) ? 1.0 : 0.0;

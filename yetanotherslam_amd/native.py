@@ -58,6 +58,7 @@ SIGNATURES = {
     "yas_version": (ctypes.c_char_p, []),
     "yas_last_error": (ctypes.c_char_p, []),
     "yas_ctx_create": (I, [I, P, PP]),
+    "yas_ctx_create_null_stream": (I, [I, PP]),
     "yas_ctx_destroy": (I, [P]),
     "yas_ctx_synchronize": (I, [P]),
     "yas_ctx_stream": (P, [P]),

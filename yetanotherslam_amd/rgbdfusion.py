@@ -4,13 +4,25 @@ Class and method names follow RGBDFusion/include (MapStruct, RayTraceEngine,
 MeshEngine, RGBDTracking and the ImageProc free functions) so that tests read
 like calls into the reference.  Device images are torch tensors on the
 context's device (the reference passes cv::cuda::GpuMat); poses are 4x4 numpy
-arrays (the reference passes Sophus::SE3d).  Work runs on torch's current
-stream of that device, so torch ops and library kernels are ordered.
+arrays (the reference passes Sophus::SE3d).  A Context runs the library's work
+on the torch stream that is current when it is made: a stream torch created
+is shared, and torch's default stream (handle 0) is the device's null stream,
+which the library then binds to explicitly.  Torch work and library work are
+therefore ordered on one stream, and torch's caching allocator, which reuses
+a freed block only in that stream's order, cannot hand an input still being
+read to a later copy.
+
+Every native handle is closed in dependency order (servers and batches,
+pipelines, engines, maps, contexts) by an at-exit hook that runs before the
+interpreter starts tearing modules down, so no HIP call is made during
+interpreter finalisation.
 
 PyTorch is plumbing here (device memory and streams); every computation runs
 in libyas.so's HIP kernels, and nothing falls back to the CPU.
 """
+import atexit
 import ctypes
+import weakref
 
 import numpy as np
 import torch
@@ -62,30 +74,79 @@ def _pitch(t):
     return t.stride(0) * t.element_size()
 
 
-class Context:
-    """yas_ctx bound to torch's current stream on `device`."""
+# Live native handles by teardown rank: 0 servers and batches, 1 pipelines,
+# 2 trackers, ray casters and mesh engines, 3 maps, 4 contexts.
+_live = [weakref.WeakSet() for _ in range(5)]
+_exiting = False
+
+
+def close_all():
+    """Close every live handle, dependents before what they depend on."""
+    for rank in _live:
+        for obj in list(rank):
+            try:
+                obj.close()
+            except Exception:
+                pass
+
+
+@atexit.register
+def _close_at_exit():
+    global _exiting
+    close_all()
+    _exiting = True
+
+
+class _Native:
+    """A library handle `self.h`, destroyed by `_DESTROY` on close()."""
+    _DESTROY = None
+    _RANK = 3
+
+    def _own(self, h):
+        self.h = h
+        _live[self._RANK].add(self)
+
+    def close(self):
+        if getattr(self, "h", None):
+            getattr(native.lib(), self._DESTROY)(self.h)
+            self.h = None
+        _live[self._RANK].discard(self)
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        if _exiting:
+            return
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Context(_Native):
+    """yas_ctx on `device`, bound to torch's current stream of that device
+    (the null stream when that is torch's default stream)."""
+    _DESTROY = "yas_ctx_destroy"
+    _RANK = 4
 
     def __init__(self, device=0):
         self.device = device
         with torch.cuda.device(device):
             stream = torch.cuda.current_stream(device).cuda_stream
         h = ctypes.c_void_p()
-        call("yas_ctx_create", device, ctypes.c_void_p(stream), ctypes.byref(h))
-        self.h = h
+        if stream:
+            call("yas_ctx_create", device, ctypes.c_void_p(stream), ctypes.byref(h))
+        else:
+            call("yas_ctx_create_null_stream", device, ctypes.byref(h))
+        self._own(h)
+        self.stream_handle = stream
 
     def synchronize(self):
         call("yas_ctx_synchronize", self.h)
-
-    def close(self):
-        if getattr(self, "h", None):
-            native.lib().yas_ctx_destroy(self.h)
-            self.h = None
-
-    def __del__(self):
-        try:
-            self.close()
-        except Exception:
-            pass
 
 
 _default_ctx = {}
@@ -113,8 +174,10 @@ def decode_voxels(raw, fmt):
     return sdf, wt
 
 
-class MapStruct:
+class MapStruct(_Native):
     """MapStruct (RGBDFusion/include/VoxelMap.h:32-144)."""
+    _DESTROY = "yas_map_destroy"
+    _RANK = 3
 
     def __init__(self, K, num_buckets, num_blocks, voxel_size, trunc_dist, voxel_format=VOXEL_F32_F32,
                  max_weight=255, depth_min=0.1, depth_max=3.0, ctx=None):
@@ -124,19 +187,8 @@ class MapStruct:
         self.K = np.asarray(K, np.float32)
         h = ctypes.c_void_p()
         call("yas_map_create", self.ctx.h, ctypes.byref(p), _ptr(self.K), ctypes.byref(h))
-        self.h = h
+        self._own(h)
         self.voxel_format = voxel_format
-
-    def close(self):
-        if getattr(self, "h", None):
-            native.lib().yas_map_destroy(self.h)
-            self.h = None
-
-    def __del__(self):
-        try:
-            self.close()
-        except Exception:
-            pass
 
     def Reset(self):
         call("yas_map_reset", self.h)
@@ -213,7 +265,8 @@ class MapStruct:
         h = ctypes.c_void_p()
         call("yas_map_load", ctx.h, str(path).encode(), ctypes.byref(h))
         m = cls.__new__(cls)
-        m.ctx, m.h = ctx, h
+        m.ctx = ctx
+        m._own(h)
         m.params = MapParams()
         m.K = np.zeros(4, np.float32)
         call("yas_map_get_params", h, ctypes.byref(m.params), _ptr(m.K))
@@ -221,9 +274,24 @@ class MapStruct:
         return m
 
 
-class Server:
+class _MapView(MapStruct):
+    """A MapStruct that borrows the volume of a pipeline or server stream;
+    closing it never destroys the volume."""
+
+    def __init__(self, h, voxel_format, owner):
+        self.h = h
+        self.voxel_format = voxel_format
+        self._owner = owner
+
+    def close(self):
+        self.h = None
+
+
+class Server(_Native):
     """Many RGB-D streams tracked and fused with continuous batching
     (yas_server_*)."""
+    _DESTROY = "yas_server_destroy"
+    _RANK = 0
 
     def __init__(self, max_width, max_height, max_streams, tracker_params, voxel_format=VOXEL_F32_F32,
                  hbm_budget=200 << 30, ctx=None):
@@ -231,19 +299,8 @@ class Server:
         h = ctypes.c_void_p()
         call("yas_server_create", self.ctx.h, max_width, max_height, max_streams, ctypes.byref(tracker_params),
              voxel_format, hbm_budget, ctypes.byref(h))
-        self.h = h
+        self._own(h)
         self._inputs = {}
-
-    def close(self):
-        if getattr(self, "h", None):
-            native.lib().yas_server_destroy(self.h)
-            self.h = None
-
-    def __del__(self):
-        try:
-            self.close()
-        except Exception:
-            pass
 
     def add_stream(self, W, H, K, map_params, depth_scale=1000.0, gray_mode=0):
         Kf = np.asarray(K, np.float32)
@@ -306,15 +363,13 @@ class Server:
         """A non-owning MapStruct view of the stream's volume."""
         m = ctypes.c_void_p()
         call("yas_server_stream_map", self.h, sid, ctypes.byref(m))
-        view = MapStruct.__new__(MapStruct)
-        view.h = m
-        view.voxel_format = voxel_format
-        view.close = lambda: None
-        return view
+        return _MapView(m, voxel_format, self)
 
 
-class FusionBatch:
+class FusionBatch(_Native):
     """Many maps fused together, one launch per stage (yas_batch_*)."""
+    _DESTROY = "yas_batch_destroy"
+    _RANK = 0
 
     def __init__(self, maps, ctx=None):
         self.ctx = ctx or default_context()
@@ -322,18 +377,7 @@ class FusionBatch:
         arr = (ctypes.c_void_p * len(self.maps))(*[m.h.value for m in self.maps])
         h = ctypes.c_void_p()
         call("yas_batch_create", self.ctx.h, len(self.maps), arr, ctypes.byref(h))
-        self.h = h
-
-    def close(self):
-        if getattr(self, "h", None):
-            native.lib().yas_batch_destroy(self.h)
-            self.h = None
-
-    def __del__(self):
-        try:
-            self.close()
-        except Exception:
-            pass
+        self._own(h)
 
     def plan(self, depth, poses, depth_scale=1000.0):
         """depth[s][i]: uint16/int16 device tensor (H, W) or None (map i idles
@@ -378,8 +422,10 @@ class FusionBatch:
         call("yas_batch_run", self.h, first, self.steps - first if count is None else count)
 
 
-class RayTraceEngine:
+class RayTraceEngine(_Native):
     """RayTraceEngine (RGBDFusion/include/RayTraceEngine.h:11-48)."""
+    _DESTROY = "yas_raycaster_destroy"
+    _RANK = 2
 
     def __init__(self, w, h, K, ctx=None):
         self.ctx = ctx or default_context()
@@ -387,18 +433,7 @@ class RayTraceEngine:
         Kf = np.asarray(K, np.float32)
         h_ = ctypes.c_void_p()
         call("yas_raycaster_create", self.ctx.h, w, h, _ptr(Kf), ctypes.byref(h_))
-        self.h = h_
-
-    def close(self):
-        if getattr(self, "h", None):
-            native.lib().yas_raycaster_destroy(self.h)
-            self.h = None
-
-    def __del__(self):
-        try:
-            self.close()
-        except Exception:
-            pass
+        self._own(h_)
 
     def RayTrace(self, map_struct, T):
         call("yas_raycast", self.h, map_struct.h, _ptr(_f64(T)))
@@ -430,26 +465,17 @@ class RayTraceEngine:
         return n.value
 
 
-class MeshEngine:
+class MeshEngine(_Native):
     """MeshEngine (RGBDFusion/include/MeshEngine.h:8-24)."""
+    _DESTROY = "yas_mesher_destroy"
+    _RANK = 2
 
     def __init__(self, max_triangles, ctx=None):
         self.ctx = ctx or default_context()
         h = ctypes.c_void_p()
         call("yas_mesher_create", self.ctx.h, max_triangles, ctypes.byref(h))
-        self.h = h
+        self._own(h)
         self.max_triangles = max_triangles
-
-    def close(self):
-        if getattr(self, "h", None):
-            native.lib().yas_mesher_destroy(self.h)
-            self.h = None
-
-    def __del__(self):
-        try:
-            self.close()
-        except Exception:
-            pass
 
     def Meshify(self, map_struct, with_cases=False):
         n = ctypes.c_int()
@@ -503,8 +529,10 @@ def canonical_mesh(m):
     return keys[order], m["count"][order], cases, m["tris"][idx], m["normals"][idx]
 
 
-class RGBDTracking:
+class RGBDTracking(_Native):
     """RGBDTracking (RGBDFusion/include/RGBDTracking.h:9-107)."""
+    _DESTROY = "yas_tracker_destroy"
+    _RANK = 2
 
     def __init__(self, w, h, K, levels=3, iters=(10, 5, 3), mode=TRACK_DEPTH, weights=(1.0, 1.0, 100.0, 10.0),
                  ctx=None):
@@ -515,18 +543,7 @@ class RGBDTracking:
         Kd = _f64(K)
         h_ = ctypes.c_void_p()
         call("yas_tracker_create", self.ctx.h, w, h, _ptr(Kd), ctypes.byref(self.params), ctypes.byref(h_))
-        self.h = h_
-
-    def close(self):
-        if getattr(self, "h", None):
-            native.lib().yas_tracker_destroy(self.h)
-            self.h = None
-
-    def __del__(self):
-        try:
-            self.close()
-        except Exception:
-            pass
+        self._own(h_)
 
     def SetReferenceImage(self, gray):
         call("yas_tracker_set_reference_image", self.h, _dev(gray), _pitch(gray))
@@ -571,18 +588,27 @@ class RGBDTracking:
 
     def level(self, which, lvl):
         self.ctx.synchronize()
-        p = ctypes.c_void_p()
-        pitch = ctypes.c_size_t()
-        w, h = ctypes.c_int(), ctypes.c_int()
-        call("yas_tracker_level", self.h, which, lvl, ctypes.byref(p), ctypes.byref(pitch), ctypes.byref(w),
-             ctypes.byref(h))
-        if which < 6:
-            return download(p.value, (h.value, w.value), np.float32)
-        return download(p.value, (h.value, w.value, 4 if which < 10 else 3), np.float32)
+        return _tracker_level(self.h, which, lvl)
 
 
-class DensePipeline:
+def _tracker_level(h_, which, lvl):
+    """Host copy of one tracker buffer (which: 0 current intensity, 1
+    reference intensity, 2/3 gradients, 4/5 current/reference inverse depth,
+    6-9 vertex/normal maps, 10 photometric rows) at pyramid level lvl."""
+    p = ctypes.c_void_p()
+    pitch = ctypes.c_size_t()
+    w, h = ctypes.c_int(), ctypes.c_int()
+    call("yas_tracker_level", h_, which, lvl, ctypes.byref(p), ctypes.byref(pitch), ctypes.byref(w),
+         ctypes.byref(h))
+    if which < 6:
+        return download(p.value, (h.value, w.value), np.float32)
+    return download(p.value, (h.value, w.value, 4 if which < 10 else 3), np.float32)
+
+
+class DensePipeline(_Native):
     """Tracking::TrackRGBD's dense steps on one volume (yas_pipeline_*)."""
+    _DESTROY = "yas_pipeline_destroy"
+    _RANK = 1
 
     def __init__(self, w, h, K, map_params, tracker_params, depth_scale=1000.0, gray_mode=0, ctx=None):
         self.ctx = ctx or default_context()
@@ -593,18 +619,7 @@ class DensePipeline:
         h_ = ctypes.c_void_p()
         call("yas_pipeline_create", self.ctx.h, w, h, _ptr(Kf), ctypes.byref(map_params),
              ctypes.byref(tracker_params), depth_scale, gray_mode, ctypes.byref(h_))
-        self.h = h_
-
-    def close(self):
-        if getattr(self, "h", None):
-            native.lib().yas_pipeline_destroy(self.h)
-            self.h = None
-
-    def __del__(self):
-        try:
-            self.close()
-        except Exception:
-            pass
+        self._own(h_)
 
     def step(self, depth_u16, rgb_u8):
         call("yas_pipeline_step", self.h, _dev(depth_u16), _pitch(depth_u16), _dev(rgb_u8), _pitch(rgb_u8))
@@ -625,15 +640,21 @@ class DensePipeline:
         call("yas_pipeline_map", self.h, ctypes.byref(m))
         return m
 
+    def tracker_level(self, which, lvl):
+        """A buffer of the pipeline's tracker (see RGBDTracking.level)."""
+        t = ctypes.c_void_p()
+        call("yas_pipeline_tracker", self.h, ctypes.byref(t))
+        self.ctx.synchronize()
+        return _tracker_level(t, which, lvl)
+
+    def map_stats(self):
+        with _MapView(self.map_handle(), VOXEL_F32_F32, self) as view:
+            return view.stats()
+
     def export_map_sorted(self, voxel_format):
         """Sorted blocks of the pipeline's volume (see MapStruct.export_sorted)."""
-        view = MapStruct.__new__(MapStruct)
-        view.h = self.map_handle()
-        view.voxel_format = voxel_format
-        try:
-            return MapStruct.export_sorted(view)
-        finally:
-            view.h = None
+        with _MapView(self.map_handle(), voxel_format, self) as view:
+            return view.export_sorted()
 
 
 def map_params(num_buckets, num_blocks, voxel_size, trunc_dist, voxel_format=VOXEL_F32_F32, max_weight=255,

@@ -41,6 +41,9 @@ def lib():
                                          ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
         _lib.yas_synth_pose.argtypes = [ctypes.POINTER(SynthParams), ctypes.c_int, ctypes.c_void_p]
         _lib.yas_synth_spheres.argtypes = [ctypes.POINTER(SynthParams), ctypes.c_void_p]
+        _lib.yas_synth_render.restype = ctypes.c_int
+        _lib.yas_synth_render.argtypes = [ctypes.POINTER(SynthParams), ctypes.c_void_p, ctypes.c_int,
+                                          ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
     return _lib
 
 
@@ -78,6 +81,34 @@ def frame(p, t, clean=False):
     if clean:
         return depth, rgb, T, cd
     return depth, rgb, T
+
+
+def frame_at(p, T_world_cam, t):
+    """The scene of `p` from camera pose T_world_cam (4x4); `t` selects the
+    noise draws.  Returns (depth_u16 HxW, rgb_u8 HxWx3)."""
+    H, W = p.height, p.width
+    depth = np.zeros((H, W), np.uint16)
+    rgb = np.zeros((H, W, 3), np.uint8)
+    T = np.ascontiguousarray(T_world_cam, np.float64)
+    if lib().yas_synth_render(ctypes.byref(p), T.ctypes.data, int(t), depth.ctypes.data, rgb.ctypes.data, None):
+        raise RuntimeError("yas_synth_render failed")
+    return depth, rgb
+
+
+def wobble_pose(p, t, pitch=0.05, lift=0.03, roll=0.0):
+    """The circular path's pose at frame t with the camera pitched by
+    pitch*sin(t/2) rad about its x axis, rolled by roll*t rad about its
+    optical axis and raised by lift*sin(t/3) m: motions between frames no
+    longer share one screw axis, so they do not commute."""
+    T = pose(p, t)
+    a = pitch * np.sin(t / 2.0)
+    Rx = np.array([[1, 0, 0], [0, np.cos(a), -np.sin(a)], [0, np.sin(a), np.cos(a)]])
+    b = roll * t
+    Rz = np.array([[np.cos(b), -np.sin(b), 0], [np.sin(b), np.cos(b), 0], [0, 0, 1]])
+    out = T.copy()
+    out[:3, :3] = T[:3, :3] @ Rx @ Rz
+    out[1, 3] += lift * np.sin(t / 3.0)
+    return out
 
 
 def pose(p, t):
