@@ -137,6 +137,12 @@ int yas_map_fuse_u16(yas_map* map, const uint16_t* depth, size_t pitch, int widt
 /* CheckNumVisibleBlocks(cols, rows, Tcm) (VoxelMap.cu:788-823). Synchronous. */
 int yas_map_check_visible_blocks(yas_map* map, int width, int height, const double T_map_cam[16],
                                  int* count);
+/* Keys (int32 triples) of the blocks the last visibility pass found visible
+ * (fuse, yas_map_check_visible_blocks or a ray cast): the reference's
+ * visibleTable (VoxelMap.h:125), which its engines read directly
+ * (RayTraceEngine.cu:240-246).  At most `capacity`; *count receives the
+ * number visible. Synchronous. */
+int yas_map_visible_blocks(yas_map* map, int32_t* keys, int capacity, int* count);
 /* Counters and capacity flags (GetNumVisibleBlocks, VoxelMap.cu:201-206). Synchronous. */
 int yas_map_get_stats(yas_map* map, yas_map_stats* stats);
 /* Copy every allocated block to host memory in pool order: keys as int32
@@ -225,7 +231,8 @@ int yas_batch_get_config(yas_batch* b, yas_batch_config* config);
  * replay of a captured graph over all its streams, and batches queue back to
  * back on the server's own HIP stream.  A stream's results are identical
  * alone and in any mix.  Streams are admitted while their device buffers fit
- * the HBM budget.  A server is driven from one host thread. */
+ * the HBM budget; others wait and are admitted as memory frees up.  A server
+ * is driven from one host thread. */
 typedef struct yas_server yas_server;
 typedef struct {
   int stream;          /* stream id */
@@ -234,11 +241,15 @@ typedef struct {
   double pose[16];     /* row-major camera -> map pose of the frame */
 } yas_result;
 typedef struct {
-  int streams;
+  int streams;              /* admitted streams (active or hibernated) */
   long long frames_done;
   long long batches_done;
   size_t bytes_in_use;
   size_t bytes_budget;
+  int streams_pending;      /* opened, waiting for HBM to be admitted */
+  int streams_hibernated;   /* volume paged out to host memory */
+  long long hibernations;   /* volumes paged out so far */
+  long long reactivations;  /* volumes paged back in so far */
 } yas_server_stats;
 /* All streams share the tracker parameters and voxel format; images are at
  * most max_width x max_height. */
@@ -257,13 +268,35 @@ int yas_server_set_gn_group(yas_server* sv, int slots);
  * size is first launched, so that no batch waits for a capture.  Call it
  * after the last yas_server_set_gn_group; it returns once they are built. */
 int yas_server_prepare(yas_server* sv);
-/* Admit a stream (its volume, ray caster, tracker and pose chain are
- * allocated now); YAS_E_CAPACITY when its buffers would exceed the budget or
- * all stream slots are taken. Synchronous. */
+/* Admit a stream now (its volume, ray caster, tracker and pose chain are
+ * allocated); YAS_E_CAPACITY when its buffers would exceed the budget (after
+ * any eviction, below) or all stream slots are taken. Synchronous. */
 int yas_server_add_stream(yas_server* sv, int W, int H, const float K[4], const yas_map_params* mp, float depth_scale,
                           int gray_mode, int* stream_id);
+/* Open a stream that is admitted as soon as its buffers fit the budget:
+ * now if they do, otherwise it waits (pending) and is admitted, first come
+ * first served, when streams close or are removed, or when idle streams are
+ * hibernated (yas_server_set_eviction).  Frames may be submitted to a
+ * pending stream; they wait for its admission.  YAS_E_CAPACITY only when all
+ * stream slots are taken. */
+int yas_server_open_stream(yas_server* sv, int W, int H, const float K[4], const yas_map_params* mp,
+                           float depth_scale, int gray_mode, int* stream_id);
+/* Close a stream: it takes no more frames, finishes those submitted (a
+ * pending stream's are dropped) and is then released, freeing its memory
+ * for pending streams.  Never blocks. */
+int yas_server_close_stream(yas_server* sv, int stream_id);
 /* Release a stream once its frames in flight are done (none may be waiting). */
 int yas_server_remove_stream(yas_server* sv, int stream_id);
+/* Eviction: with idle_ms >= 0, a stream that has been idle (no frame waiting
+ * or in flight) for at least idle_ms may be hibernated, its volume paged out
+ * to pinned host memory as MapStruct::Hibernate does (VoxelMap.cu:894-923),
+ * least recently used first, when a pending or re-activating stream needs
+ * the memory; its next frame pages it back in (ReActivate,
+ * VoxelMap.cu:925-954) once it fits.  Results are unchanged by paging.
+ * idle_ms < 0 (the default) never hibernates. */
+int yas_server_set_eviction(yas_server* sv, double idle_ms);
+/* 0 free, 1 pending, 2 active, 3 hibernated. */
+int yas_server_stream_state(yas_server* sv, int stream_id, int* state);
 /* The stream's volume (for meshing or export between batches). */
 int yas_server_stream_map(yas_server* sv, int stream_id, yas_map** map);
 /* The stream's pipeline (its map, ray caster and tracker) for inspection
@@ -278,7 +311,8 @@ int yas_server_submit(yas_server* sv, int stream_id, const uint16_t* depth, size
 int yas_server_pump(yas_server* sv, int* batches_launched);
 /* Results of finished frames, oldest first. Never blocks. */
 int yas_server_poll(yas_server* sv, yas_result* out, int capacity, int* count);
-/* Pump until every submitted frame is finished. */
+/* Pump until every submitted frame is finished; YAS_E_CAPACITY if frames
+ * wait for streams that cannot be admitted. */
 int yas_server_drain(yas_server* sv);
 int yas_server_get_stats(yas_server* sv, yas_server_stats* stats);
 
@@ -292,6 +326,13 @@ int yas_raycast(yas_raycaster* rc, yas_map* map, const double T_map_cam[16]);
 /* GetVMap() (RayTraceEngine.cu:452-454): device alias of the last result,
  * float4 per pixel, overwritten by the next yas_raycast. */
 int yas_raycaster_vmap(yas_raycaster* rc, float** vmap, size_t* pitch);
+/* MapStruct::GetRayTracingResultDepth() (VoxelMap.cu:981-987, called at
+ * Tracking.cpp:622): VMapToDepth (ImageProc.cu:262-285) of the last vertex
+ * map, z where w > 0 and 0 elsewhere, into the ray caster's depth image
+ * (device, float W*H, overwritten by the next call; the reference allocates a
+ * new GpuMat).  depth/pitch may be NULL.  If host != NULL the image is also
+ * copied there (row pitch host_pitch bytes) and the call waits for it. */
+int yas_raycaster_depth(yas_raycaster* rc, float** depth, size_t* pitch, float* host, size_t host_pitch);
 /* The 1/8-resolution [zmin, zmax] maps of the last ray cast (device, float). */
 int yas_raycaster_zrange(yas_raycaster* rc, float** zmin, float** zmax, int* cols, int* rows);
 /* GetNumVisibleBlock() (RayTraceEngine.cu:38-43). Synchronous. */
@@ -353,6 +394,24 @@ int yas_tracker_set_tracking_image(yas_tracker* trk, const float* gray, size_t p
  * float depth in metres, device memory. */
 int yas_tracker_set_reference_depth(yas_tracker* trk, const float* depth, size_t pitch);
 int yas_tracker_set_tracking_depth(yas_tracker* trk, const float* depth, size_t pitch);
+/* The same setters from host images, as the reference's cv::Mat overloads
+ * (RGBDTracking.h:17-21): an 8-bit gray image is converted to float on the
+ * host and uploaded (RGBDTracking.cu:68-71,104-107); float depth in metres is
+ * uploaded into the tracker's raw-depth buffer (mGpuBufferRawDepth,
+ * RGBDTracking.cu:86-87,122-123).  Each call waits for its upload. */
+int yas_tracker_set_reference_image_host(yas_tracker* trk, const uint8_t* gray, size_t pitch);
+int yas_tracker_set_tracking_image_host(yas_tracker* trk, const uint8_t* gray, size_t pitch);
+int yas_tracker_set_reference_depth_host(yas_tracker* trk, const float* depth, size_t pitch);
+int yas_tracker_set_tracking_depth_host(yas_tracker* trk, const float* depth, size_t pitch);
+/* GetReferenceDepth() (RGBDTracking.cu:547-550): the raw-depth buffer, i.e.
+ * the depth most recently uploaded by a host-image setter (device, float;
+ * NULL before the first). */
+int yas_tracker_raw_depth(yas_tracker* trk, float** depth, size_t* pitch);
+/* WriteDebugImages() (RGBDTracking.cu:557-568, called at Tracking.cpp:182):
+ * level-0 current and reference intensity and the x/y gradients as 8-bit
+ * binary PGM files curr_image.pgm, last_image.pgm, gx.pgm, gy.pgm in `dir`
+ * (rounded to nearest, clamped to [0, 255]). Synchronous. */
+int yas_tracker_write_debug_images(yas_tracker* trk, const char* dir);
 /* SetReferenceModel(GpuMat vmap) (RGBDTracking.cu:140-150). */
 int yas_tracker_set_reference_model(yas_tracker* trk, const float* vmap, size_t pitch);
 /* GetTransform(init, bSwapBuffer) (RGBDTracking.cu:152-214). Synchronous. */

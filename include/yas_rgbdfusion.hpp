@@ -9,6 +9,7 @@
 // transpose to row-major (Eigen is column-major).  Errors throw yas::Error.
 #pragma once
 
+#include <cmath>
 #include <cstring>
 #include <stdexcept>
 #include <string>
@@ -35,7 +36,16 @@ struct DeviceImage {
   int cols = 0, rows = 0;
 };
 
-// A rigid transform as a row-major 4x4 double matrix.
+// A host image (the reference's cv::Mat): pointer, row step in bytes, size.
+// Gray images are 8-bit, depth images float metres.
+struct HostImage {
+  const void* data = nullptr;
+  size_t step = 0;
+  int cols = 0, rows = 0;
+};
+
+// A rigid transform as a row-major 4x4 double matrix, with the two
+// Sophus::SE3d operations the reference's callers use (Tracking.cpp:164-186).
 struct Pose {
   double m[16];
   Pose() {
@@ -46,6 +56,24 @@ struct Pose {
     std::memcpy(p.m, row_major, sizeof(p.m));
     return p;
   }
+  // [R t]^-1 = [R^T  -R^T t], each sum left to right.
+  Pose inverse() const {
+    Pose o;
+    for (int i = 0; i < 3; ++i) {
+      for (int j = 0; j < 3; ++j) o.m[4 * i + j] = m[4 * j + i];
+      o.m[4 * i + 3] = -(m[i] * m[3] + m[4 + i] * m[7] + m[8 + i] * m[11]);
+    }
+    return o;
+  }
+  Pose operator*(const Pose& b) const {
+    Pose o;
+    for (int i = 0; i < 4; ++i)
+      for (int j = 0; j < 4; ++j)
+        o.m[4 * i + j] = m[4 * i] * b.m[j] + m[4 * i + 1] * b.m[4 + j] + m[4 * i + 2] * b.m[8 + j] +
+                         m[4 * i + 3] * b.m[12 + j];
+    return o;
+  }
+  double translation_norm() const { return std::sqrt(m[3] * m[3] + m[7] * m[7] + m[11] * m[11]); }
 };
 
 class Context {
@@ -92,6 +120,19 @@ class MapStruct {
     check(yas_map_get_pose(h_, T.m), "yas_map_get_pose");
     return T;
   }
+  // The public pose member the reference's callers read and assign directly
+  // (Tracking.cpp:164 `Tmw = map->mTcw`, :716 `map->mTcw = pKF->GetPose()`):
+  // both go through GetPose / SetPose, so the map keeps one pose.
+  struct PoseField {
+    MapStruct* map;
+    operator Pose() const { return map->GetPose(); }
+    PoseField& operator=(const Pose& T) {
+      map->SetPose(T);
+      return *this;
+    }
+    Pose inverse() const { return map->GetPose().inverse(); }
+  };
+  PoseField mTcw{this};
   // Fuse(GpuMat depth, SE3d Tcm) (VoxelMap.cu:825-884); depth in float metres.
   void Fuse(const DeviceImage& depth, const Pose& Tcm) {
     check(yas_map_fuse(h_, (const float*)depth.data, depth.pitch, depth.cols, depth.rows, Tcm.m), "yas_map_fuse");
@@ -112,6 +153,12 @@ class MapStruct {
   void SetMeshEngine(MeshEngine* e) { me_ = e; }
   inline void RayTrace(const Pose& Tcm);
   inline DeviceImage GetRayTracingResult();
+  // GetRayTracingResultDepth() (VoxelMap.cu:981-987): the last ray cast's z
+  // (0 where invalid), a device image overwritten by the next call; the
+  // second form also copies it into `host` (W*H floats), as Tracking.cpp:622
+  // does with cv::Mat(GpuMat).
+  inline DeviceImage GetRayTracingResultDepth();
+  inline void GetRayTracingResultDepth(std::vector<float>& host);
   // GenerateMesh() (VoxelMap.cu:143-150): host copies of vertices/normals.
   inline void GenerateMesh();
   void DeleteMesh() {
@@ -207,6 +254,17 @@ class RayTraceEngine {
     check(yas_raycaster_num_visible(rc_, &n), "yas_raycaster_num_visible");
     return (unsigned)n;
   }
+  DeviceImage GetDepth(float* host = nullptr) {
+    DeviceImage im;
+    float* p = nullptr;
+    check(yas_raycaster_depth(rc_, &p, &im.pitch, host, sizeof(float) * (size_t)w_), "yas_raycaster_depth");
+    im.data = p;
+    im.cols = w_;
+    im.rows = h_;
+    return im;
+  }
+  int width() const { return w_; }
+  int height() const { return h_; }
 
  private:
   int w_, h_;
@@ -238,12 +296,19 @@ inline void MapStruct::RayTrace(const Pose& Tcm) {
   if (rt_) rt_->RayTrace(this, Tcm);
 }
 inline DeviceImage MapStruct::GetRayTracingResult() { return rt_ ? rt_->GetVMap() : DeviceImage(); }
+inline DeviceImage MapStruct::GetRayTracingResultDepth() { return rt_ ? rt_->GetDepth() : DeviceImage(); }
+inline void MapStruct::GetRayTracingResultDepth(std::vector<float>& host) {
+  if (!rt_) return;
+  host.resize((size_t)rt_->width() * rt_->height());
+  rt_->GetDepth(host.data());
+}
 inline void MapStruct::GenerateMesh() {
   if (me_) me_->Meshify(this);
 }
 
-// RGBDTracking (RGBDTracking.h:9-107).  Images are device memory here (the
-// reference uploads host cv::Mat inside SetTracking*).
+// RGBDTracking (RGBDTracking.h:9-107).  Images may be device memory
+// (DeviceImage) or host memory (HostImage, the reference's cv::Mat: 8-bit
+// gray, float depth), which the setters upload as the reference does.
 class RGBDTracking {
  public:
   RGBDTracking(Context& ctx, int w, int h, const double K[4], bool bRGB, bool bIcp, int levels = 5,
@@ -258,6 +323,8 @@ class RGBDTracking {
     p.w_icp_h = 100.0f;
     p.w_icp_b = 10.0f;
     check(yas_tracker_create(ctx.get(), w, h, K, &p, &t_), "yas_tracker_create");
+    w_ = w;
+    h_ = h;
   }
   ~RGBDTracking() { yas_tracker_destroy(t_); }
   RGBDTracking(const RGBDTracking&) = delete;
@@ -273,6 +340,34 @@ class RGBDTracking {
   }
   void SetTrackingDepth(const DeviceImage& d) {
     check(yas_tracker_set_tracking_depth(t_, (const float*)d.data, d.pitch), "SetTrackingDepth");
+  }
+  void SetReferenceImage(const HostImage& g) {
+    check(yas_tracker_set_reference_image_host(t_, (const uint8_t*)g.data, g.step), "SetReferenceImage");
+  }
+  void SetReferenceDepth(const HostImage& d) {
+    check(yas_tracker_set_reference_depth_host(t_, (const float*)d.data, d.step), "SetReferenceDepth");
+  }
+  void SetTrackingImage(const HostImage& g) {
+    check(yas_tracker_set_tracking_image_host(t_, (const uint8_t*)g.data, g.step), "SetTrackingImage");
+  }
+  void SetTrackingDepth(const HostImage& d) {
+    check(yas_tracker_set_tracking_depth_host(t_, (const float*)d.data, d.step), "SetTrackingDepth");
+  }
+  // GetReferenceDepth(lvl) (RGBDTracking.cu:547-550): the raw depth most
+  // recently uploaded from the host, whatever the level (as the reference).
+  DeviceImage GetReferenceDepth(int /*lvl*/ = 0) const {
+    DeviceImage im;
+    float* p = nullptr;
+    check(yas_tracker_raw_depth(t_, &p, &im.pitch), "GetReferenceDepth");
+    im.data = p;
+    im.cols = w_;
+    im.rows = h_;
+    return im;
+  }
+  // WriteDebugImages() (RGBDTracking.cu:557-568): curr_image, last_image, gx
+  // and gy of level 0 as PGM files in `dir`.
+  void WriteDebugImages(const std::string& dir = ".") {
+    check(yas_tracker_write_debug_images(t_, dir.c_str()), "WriteDebugImages");
   }
   void SetReferenceModel(const DeviceImage& vmap) {
     check(yas_tracker_set_reference_model(t_, (const float*)vmap.data, vmap.pitch), "SetReferenceModel");
@@ -291,6 +386,7 @@ class RGBDTracking {
  private:
   yas_tracker* t_ = nullptr;
   bool good_ = true;
+  int w_ = 0, h_ = 0;
 };
 
 }  // namespace yas

@@ -5,6 +5,9 @@ maps, image pyramids and marching-cubes cases/triangles are bit-exact; the
 tracker's reduced sums are within FP32 reduction-order tolerance of the
 oracle's exact sums, and poses within 1e-4 m / 1e-4 rad.
 """
+import os
+import subprocess
+
 import numpy as np
 import pytest
 import torch
@@ -56,12 +59,17 @@ def test_fuse_c1_full_resolution_bit_exact():
     assert_bits_equal(wo, wg, "weight")
 
 
-def test_visible_block_count_matches():
+def test_visible_block_set_matches():
+    """CheckNumVisibleBlocks (VoxelMap.cu:788-823): the count and the set of
+    visible blocks equal the oracle's (the reference's list order is
+    nondeterministic, so sets are compared)."""
     p, K, fr, gt, om, gm = fuse_both(FMTS[1])
-    for T in (gt[0], gt[2]):
-        n_or = len(om.visible(p.width, p.height, K, T))
+    for T in (gt[0], gt[2], np.linalg.inv(gt[2]) @ gt[0]):
+        ko = om.visible(p.width, p.height, K, T)
         n_gpu = gm.CheckNumVisibleBlocks(p.width, p.height, T)
-        assert n_or == n_gpu
+        kg = gm.visible_blocks()
+        assert len(ko) == n_gpu == len(kg) > 100
+        assert {tuple(k) for k in kg} == {tuple(k) for k in ko}
 
 
 def test_raycast_bit_exact():
@@ -160,6 +168,25 @@ def test_tracker_pyramids_bit_exact():
     for lvl in range(3):
         for which in range(10):
             assert_bits_equal(ot.level(which, lvl), gt_.level(which, lvl), "buffer %d level %d" % (which, lvl))
+
+
+def test_reference_default_five_level_tracker_matches_oracle():
+    """The reference's default tracker (NUM_PYR 5, RGBDTracking.h:7;
+    iterations {10,5,3,3,3}, RGBDTracking.cu:159; weights 1/1/100/10): all
+    five levels of every buffer bit-exact, and the pose within 1e-4 m /
+    1e-4 rad of the oracle's, in each mode."""
+    for mode in (po.MODE_RGBD, po.MODE_DEPTH, po.MODE_RGB):
+        p, K, ot, gt_, DT = tracker_pair(mode, levels=5, iters=(10, 5, 3, 3, 3))
+        if mode == po.MODE_RGBD:
+            for lvl in range(5):
+                for which in range(10):
+                    assert_bits_equal(ot.level(which, lvl), gt_.level(which, lvl), "buffer %d level %d" % (which, lvl))
+        To, good_o = ot.get_transform(swap=False)
+        Tg, st = gt_.GetTransform(swap=False)
+        assert good_o and st.good
+        et, er = pose_err(Tg, To)
+        assert et < 1e-4 and er < 1e-4, (mode, et, er)
+        gt_.close()
 
 
 @pytest.mark.parametrize("kind", [0, 1], ids=["icp", "photometric"])
@@ -295,16 +322,76 @@ def test_empty_depth_frame_allocates_nothing_and_tracks_identity():
     np.testing.assert_array_equal(T, np.eye(4))
 
 
+FACADE_EXE = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "tests", "cpp",
+                          "facade_replay")
+
+
 def test_cpp_facade_replays_track_rgbd():
-    """The reference-shaped C++ facade, driven like Tracking::TrackRGBD on a
-    plane: tracks identity, fuses, meshes (built by __graft_entry__.build())."""
-    import os
-    import subprocess
-    exe = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "tests", "cpp", "facade_replay")
-    assert os.path.exists(exe), "run __graft_entry__.build() first"
-    r = subprocess.run([exe], capture_output=True, text=True, timeout=120)
+    """The reference-shaped C++ facade on a wavy surface: tracks identity,
+    fuses, merges a submap, meshes (built by __graft_entry__.build())."""
+    assert os.path.exists(FACADE_EXE), "run __graft_entry__.build() first"
+    r = subprocess.run([FACADE_EXE], capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stdout + r.stderr
     assert "tracked=1" in r.stdout
+
+
+def test_cpp_facade_sequence_equals_the_pipeline(tmp_path):
+    """The facade driven exactly as Tracking.cpp:139-197 drives RGBDFusion:
+    host images (8-bit gray as System.cpp:47-49 makes it, float depth), the
+    reference's default 5-level pyramid with {10,5,3,3,3} iterations and its
+    1/1/100/10 weights, reference int16 + uint8 storage, the map pose through
+    mTcw, rejected frames written out by WriteDebugImages.  Ten frames of the
+    room at 320x240 with a 15 cm jump at frame 6.  Every pose and decision
+    equals the C-ABI pipeline's on the same frames, bit for bit;
+    GetRayTracingResultDepth equals the ray cast's z; GetReferenceDepth
+    returns the depth just uploaded."""
+    W, H, n = 320, 240, 10
+    p = synth.params(width=W, height=H, theta0=np.pi / 4)
+    K = synth.intrinsics(p)
+    frames_ = [synth.frame(p, t)[:2] for t in range(6)]
+    Tj = synth.pose(p, 5).copy()
+    Tj[:3, 3] += 0.15 * Tj[:3, 0]
+    frames_.append(synth.frame_at(p, Tj, 6))
+    frames_ += [synth.frame(p, t)[:2] for t in range(6, 9)]
+    path = tmp_path / "frames.bin"
+    with open(path, "wb") as f:
+        f.write(np.array([W, H, n], np.int32).tobytes())
+        f.write(np.asarray(K, np.float64).tobytes())
+        for d, c in frames_:
+            f.write(po.rgb_to_intensity(c, mode=1).astype(np.uint8).tobytes())
+            f.write(po.depth_u16_to_f32(d).tobytes())
+    r = subprocess.run([FACADE_EXE, str(path), "5", str(tmp_path)], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stdout + r.stderr
+    rows = [l.split() for l in r.stdout.splitlines() if l.startswith("frame ")]
+    info = dict(l.split() for l in r.stdout.splitlines() if not l.startswith("frame "))
+    assert len(rows) == n
+    mp = rf.map_params(1 << 18, 1 << 15, 0.01, 0.03, rf.VOXEL_I16_U8)
+    tp = rf.tracker_params(5, (10, 5, 3, 3, 3), rf.TRACK_RGBD, (1.0, 1.0, 100.0, 10.0))
+    pl = rf.DensePipeline(W, H, K, mp, tp, gray_mode=1)
+    for d, c in frames_:
+        pl.step(to_dev(d), to_dev(c))
+    for t, row in enumerate(rows):
+        Tf = np.array([float(v) for v in row[3:]]).reshape(4, 4)
+        Tp, okp = pl.pose(t)
+        assert int(row[2]) == int(okp), (t, row[2], okp)
+        np.testing.assert_array_equal(Tf, Tp, err_msg="frame %d" % t)
+    assert [int(r_[2]) for r_ in rows][6] == 0  # the jump
+    for name in ("curr_image.pgm", "last_image.pgm", "gx.pgm", "gy.pgm"):
+        assert (tmp_path / name).stat().st_size == len("P5\n%d %d\n255\n" % (W, H)) + W * H
+    assert float(info["depth_max_abs_diff"]) == 0.0
+    assert info["raw_depth_ok"] == "1" and int(info["triangles"]) > 10000
+    pl.close()
+    # the same sequence through the oracle's pipeline (float32): same
+    # decisions (frames 6 and 7 rejected: after the jump the reference
+    # intensity is the jump frame's), poses within 1e-4 m / 1e-4 rad
+    op = po.OraclePipeline(W, H, K, 0.01, 0.03, 5, (10, 5, 3, 3, 3), po.MODE_RGBD, (1, 1, 100, 10),
+                           voxfmt=po.VOX_REF, gray_mode=1)
+    for t, (d, c) in enumerate(frames_):
+        To, ok, _ = op.step(d, c)
+        Tf = np.array([float(v) for v in rows[t][3:]]).reshape(4, 4)
+        assert int(rows[t][2]) == int(ok), t
+        et, er = pose_err(Tf, To)
+        assert et <= 1e-4 and er <= 1e-4, (t, et, er)
 
 
 def test_plain_method_sequence_against_oracle():

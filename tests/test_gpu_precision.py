@@ -6,9 +6,14 @@ is float32; J^T J sums are pairwise (short per-thread runs, butterfly trees
 over lanes and waves, a fixed-order fold of the workgroup rows); the 6x6
 solve and the SE(3) update are float64.
 
+Sizes are the benchmark's: 2^22 hash buckets and a pool of 2^19 blocks.
+
 Bounds: TSDF RMS <= 0.1 voxel, vertex map p99 <= half a voxel, tracker
-arithmetic <= 1e-4 m / 1e-4 rad, whole-pipeline motion per frame <= 3e-4 m
-/ 3e-4 rad.  The measured errors are printed (run with -s to see them)."""
+arithmetic <= 1e-4 m / 1e-4 rad.  The whole pipeline over 30 frames: with
+float16 sdf + uint16 weight storage every frame's motion and every chained
+pose is within 1e-4 m / 1e-4 rad of FP64; with float32 storage that bound is
+missed on a few frames (see the test).  The measured errors are printed (run
+with -s to see them)."""
 import json
 import math
 
@@ -28,13 +33,15 @@ W, H = 640, 480
 VOX, MU = 0.005, 0.02
 WEIGHTS = (0.1, 0.1, 1.0, 1.0)
 FMTS = {"f32": rf.VOXEL_F32_F32, "f16u16": rf.VOXEL_F16_U16}
-N = 6
+BUCKETS, BLOCKS = 1 << 22, 1 << 19
+N = 6            # frames fused at the true poses (maps, tracker)
+N_PIPE = 30      # frames of the whole dense step
 
 
 @pytest.fixture(scope="module")
 def seq():
-    p, K, fr = frames(N, W, H)
-    gt = [synth.pose(p, t) for t in range(N)]
+    p, K, fr = frames(N_PIPE, W, H)
+    gt = [synth.pose(p, t) for t in range(N_PIPE)]
     gt = [np.linalg.inv(gt[0]) @ T for T in gt]
     z = [po.depth_u16_to_f32(f[0]) for f in fr]
     g = [po.rgb_to_intensity(f[1]) for f in fr]
@@ -48,7 +55,7 @@ def report(name, d):
 def test_tsdf_and_vertex_map_errors_against_fp64(seq):
     p, K, fr, gt, z, _ = seq
     o64 = po.OracleMap(VOX, MU, precision=64)
-    gm = {k: rf.MapStruct(K.astype(np.float32), 1 << 20, 1 << 17, VOX, MU, voxel_format=f) for k, f in FMTS.items()}
+    gm = {k: rf.MapStruct(K.astype(np.float32), BUCKETS, BLOCKS, VOX, MU, voxel_format=f) for k, f in FMTS.items()}
     for t in range(N):
         o64.fuse(z[t], K, gt[t])
         for m in gm.values():
@@ -56,7 +63,7 @@ def test_tsdf_and_vertex_map_errors_against_fp64(seq):
     torch.cuda.synchronize()
     k64, s64, w64 = o64.export()
     idx = {tuple(k): i for i, k in enumerate(k64)}
-    v64, _, _ = o64.raycast(W, H, K, gt[-1])
+    v64, _, _ = o64.raycast(W, H, K, gt[N - 1])
     rc = rf.RayTraceEngine(W, H, K)
     for name, m in gm.items():
         k, s, w = m.export_sorted()
@@ -66,7 +73,7 @@ def test_tsdf_and_vertex_map_errors_against_fp64(seq):
         both = (w[a] > 0) & (w64[b] > 0)
         err = (s[a][both] - s64[b][both]) * (MU / VOX)
         rms = float(np.sqrt(np.mean(err ** 2)))
-        rc.RayTrace(m, gt[-1])
+        rc.RayTrace(m, gt[N - 1])
         v = rc.GetVMap()
         ok = (v[..., 3] > 0) & (v64[..., 3] > 0)
         dist = np.linalg.norm(v[..., :3][ok].astype(np.float64) - v64[..., :3][ok], axis=1)
@@ -123,39 +130,64 @@ def test_tracker_pose_and_sums_errors_against_fp64(seq):
     assert worst["t_m"] <= 1e-4 and worst["r_rad"] <= 1e-4, worst
 
 
+@pytest.fixture(scope="module")
+def fp64_pipeline(seq):
+    """The FP64 oracle's poses and decisions over the sequence."""
+    p, K, fr, _, _, _ = seq
+    op = po.OraclePipeline(W, H, K, VOX, MU, 3, (10, 5, 3), po.MODE_RGBD, WEIGHTS, precision=64)
+    return [op.step(d, rgb)[:2] for d, rgb, _ in fr]
+
+
 @pytest.mark.parametrize("fmt", list(FMTS), ids=list(FMTS))
-def test_pipeline_poses_against_fp64(seq, fmt):
-    """The whole dense step (ray cast, track, fuse) over the sequence: GPU
-    with float32 math and the given storage against the FP64 pipeline.  Each
-    frame's tracked motion (T_{t-1}^-1 T_t) is compared; the chained poses
-    accumulate those differences and are reported as drift."""
+def test_pipeline_poses_against_fp64(seq, fp64_pipeline, fmt):
+    """The whole dense step (ray cast, track, fuse) over 30 frames at the
+    benchmark's configuration: GPU with float32 math and the given storage
+    against the FP64 pipeline.  Each frame's tracked motion
+    (T_{t-1}^-1 T_t) and each chained pose is compared.
+
+    float16 sdf + uint16 weight (the precision form): within 1e-4 m /
+    1e-4 rad on every frame (measured: 1.8e-5 m, 6.5e-6 rad per frame;
+    1.5e-5 m chained).
+
+    float32 sdf + float32 weight: a known miss of the 1e-4 m bound on
+    frames 1 and 3 (1.3e-4 and 1.6e-4 m; 6e-5 rad).  The float32 map
+    differs from the float64 one by ~0.014 voxel RMS, and the ray-cast
+    model by a few micrometres except at depth discontinuities, where a few
+    pixels move by millimetres and flip ICP associations at the 1 cm gate;
+    the tracker's own arithmetic on identical inputs is within 1e-7 m (the
+    test above).  The f32 bound is therefore 3e-4 m per frame with at most
+    three frames above 1e-4 m, stated here rather than loosening the bound
+    of the precision form."""
     p, K, fr, gt, _, _ = seq
-    mp = rf.map_params(1 << 20, 1 << 17, VOX, MU, FMTS[fmt])
+    mp = rf.map_params(BUCKETS, BLOCKS, VOX, MU, FMTS[fmt])
     tp = rf.tracker_params(3, (10, 5, 3), rf.TRACK_RGBD, WEIGHTS)
     pl = rf.DensePipeline(W, H, K, mp, tp)
-    op = po.OraclePipeline(W, H, K, VOX, MU, 3, (10, 5, 3), po.MODE_RGBD, WEIGHTS, precision=64)
-    step_t = step_r = drift_t = drift_r = truth_t = 0.0
-    prev = None
-    for t in range(N):
-        d, rgb, _ = fr[t]
+    for d, rgb, _ in fr:
         pl.step(to_dev(d), to_dev(rgb))
-        To, ok_o, _ = op.step(d, rgb)
+    motion, chained, over = [], [], []
+    truth_t = 0.0
+    for t in range(N_PIPE):
         Tg, ok_g = pl.pose(t)
-        assert ok_o and ok_g
-        if prev is not None:
-            et, er = pose_err(np.linalg.inv(prev[0]) @ Tg, np.linalg.inv(prev[1]) @ To)
-            step_t, step_r = max(step_t, et), max(step_r, er)
-        et, er = pose_err(Tg, To)
-        drift_t, drift_r = max(drift_t, et), max(drift_r, er)
+        To, ok_o = fp64_pipeline[t]
+        assert ok_o and ok_g, t
+        if t > 0:
+            Tgp, Top = pl.pose(t - 1)[0], fp64_pipeline[t - 1][0]
+            e = pose_err(np.linalg.inv(Tgp) @ Tg, np.linalg.inv(Top) @ To)
+            motion.append(e)
+            if e[0] > 1e-4 or e[1] > 1e-4:
+                over.append((t, e))
+        chained.append(pose_err(Tg, To))
         truth_t = max(truth_t, pose_err(Tg, gt[t])[0])
-        prev = (Tg, To)
-    out = {"frames": N, "motion_t_m": step_t, "motion_r_rad": step_r, "chained_t_m": drift_t,
-           "chained_r_rad": drift_r, "gpu_vs_truth_t_m": truth_t}
+    pl.close()
+    out = {"frames": N_PIPE, "motion_t_m": max(e[0] for e in motion), "motion_r_rad": max(e[1] for e in motion),
+           "chained_t_m": max(e[0] for e in chained), "chained_r_rad": max(e[1] for e in chained),
+           "frames_over_1e-4": [(t, round(e[0], 7), round(e[1], 7)) for t, e in over],
+           "gpu_vs_truth_t_m": truth_t}
     report("pipeline_" + fmt, out)
-    # The float32 map differs from the float64 one by ~0.014 voxel RMS, and
-    # the ray-cast model by a few micrometres except at depth
-    # discontinuities, where a few pixels move by millimetres and flip ICP
-    # associations: the tracked motion then differs by ~1.6e-4 m on this
-    # sequence (the tracker's own arithmetic, above, by ~1e-7 m).
-    assert step_t <= 3e-4 and step_r <= 3e-4, out
-    assert drift_t <= 3e-4 * (N - 1) and drift_r <= 3e-4 * (N - 1), out
+    if fmt == "f16u16":
+        assert not over, out
+        assert out["chained_t_m"] <= 1e-4 and out["chained_r_rad"] <= 1e-4, out
+    else:
+        assert len(over) <= 3, out
+        assert out["motion_t_m"] <= 3e-4 and out["motion_r_rad"] <= 1e-4, out
+        assert out["chained_t_m"] <= 1e-3 and out["chained_r_rad"] <= 3e-4, out

@@ -144,3 +144,140 @@ def test_frames_arriving_during_a_batch_join_later_batches():
         poses, _ = run_single(s[0], s[1], s[2])
         for t in range(n):
             np.testing.assert_array_equal(by[sid][t]["pose"], poses[t][0])
+
+
+def results_by_stream(res):
+    out = {}
+    for r in res:
+        out.setdefault(r["stream"], []).append(r)
+    for v in out.values():
+        v.sort(key=lambda r: r["request"])
+    return out
+
+
+def check_against_single(rs, s, n=None):
+    """Results rs of a stream equal its own single pipeline's first n frames."""
+    n = len(rs) if n is None else n
+    poses, m = run_single(s[0], s[1], s[2][:n])
+    assert len(rs) == n
+    for t in range(n):
+        assert rs[t]["tracked"] == poses[t][1]
+        np.testing.assert_array_equal(rs[t]["pose"], poses[t][0])
+    return m
+
+
+def test_closing_a_stream_mid_run_leaves_the_others_identical():
+    """One of three streams is closed after its third frame while the others
+    keep going: the closed stream's three results and the others' five equal
+    each stream run alone in a single pipeline, volumes included."""
+    n = 5
+    a = stream_frames(320, 240, 0.4, n)
+    b = stream_frames(160, 120, 1.3, n)
+    c = stream_frames(320, 240, 2.2, n)
+    sv = rf.Server(320, 240, 4, tracker_params())
+    ids = [sv.open_stream(s[0].width, s[0].height, s[1], map_params()) for s in (a, b, c)]
+    sv.prepare()
+    res = []
+    for t in range(n):
+        for sid, s in zip(ids, (a, b, c)):
+            if sid == ids[1] and t >= 3:
+                continue
+            sv.submit(sid, *s[2][t])
+        if t == 2:
+            sv.close_stream(ids[1])  # its three frames still run
+        sv.pump()
+        res += sv.poll()
+    sv.drain()
+    res += sv.poll()
+    by = results_by_stream(res)
+    assert sv.stream_state(ids[1]) == "free"
+    check_against_single(by[ids[1]], b, 3)
+    for sid, s in ((ids[0], a), (ids[2], c)):
+        m = check_against_single(by[sid], s)
+        for arr, ref in zip(sv.stream_map(sid).export_sorted(), m):
+            np.testing.assert_array_equal(arr, ref)
+    sv.close()
+
+
+def stream_budget(K):
+    probe = rf.Server(160, 120, 4, tracker_params(), hbm_budget=1 << 40)
+    probe.add_stream(160, 120, K, map_params())
+    one = probe.stats()["bytes_in_use"]
+    probe.close()
+    return one
+
+
+def test_streams_over_budget_wait_and_are_admitted_as_memory_frees():
+    """Budget for two streams, three opened: the third waits (pending) with
+    its frames queued, is admitted when the first closes, and its results
+    equal its own single pipeline's, as do the others'."""
+    n = 4
+    s3 = [stream_frames(160, 120, th, n) for th in (0.3, 1.5, 2.7)]
+    one = stream_budget(s3[0][1])
+    sv = rf.Server(160, 120, 4, tracker_params(), hbm_budget=int(2.5 * one))
+    ids = [sv.open_stream(160, 120, s[1], map_params()) for s in s3]
+    assert [sv.stream_state(i) for i in ids] == ["active", "active", "pending"]
+    assert sv.stats()["streams_pending"] == 1
+    for t in range(n):
+        for sid, s in zip(ids, s3):
+            sv.submit(sid, *s[2][t])
+    res = []
+    for _ in range(3):
+        sv.pump()
+        res += sv.poll()
+    assert sv.stream_state(ids[2]) == "pending"
+    sv.close_stream(ids[0])  # its frames finish, then its memory goes to the third stream
+    sv.drain()
+    res += sv.poll()
+    assert sv.stream_state(ids[0]) == "free" and sv.stream_state(ids[2]) == "active"
+    # results come back in completion order: the first stream's last frame
+    # finished before the third stream's first
+    order = [r["stream"] for r in res]
+    assert max(i for i, s in enumerate(order) if s == ids[0]) < order.index(ids[2])
+    by = results_by_stream(res)
+    for sid, s in zip(ids, s3):
+        check_against_single(by[sid], s)
+    sv.close()
+
+
+def test_idle_streams_are_hibernated_to_admit_others_and_resume_exactly():
+    """Budget for two streams with eviction on: opening a third hibernates
+    the least recently used idle stream (its volume paged out to host
+    memory); that stream's next frame pages it back in, hibernating the other
+    idle one.  Every stream's results equal its own single pipeline's, so
+    paging changes nothing."""
+    n = 4
+    s3 = [stream_frames(160, 120, th, n) for th in (0.6, 1.8, 3.0)]
+    one = stream_budget(s3[0][1])
+    sv = rf.Server(160, 120, 4, tracker_params(), hbm_budget=int(2.5 * one))
+    sv.set_eviction(0.0)
+    a, b = (sv.open_stream(160, 120, s[1], map_params()) for s in s3[:2])
+    res = []
+    for t in range(2):
+        sv.submit(a, *s3[0][2][t])
+        sv.submit(b, *s3[1][2][t])
+        sv.drain()
+        res += sv.poll()
+    c = sv.open_stream(160, 120, s3[2][1], map_params())
+    st = sv.stats()
+    assert sv.stream_state(c) == "active" and st["hibernations"] == 1 and st["streams_hibernated"] == 1
+    hib = a if sv.stream_state(a) == "hibernated" else b
+    for t in range(n):
+        sv.submit(c, *s3[2][2][t])
+    sv.drain()
+    res += sv.poll()
+    for t in range(2, n):
+        sv.submit(a, *s3[0][2][t])
+        sv.submit(b, *s3[1][2][t])
+    sv.drain()
+    res += sv.poll()
+    st = sv.stats()
+    assert st["reactivations"] >= 1 and st["hibernations"] >= 2
+    by = results_by_stream(res)
+    for sid, s in zip((a, b, c), s3):
+        m = check_against_single(by[sid], s)
+        if sv.stream_state(sid) == "active":
+            for arr, ref in zip(sv.stream_map(sid).export_sorted(), m):
+                np.testing.assert_array_equal(arr, ref)
+    print("hibernated first:", hib, "stats:", st)
+    sv.close()

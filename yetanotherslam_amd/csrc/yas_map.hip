@@ -436,6 +436,27 @@ int yas_map_check_visible_blocks(yas_map* m, int W, int H, const double T[16], i
   return YAS_OK;
 }
 
+// The blocks of the last visibility pass (the reference's visibleTable,
+// VoxelMap.h:125, which its engines read directly): keys in list order.
+int yas_map_visible_blocks(yas_map* m, int32_t* keys, int capacity, int* count) {
+  YAS_REQUIRE(m && count && (keys || capacity == 0), "yas_map_visible_blocks: null argument");
+  YAS_REQUIRE_ACTIVE(m);
+  YAS_HIP(hipSetDevice(m->ctx->device));
+  YAS_HIP(hipMemcpyAsync(m->host_ctr, m->v.ctr, sizeof(int) * kCtrCount, hipMemcpyDeviceToHost, m->ctx->stream));
+  YAS_HIP(hipStreamSynchronize(m->ctx->stream));
+  const int nv = m->host_ctr[kCtrVisible];
+  *count = nv;
+  const int n = nv < capacity ? nv : capacity;
+  if (n > 0) {
+    std::vector<int> idx(n);
+    std::vector<unsigned long long> k((size_t)m->P.num_blocks);
+    YAS_HIP(hipMemcpy(idx.data(), m->v.visible, sizeof(int) * n, hipMemcpyDeviceToHost));
+    YAS_HIP(hipMemcpy(k.data(), m->v.block_keys, sizeof(unsigned long long) * m->P.num_blocks, hipMemcpyDeviceToHost));
+    for (int i = 0; i < n; ++i) unpack_key(k[idx[i]], keys[3 * i], keys[3 * i + 1], keys[3 * i + 2]);
+  }
+  return nv > capacity ? YAS_E_CAPACITY : YAS_OK;
+}
+
 int yas_map_get_stats(yas_map* m, yas_map_stats* st) {
   YAS_REQUIRE(m && st, "null argument");
   YAS_HIP(hipSetDevice(m->ctx->device));

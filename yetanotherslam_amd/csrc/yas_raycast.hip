@@ -25,6 +25,7 @@ struct yas_raycaster {
   unsigned* zmax;
   int* host_ctr;
   yas_map* last_map;
+  float* depth;  // VMapToDepth of the last vertex map (lazily allocated)
 };
 
 __global__ void __launch_bounds__(256) k_zrange_init(unsigned* zmin, unsigned* zmax, int n) {
@@ -135,6 +136,7 @@ int yas_raycaster_destroy(yas_raycaster* rc) {
   hipFree(rc->vmap);
   hipFree(rc->zmin);
   hipFree(rc->zmax);
+  hipFree(rc->depth);
   if (rc->host_ctr) hipHostFree(rc->host_ctr);
   yas_ctx_release(rc->ctx);
   delete rc;
@@ -155,6 +157,28 @@ int yas_raycaster_vmap(yas_raycaster* rc, float** vmap, size_t* pitch) {
   YAS_REQUIRE(rc && vmap, "null argument");
   *vmap = (float*)rc->vmap;
   if (pitch) *pitch = sizeof(float4) * (size_t)rc->W;
+  return YAS_OK;
+}
+
+// GetRayTracingResultDepth (VoxelMap.cu:981-987): VMapToDepth
+// (ImageProc.cu:262-285) of the last vertex map into the ray caster's depth
+// image (overwritten by the next call; the reference returns a new GpuMat).
+// With a host buffer the image is also copied there and the call waits.
+int yas_raycaster_depth(yas_raycaster* rc, float** depth, size_t* pitch, float* host, size_t host_pitch) {
+  YAS_REQUIRE(rc, "yas_raycaster_depth: null ray caster");
+  YAS_REQUIRE(!host || host_pitch >= sizeof(float) * (size_t)rc->W, "yas_raycaster_depth: host pitch too small");
+  YAS_HIP(hipSetDevice(rc->ctx->device));
+  const size_t row = sizeof(float) * (size_t)rc->W;
+  if (!rc->depth) YAS_HIP(hipMalloc((void**)&rc->depth, row * rc->H));
+  int r = yas_vmap_to_depth(rc->ctx, (const float*)rc->vmap, sizeof(float4) * (size_t)rc->W, rc->depth, row, rc->W,
+                            rc->H);
+  if (r) return r;
+  if (host) {
+    YAS_HIP(hipMemcpy2DAsync(host, host_pitch, rc->depth, row, row, rc->H, hipMemcpyDeviceToHost, rc->ctx->stream));
+    YAS_HIP(hipStreamSynchronize(rc->ctx->stream));
+  }
+  if (depth) *depth = rc->depth;
+  if (pitch) *pitch = row;
   return YAS_OK;
 }
 

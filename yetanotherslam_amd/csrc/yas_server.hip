@@ -334,13 +334,35 @@ struct Request {
   size_t rpitch;
 };
 
+// A stream slot: free; pending (opened, waiting for HBM to admit it, no
+// device memory yet); active (its pipeline resident); or hibernated (its
+// volume paged out to pinned host memory to make room for others; the rest
+// of its pipeline stays resident).
+enum StreamState : int { kFree = 0, kPending = 1, kActive = 2, kHibernated = 3 };
+
 struct HostStream {
   yas_pipeline* pipe = nullptr;
-  size_t bytes = 0;
-  long long frames = 0;  // frames launched so far
+  int state = kFree;
+  bool closing = false;   // released once its waiting and in-flight frames are done
+  size_t bytes = 0;       // device bytes held now
+  size_t full_bytes = 0;  // device bytes while active
+  size_t map_bytes = 0;   // of which the volume's (released while hibernated)
+  long long frames = 0;   // frames launched so far
   std::deque<Request> waiting;
   int in_flight = 0;
+  long long order = 0;      // open order: pending streams are admitted first come, first served
+  double last_use_ms = 0;   // host time of its last submit or launch (for idle eviction)
+  // what it was opened with (a pending stream is created on admission)
+  int W = 0, H = 0;
+  float K[4] = {0, 0, 0, 0};
+  yas_map_params mp{};
+  float depth_scale = 1000.f;
+  int gray_mode = 0;
 };
+
+inline double now_ms() {
+  return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
 
 struct Batch {
   Table* staged = nullptr;   // pinned
@@ -352,10 +374,16 @@ struct Batch {
   long long req_of[kMaxSlots];
 };
 
+// Device bytes of a volume: hash table, block pool and block lists.
+size_t map_bytes(const yas_map_params& mp) {
+  const size_t vb = mp.voxel_format == YAS_VOXEL_F32_F32 ? 8 : 4;
+  return (size_t)mp.num_buckets * 16 + (size_t)mp.num_blocks * (vb * 512 + 8 + 4 + 4);
+}
+
 // Device bytes held by one stream (its map, ray caster, tracker and inputs).
 size_t stream_bytes(int W, int H, const yas_map_params& mp, int levels) {
   const size_t vb = mp.voxel_format == YAS_VOXEL_F32_F32 ? 8 : 4;
-  size_t b = (size_t)mp.num_buckets * 16 + (size_t)mp.num_blocks * (vb * 512 + 8 + 4 + 4);
+  size_t b = map_bytes(mp);
   b += (size_t)W * H * 16 + (size_t)(W / 8) * (H / 8) * 8;  // vertex map, z-ranges
   size_t px = 0;
   for (int l = 0; l < levels; ++l) px += (size_t)(W >> l) * (H >> l);
@@ -377,7 +405,10 @@ struct yas_server {
   int max_streams;
   int max_batch;  // frames per batch (<= kMaxSlots)
   int gn_group;   // slots per Gauss-Newton sub-range (0: the whole batch)
-  std::vector<HostStream> streams;  // slot per stream id (pipe == nullptr: free)
+  std::vector<HostStream> streams;  // slot per stream id
+  double evict_idle_ms;  // < 0: never hibernate; else idle streams older than this may be paged out
+  long long next_order;
+  long long hibernations, reactivations;
   StreamDev* dev_streams;
   Table* dev_table;
   double* dev_results;  // [kInFlight][kMaxSlots * kResultStride]
@@ -540,6 +571,9 @@ int yas_server_create(yas_ctx* ctx, int max_width, int max_height, int max_strea
   sv->next_request = 0;
   sv->rr = 0;
   sv->frames_done = sv->batches_done = sv->slots_done = 0;
+  sv->evict_idle_ms = -1.0;
+  sv->next_order = 0;
+  sv->hibernations = sv->reactivations = 0;
   int rc = yas_ctx_create(ctx->device, nullptr, &sv->ctx);
   if (rc) {
     delete sv;
@@ -589,62 +623,228 @@ int yas_server_destroy(yas_server* sv) {
   return YAS_OK;
 }
 
-int yas_server_add_stream(yas_server* sv, int W, int H, const float K[4], const yas_map_params* mp, float depth_scale,
-                          int gray_mode, int* stream_id) {
-  YAS_REQUIRE(sv && K && mp && stream_id, "yas_server_add_stream: null argument");
-  YAS_REQUIRE(W <= sv->maxW && H <= sv->maxH, "yas_server_add_stream: image larger than the server's maximum");
-  YAS_REQUIRE(mp->voxel_format == sv->voxel_format, "yas_server_add_stream: voxel format differs from the server's");
+// ------------------------------------------------------------------ admission
+// Streams are admitted by HBM budget.  A stream that does not fit waits
+// (pending) and is admitted, first come first served, as memory frees up:
+// when a stream is closed or removed, or, with eviction enabled, when idle
+// streams are hibernated (their volumes paged out to pinned host memory,
+// VoxelMap.cu:894-954).  A hibernated stream that receives a frame is
+// re-activated as soon as its volume fits again.
+
+static int refresh_descriptor(yas_server* sv, int id) {
+  StreamDev d;
+  int rc = yas_pipeline_fill(sv->streams[id].pipe, d);
+  if (rc) return rc;
+  YAS_HIP(hipMemcpyAsync(sv->dev_streams + id, &d, sizeof(d), hipMemcpyHostToDevice, sv->ctx->stream));
+  YAS_HIP(hipStreamSynchronize(sv->ctx->stream));
+  return YAS_OK;
+}
+
+static int activate(yas_server* sv, int id) {
+  HostStream& hs = sv->streams[id];
+  yas_pipeline* p = nullptr;
+  int rc = yas_pipeline_create(sv->ctx, hs.W, hs.H, hs.K, &hs.mp, &sv->tp, hs.depth_scale, hs.gray_mode, &p);
+  if (rc) return rc;
+  hs.pipe = p;
+  if ((rc = refresh_descriptor(sv, id))) {
+    yas_pipeline_destroy(p);
+    hs.pipe = nullptr;
+    return rc;
+  }
+  hs.state = kActive;
+  hs.bytes = hs.full_bytes;
+  hs.frames = 0;
+  hs.in_flight = 0;
+  hs.last_use_ms = now_ms();
+  sv->used += hs.full_bytes;
+  return YAS_OK;
+}
+
+static void release(yas_server* sv, int id) {
+  HostStream& hs = sv->streams[id];
+  if (hs.pipe) yas_pipeline_destroy(hs.pipe);
+  sv->used -= hs.bytes;
+  hs = HostStream();
+}
+
+static bool idle_for_eviction(const yas_server* sv, const HostStream& hs, double now) {
+  return hs.state == kActive && !hs.closing && hs.waiting.empty() && hs.in_flight == 0 &&
+         now - hs.last_use_ms >= sv->evict_idle_ms;
+}
+
+// Make `need` bytes fit under the budget by hibernating idle streams (least
+// recently used first), if eviction is on and enough of them are idle.
+static int make_room(yas_server* sv, size_t need, bool* fits) {
+  *fits = sv->used + need <= sv->budget;
+  if (*fits || sv->evict_idle_ms < 0) return YAS_OK;
+  const double now = now_ms();
+  std::vector<int> cand;
+  size_t freeable = 0;
+  for (int i = 0; i < sv->max_streams; ++i)
+    if (idle_for_eviction(sv, sv->streams[i], now)) {
+      cand.push_back(i);
+      freeable += sv->streams[i].map_bytes;
+    }
+  if (sv->used - freeable + need > sv->budget) return YAS_OK;  // not even all of them would do
+  std::sort(cand.begin(), cand.end(),
+            [&](int a, int b) { return sv->streams[a].last_use_ms < sv->streams[b].last_use_ms; });
+  for (int i : cand) {
+    if (sv->used + need <= sv->budget) break;
+    HostStream& hs = sv->streams[i];
+    int rc = yas_map_hibernate(yas_pipeline_map_of(hs.pipe));
+    if (rc) return rc;
+    hs.state = kHibernated;
+    hs.bytes -= hs.map_bytes;
+    sv->used -= hs.map_bytes;
+    sv->hibernations++;
+  }
+  *fits = sv->used + need <= sv->budget;
+  return YAS_OK;
+}
+
+// Release finished closing streams, re-activate hibernated streams that have
+// frames waiting (oldest request first), then admit pending streams in open
+// order while they fit.
+static int admit(yas_server* sv) {
+  for (int i = 0; i < sv->max_streams; ++i) {
+    HostStream& hs = sv->streams[i];
+    if (hs.state != kFree && hs.closing && hs.waiting.empty() && hs.in_flight == 0) release(sv, i);
+  }
+  std::vector<int> wake;
+  for (int i = 0; i < sv->max_streams; ++i)
+    if (sv->streams[i].state == kHibernated && !sv->streams[i].waiting.empty()) wake.push_back(i);
+  std::sort(wake.begin(), wake.end(),
+            [&](int a, int b) { return sv->streams[a].waiting.front().id < sv->streams[b].waiting.front().id; });
+  for (int i : wake) {
+    HostStream& hs = sv->streams[i];
+    bool fits = false;
+    int rc = make_room(sv, hs.map_bytes, &fits);
+    if (rc) return rc;
+    if (!fits) continue;
+    if ((rc = yas_map_reactivate(yas_pipeline_map_of(hs.pipe)))) return rc;
+    hs.state = kActive;
+    hs.bytes += hs.map_bytes;
+    sv->used += hs.map_bytes;
+    sv->reactivations++;
+    if ((rc = refresh_descriptor(sv, i))) return rc;
+  }
+  std::vector<int> pend;
+  for (int i = 0; i < sv->max_streams; ++i)
+    if (sv->streams[i].state == kPending) pend.push_back(i);
+  std::sort(pend.begin(), pend.end(), [&](int a, int b) { return sv->streams[a].order < sv->streams[b].order; });
+  for (int i : pend) {
+    bool fits = false;
+    int rc = make_room(sv, sv->streams[i].full_bytes, &fits);
+    if (rc) return rc;
+    if (!fits) break;
+    if ((rc = activate(sv, i))) return rc;
+  }
+  return YAS_OK;
+}
+
+static int open_slot(yas_server* sv, int W, int H, const float K[4], const yas_map_params* mp, float depth_scale,
+                     int gray_mode, int* id_out) {
+  YAS_REQUIRE(W <= sv->maxW && H <= sv->maxH, "yas_server: image larger than the server's maximum");
+  YAS_REQUIRE(mp->voxel_format == sv->voxel_format, "yas_server: voxel format differs from the server's");
+  YAS_REQUIRE(depth_scale > 0.f, "yas_server: depth_scale must be > 0");
   int id = -1;
   for (int i = 0; i < sv->max_streams; ++i)
-    if (!sv->streams[i].pipe) {
+    if (sv->streams[i].state == kFree) {
       id = i;
       break;
     }
   if (id < 0) {
-    yas_set_error("yas_server_add_stream: all %d stream slots are taken", sv->max_streams);
+    yas_set_error("yas_server: all %d stream slots are taken", sv->max_streams);
     return YAS_E_CAPACITY;
   }
-  const size_t need = stream_bytes(W, H, *mp, sv->tp.levels);
-  if (sv->used + need > sv->budget) {
-    yas_set_error("yas_server_add_stream: %zu bytes would exceed the HBM budget (%zu of %zu in use)", need, sv->used,
-                  sv->budget);
-    return YAS_E_CAPACITY;
-  }
+  HostStream& hs = sv->streams[id];
+  hs = HostStream();
+  hs.W = W;
+  hs.H = H;
+  std::memcpy(hs.K, K, sizeof(hs.K));
+  hs.mp = *mp;
+  hs.depth_scale = depth_scale;
+  hs.gray_mode = gray_mode;
+  hs.full_bytes = stream_bytes(W, H, *mp, sv->tp.levels);
+  hs.map_bytes = map_bytes(*mp);
+  hs.order = sv->next_order++;
+  *id_out = id;
+  return YAS_OK;
+}
+
+int yas_server_add_stream(yas_server* sv, int W, int H, const float K[4], const yas_map_params* mp, float depth_scale,
+                          int gray_mode, int* stream_id) {
+  YAS_REQUIRE(sv && K && mp && stream_id, "yas_server_add_stream: null argument");
   YAS_HIP(hipSetDevice(sv->device));
-  yas_pipeline* p = nullptr;
-  int rc = yas_pipeline_create(sv->ctx, W, H, K, mp, &sv->tp, depth_scale, gray_mode, &p);
+  int id = -1;
+  int rc = open_slot(sv, W, H, K, mp, depth_scale, gray_mode, &id);
   if (rc) return rc;
-  StreamDev d;
-  if ((rc = yas_pipeline_fill(p, d))) {
-    yas_pipeline_destroy(p);
+  HostStream& hs = sv->streams[id];
+  bool fits = false;
+  if ((rc = make_room(sv, hs.full_bytes, &fits))) {
+    hs = HostStream();
     return rc;
   }
-  YAS_HIP(hipMemcpyAsync(sv->dev_streams + id, &d, sizeof(d), hipMemcpyHostToDevice, sv->ctx->stream));
-  YAS_HIP(hipStreamSynchronize(sv->ctx->stream));
-  HostStream& hs = sv->streams[id];
-  hs.pipe = p;
-  hs.bytes = need;
-  hs.frames = 0;
-  hs.waiting.clear();
-  hs.in_flight = 0;
-  sv->used += need;
+  if (!fits) {
+    yas_set_error("yas_server_add_stream: %zu bytes would exceed the HBM budget (%zu of %zu in use)", hs.full_bytes,
+                  sv->used, sv->budget);
+    hs = HostStream();
+    return YAS_E_CAPACITY;
+  }
+  if ((rc = activate(sv, id))) {
+    hs = HostStream();
+    return rc;
+  }
   *stream_id = id;
   return YAS_OK;
 }
 
+int yas_server_open_stream(yas_server* sv, int W, int H, const float K[4], const yas_map_params* mp,
+                           float depth_scale, int gray_mode, int* stream_id) {
+  YAS_REQUIRE(sv && K && mp && stream_id, "yas_server_open_stream: null argument");
+  YAS_HIP(hipSetDevice(sv->device));
+  int id = -1;
+  int rc = open_slot(sv, W, H, K, mp, depth_scale, gray_mode, &id);
+  if (rc) return rc;
+  sv->streams[id].state = kPending;
+  *stream_id = id;
+  return admit(sv);
+}
+
+int yas_server_close_stream(yas_server* sv, int stream_id) {
+  YAS_REQUIRE(sv && stream_id >= 0 && stream_id < sv->max_streams && sv->streams[stream_id].state != kFree,
+              "yas_server_close_stream: no such stream");
+  YAS_HIP(hipSetDevice(sv->device));
+  HostStream& hs = sv->streams[stream_id];
+  if (hs.state == kPending) hs.waiting.clear();  // never admitted: its frames are dropped
+  hs.closing = true;
+  int rc = collect(sv, false);
+  return rc ? rc : admit(sv);
+}
+
 int yas_server_remove_stream(yas_server* sv, int stream_id) {
-  YAS_REQUIRE(sv && stream_id >= 0 && stream_id < sv->max_streams && sv->streams[stream_id].pipe,
+  YAS_REQUIRE(sv && stream_id >= 0 && stream_id < sv->max_streams && sv->streams[stream_id].state != kFree,
               "yas_server_remove_stream: no such stream");
+  YAS_HIP(hipSetDevice(sv->device));
   HostStream& hs = sv->streams[stream_id];
   YAS_REQUIRE(hs.waiting.empty(), "yas_server_remove_stream: the stream still has frames waiting");
   while (hs.in_flight > 0) {
     int rc = collect(sv, true);
     if (rc) return rc;
   }
-  yas_pipeline_destroy(hs.pipe);
-  hs.pipe = nullptr;
-  sv->used -= hs.bytes;
-  hs.bytes = 0;
+  release(sv, stream_id);
+  return admit(sv);
+}
+
+int yas_server_set_eviction(yas_server* sv, double idle_ms) {
+  YAS_REQUIRE(sv, "null server");
+  sv->evict_idle_ms = idle_ms;
+  return YAS_OK;
+}
+
+int yas_server_stream_state(yas_server* sv, int stream_id, int* state) {
+  YAS_REQUIRE(sv && state && stream_id >= 0 && stream_id < sv->max_streams, "yas_server_stream_state: bad argument");
+  *state = sv->streams[stream_id].state;
   return YAS_OK;
 }
 
@@ -694,10 +894,13 @@ int yas_server_stream_pipeline(yas_server* sv, int stream_id, yas_pipeline** pip
 
 int yas_server_submit(yas_server* sv, int stream_id, const uint16_t* depth, size_t dpitch, const uint8_t* rgb,
                       size_t rpitch, long long* request_id) {
-  YAS_REQUIRE(sv && depth && rgb && stream_id >= 0 && stream_id < sv->max_streams && sv->streams[stream_id].pipe,
-              "yas_server_submit: bad argument");
+  YAS_REQUIRE(sv && depth && rgb && stream_id >= 0 && stream_id < sv->max_streams &&
+                  sv->streams[stream_id].state != kFree && !sv->streams[stream_id].closing,
+              "yas_server_submit: no such open stream");
   Request r{sv->next_request++, depth, dpitch, rgb, rpitch};
-  sv->streams[stream_id].waiting.push_back(r);
+  HostStream& hs = sv->streams[stream_id];
+  hs.waiting.push_back(r);
+  hs.last_use_ms = now_ms();
   if (request_id) *request_id = r.id;
   return YAS_OK;
 }
@@ -707,6 +910,7 @@ int yas_server_pump(yas_server* sv, int* launched) {
   YAS_HIP(hipSetDevice(sv->device));
   int n_launched = 0;
   int rc = collect(sv, false);
+  if (!rc) rc = admit(sv);
   if (rc) return rc;
   for (;;) {
     Batch& b = sv->batches[sv->next_batch];
@@ -714,16 +918,17 @@ int yas_server_pump(yas_server* sv, int* launched) {
     // size of the next batch, and its graph, before any frame leaves its queue
     int want = 0;
     for (int i = 0; i < sv->max_streams && want < sv->max_batch; ++i)
-      if (sv->streams[i].pipe && !sv->streams[i].waiting.empty()) ++want;
+      if (sv->streams[i].state == kActive && !sv->streams[i].waiting.empty()) ++want;
     if (want == 0) break;
     hipGraphExec_t ex = nullptr;
     if ((rc = graph_for(sv, want, &ex))) return rc;
     int count = 0;
+    const double now = now_ms();
     // round-robin start, so a batch cap does not favour low stream ids
     for (int j = 0; j < sv->max_streams && count < sv->max_batch; ++j) {
       const int i = (sv->rr + j) % sv->max_streams;
       HostStream& hs = sv->streams[i];
-      if (!hs.pipe || hs.waiting.empty()) continue;
+      if (hs.state != kActive || hs.waiting.empty()) continue;
       const Request r = hs.waiting.front();
       hs.waiting.pop_front();
       Slot& s = b.staged->slot[count];
@@ -738,6 +943,7 @@ int yas_server_pump(yas_server* sv, int* launched) {
       b.req_of[count] = r.id;
       hs.frames++;
       hs.in_flight++;
+      hs.last_use_ms = now;
       ++count;
     }
     sv->rr = (b.stream_of[count - 1] + 1) % sv->max_streams;
@@ -778,11 +984,25 @@ int yas_server_drain(yas_server* sv) {
     int launched = 0;
     int rc = yas_server_pump(sv, &launched);
     if (rc) return rc;
-    bool waiting = false;
-    for (auto& hs : sv->streams) waiting = waiting || !hs.waiting.empty();
+    bool waiting = false, stuck = false;
+    for (auto& hs : sv->streams) {
+      if (hs.waiting.empty()) continue;
+      if (hs.state == kActive) waiting = true;
+      else stuck = true;
+    }
     bool busy = false;
     for (auto& b : sv->batches) busy = busy || b.busy;
-    if (!waiting && !busy) return YAS_OK;
+    if (!waiting && !busy) {
+      if (!stuck) return YAS_OK;
+      if ((rc = admit(sv))) return rc;
+      bool admitted = false;
+      for (auto& hs : sv->streams) admitted = admitted || (hs.state == kActive && !hs.waiting.empty());
+      if (!admitted) {
+        yas_set_error("yas_server_drain: frames wait for streams that do not fit the HBM budget");
+        return YAS_E_CAPACITY;
+      }
+      continue;
+    }
     if (!launched) {
       if ((rc = collect(sv, true))) return rc;
     }
@@ -791,13 +1011,21 @@ int yas_server_drain(yas_server* sv) {
 
 int yas_server_get_stats(yas_server* sv, yas_server_stats* st) {
   YAS_REQUIRE(sv && st, "null argument");
-  int n = 0;
-  for (auto& hs : sv->streams) n += hs.pipe ? 1 : 0;
+  int n = 0, pending = 0, hib = 0;
+  for (auto& hs : sv->streams) {
+    n += hs.state == kActive || hs.state == kHibernated;
+    pending += hs.state == kPending;
+    hib += hs.state == kHibernated;
+  }
   st->streams = n;
   st->bytes_in_use = sv->used;
   st->bytes_budget = sv->budget;
   st->frames_done = sv->frames_done;
   st->batches_done = sv->batches_done;
+  st->streams_pending = pending;
+  st->streams_hibernated = hib;
+  st->hibernations = sv->hibernations;
+  st->reactivations = sv->reactivations;
   return YAS_OK;
 }
 

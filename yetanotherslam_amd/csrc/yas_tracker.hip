@@ -21,9 +21,13 @@
 // pass 1 instead: every workgroup of that pass folds the same rows and takes
 // the same step, one launch fewer per iteration with the same values.
 // Results are deterministic run to run.
+#include <algorithm>
 #include <cfloat>
 #include <cmath>
+#include <cstdio>
 #include <cstring>
+#include <string>
+#include <vector>
 
 #include "yas_internal.hpp"
 #include "yas_tracker_dev.hpp"
@@ -195,6 +199,8 @@ struct yas_tracker {
   TrackState* st2;  // a solve's steps alternate between st and st2
   TrackState* host_st;
   float angle_th, dist_th;
+  float* raw_depth;  // depth uploaded by the host-image setters (W*H, lazily allocated)
+  float* raw_image;  // intensity uploaded by the host-image setters
 };
 
 static int image_pyramid(yas_tracker* t, const float* g, size_t pitch, float** pyr, bool grads) {
@@ -515,6 +521,8 @@ int yas_tracker_destroy(yas_tracker* t) {
   hipFree(t->part_rgb);
   hipFree(t->st);
   hipFree(t->st2);
+  hipFree(t->raw_depth);
+  hipFree(t->raw_image);
   if (t->host_st) hipHostFree(t->host_st);
   yas_ctx_release(t->ctx);
   delete t;
@@ -537,6 +545,102 @@ int yas_tracker_set_tracking_depth(yas_tracker* t, const float* d, size_t pitch)
   YAS_REQUIRE(t && d && pitch >= (size_t)t->W * 4, "yas_tracker_set_tracking_depth: bad argument");
   return depth_pyramid(t, d, pitch, t->cur_inv, t->cur_v, t->cur_p);
 }
+// Host images (RGBDTracking.h:17-21 take cv::Mat): the reference converts an
+// 8-bit gray image to float on the host and uploads it (RGBDTracking.cu:68-
+// 71,104-107), and uploads float depth into one raw-depth buffer
+// (mGpuBufferRawDepth, RGBDTracking.cu:86-87,122-123) that
+// GetReferenceDepth returns.  The copy waits for the stream, so the host
+// buffer may be reused when the call returns.
+static int upload_host(yas_tracker* t, float** dst, const void* src, size_t pitch, bool gray_u8) {
+  YAS_HIP(hipSetDevice(t->ctx->device));
+  const size_t row = (size_t)t->W * sizeof(float);
+  if (!*dst) YAS_HIP(hipMalloc((void**)dst, row * t->H));
+  std::vector<float> conv;
+  const void* from = src;
+  size_t from_pitch = pitch;
+  if (gray_u8) {
+    conv.resize((size_t)t->W * t->H);
+    for (int y = 0; y < t->H; ++y) {
+      const uint8_t* r = (const uint8_t*)src + (size_t)y * pitch;
+      for (int x = 0; x < t->W; ++x) conv[(size_t)y * t->W + x] = (float)r[x];
+    }
+    from = conv.data();
+    from_pitch = row;
+  }
+  YAS_HIP(hipMemcpy2DAsync(*dst, row, from, from_pitch, row, t->H, hipMemcpyHostToDevice, t->ctx->stream));
+  YAS_HIP(hipStreamSynchronize(t->ctx->stream));
+  return YAS_OK;
+}
+
+int yas_tracker_set_reference_image_host(yas_tracker* t, const uint8_t* g, size_t pitch) {
+  YAS_REQUIRE(t && g && pitch >= (size_t)t->W, "yas_tracker_set_reference_image_host: bad argument");
+  int rc = upload_host(t, &t->raw_image, g, pitch, true);
+  return rc ? rc : image_pyramid(t, t->raw_image, (size_t)t->W * 4, t->ref_int, false);
+}
+int yas_tracker_set_tracking_image_host(yas_tracker* t, const uint8_t* g, size_t pitch) {
+  YAS_REQUIRE(t && g && pitch >= (size_t)t->W, "yas_tracker_set_tracking_image_host: bad argument");
+  int rc = upload_host(t, &t->raw_image, g, pitch, true);
+  return rc ? rc : image_pyramid(t, t->raw_image, (size_t)t->W * 4, t->cur_int, true);
+}
+int yas_tracker_set_reference_depth_host(yas_tracker* t, const float* d, size_t pitch) {
+  YAS_REQUIRE(t && d && pitch >= (size_t)t->W * 4, "yas_tracker_set_reference_depth_host: bad argument");
+  int rc = upload_host(t, &t->raw_depth, d, pitch, false);
+  return rc ? rc : depth_pyramid(t, t->raw_depth, (size_t)t->W * 4, t->ref_inv, t->ref_v, t->ref_p);
+}
+int yas_tracker_set_tracking_depth_host(yas_tracker* t, const float* d, size_t pitch) {
+  YAS_REQUIRE(t && d && pitch >= (size_t)t->W * 4, "yas_tracker_set_tracking_depth_host: bad argument");
+  int rc = upload_host(t, &t->raw_depth, d, pitch, false);
+  return rc ? rc : depth_pyramid(t, t->raw_depth, (size_t)t->W * 4, t->cur_inv, t->cur_v, t->cur_p);
+}
+
+// GetReferenceDepth (RGBDTracking.cu:547-550) returns the raw-depth buffer,
+// i.e. the depth most recently uploaded from the host (NULL before any).
+int yas_tracker_raw_depth(yas_tracker* t, float** depth, size_t* pitch) {
+  YAS_REQUIRE(t && depth && pitch, "yas_tracker_raw_depth: null argument");
+  *depth = t->raw_depth;
+  *pitch = (size_t)t->W * sizeof(float);
+  return YAS_OK;
+}
+
+// WriteDebugImages (RGBDTracking.cu:557-568): the level-0 current and
+// reference intensity and the two gradient images, as 8-bit binary PGM
+// files curr_image.pgm, last_image.pgm, gx.pgm, gy.pgm in `dir` (values
+// rounded to nearest and clamped to [0, 255]).
+int yas_tracker_write_debug_images(yas_tracker* t, const char* dir) {
+  YAS_REQUIRE(t && dir, "yas_tracker_write_debug_images: null argument");
+  YAS_HIP(hipSetDevice(t->ctx->device));
+  YAS_HIP(hipStreamSynchronize(t->ctx->stream));
+  const size_t n = (size_t)t->W * t->H;
+  const struct {
+    const char* name;
+    const float* img;
+  } out[4] = {{"curr_image.pgm", t->cur_int[0]}, {"last_image.pgm", t->ref_int[0]}, {"gx.pgm", t->gx[0]},
+              {"gy.pgm", t->gy[0]}};
+  std::vector<float> f(n);
+  std::vector<unsigned char> b(n);
+  for (const auto& o : out) {
+    YAS_HIP(hipMemcpy(f.data(), o.img, n * sizeof(float), hipMemcpyDeviceToHost));
+    for (size_t i = 0; i < n; ++i) {
+      float v = std::nearbyint(f[i]);
+      b[i] = (unsigned char)(v != v ? 0.f : std::min(255.f, std::max(0.f, v)));
+    }
+    std::string path = std::string(dir) + "/" + o.name;
+    FILE* fp = std::fopen(path.c_str(), "wb");
+    if (!fp) {
+      yas_set_error("yas_tracker_write_debug_images: cannot open %s", path.c_str());
+      return YAS_E_INVALID;
+    }
+    std::fprintf(fp, "P5\n%d %d\n255\n", t->W, t->H);
+    bool ok = std::fwrite(b.data(), 1, n, fp) == n;
+    ok = (std::fclose(fp) == 0) && ok;
+    if (!ok) {
+      yas_set_error("yas_tracker_write_debug_images: write to %s failed", path.c_str());
+      return YAS_E_INVALID;
+    }
+  }
+  return YAS_OK;
+}
+
 int yas_tracker_set_reference_model(yas_tracker* t, const float* vmap, size_t pitch) {
   YAS_REQUIRE(t && vmap && pitch >= (size_t)t->W * 16, "yas_tracker_set_reference_model: bad argument");
   hipLaunchKernelGGL(k_copy_f4, grid64x4(t->W, t->H), dim3(256), 0, t->ctx->stream, (const float4*)vmap, pitch,

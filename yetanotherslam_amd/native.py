@@ -37,7 +37,8 @@ class Result(ctypes.Structure):
 
 class ServerStats(ctypes.Structure):
     _fields_ = [("streams", I), ("frames_done", ctypes.c_longlong), ("batches_done", ctypes.c_longlong),
-                ("bytes_in_use", ctypes.c_size_t), ("bytes_budget", ctypes.c_size_t)]
+                ("bytes_in_use", ctypes.c_size_t), ("bytes_budget", ctypes.c_size_t), ("streams_pending", I),
+                ("streams_hibernated", I), ("hibernations", ctypes.c_longlong), ("reactivations", ctypes.c_longlong)]
 
 
 class MapStats(ctypes.Structure):
@@ -71,12 +72,14 @@ SIGNATURES = {
     "yas_map_fuse_u16": (I, [P, P, SZ, I, I, F, P]),
     "yas_map_check_visible_blocks": (I, [P, I, I, P, PI]),
     "yas_map_get_stats": (I, [P, ctypes.POINTER(MapStats)]),
+    "yas_map_visible_blocks": (I, [P, P, I, PI]),
     "yas_map_export": (I, [P, P, P, I, PI]),
     "yas_map_voxel_bytes": (I, [P]),
     "yas_raycaster_create": (I, [P, I, I, P, PP]),
     "yas_raycaster_destroy": (I, [P]),
     "yas_raycast": (I, [P, P, P]),
     "yas_raycaster_vmap": (I, [P, PP, ctypes.POINTER(SZ)]),
+    "yas_raycaster_depth": (I, [P, PP, ctypes.POINTER(SZ), P, SZ]),
     "yas_raycaster_zrange": (I, [P, PP, PP, PI, PI]),
     "yas_raycaster_num_visible": (I, [P, PI]),
     "yas_mesher_create": (I, [P, I, PP]),
@@ -89,6 +92,10 @@ SIGNATURES = {
     "yas_server_create": (I, [P, I, I, I, P, I, ctypes.c_size_t, P]),
     "yas_server_destroy": (I, [P]),
     "yas_server_add_stream": (I, [P, I, I, P, P, ctypes.c_float, I, PI]),
+    "yas_server_open_stream": (I, [P, I, I, P, P, F, I, PI]),
+    "yas_server_close_stream": (I, [P, I]),
+    "yas_server_set_eviction": (I, [P, D]),
+    "yas_server_stream_state": (I, [P, I, PI]),
     "yas_server_remove_stream": (I, [P, I]),
     "yas_server_stream_map": (I, [P, I, P]),
     "yas_server_stream_pipeline": (I, [P, I, P]),
@@ -123,6 +130,12 @@ SIGNATURES = {
     "yas_tracker_set_tracking_image": (I, [P, P, SZ]),
     "yas_tracker_set_reference_depth": (I, [P, P, SZ]),
     "yas_tracker_set_tracking_depth": (I, [P, P, SZ]),
+    "yas_tracker_set_reference_image_host": (I, [P, P, SZ]),
+    "yas_tracker_set_tracking_image_host": (I, [P, P, SZ]),
+    "yas_tracker_set_reference_depth_host": (I, [P, P, SZ]),
+    "yas_tracker_set_tracking_depth_host": (I, [P, P, SZ]),
+    "yas_tracker_raw_depth": (I, [P, PP, ctypes.POINTER(SZ)]),
+    "yas_tracker_write_debug_images": (I, [P, ctypes.c_char_p]),
     "yas_tracker_set_reference_model": (I, [P, P, SZ]),
     "yas_tracker_get_transform": (I, [P, P, I, P, ctypes.POINTER(TrackStats)]),
     "yas_tracker_swap_frame_buffer": (I, [P]),

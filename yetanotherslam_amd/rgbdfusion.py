@@ -216,6 +216,14 @@ class MapStruct(_Native):
         call("yas_map_check_visible_blocks", self.h, cols, rows, _ptr(_f64(T)), ctypes.byref(n))
         return n.value
 
+    def visible_blocks(self):
+        """Keys (n, 3) of the blocks the last visibility pass found visible."""
+        n = self.stats().num_visible
+        keys = np.zeros((max(n, 1), 3), np.int32)
+        cnt = ctypes.c_int()
+        call("yas_map_visible_blocks", self.h, _ptr(keys), n, ctypes.byref(cnt))
+        return keys[:cnt.value]
+
     def stats(self):
         s = MapStats()
         call("yas_map_get_stats", self.h, ctypes.byref(s))
@@ -310,6 +318,30 @@ class Server(_Native):
         self._dims = getattr(self, "_dims", {})
         self._dims[sid.value] = (H, W)
         return sid.value
+
+    def open_stream(self, W, H, K, map_params, depth_scale=1000.0, gray_mode=0):
+        """A stream admitted as soon as it fits the HBM budget (pending until
+        then); frames may be submitted at once."""
+        Kf = np.asarray(K, np.float32)
+        sid = ctypes.c_int()
+        call("yas_server_open_stream", self.h, W, H, _ptr(Kf), ctypes.byref(map_params), depth_scale, gray_mode,
+             ctypes.byref(sid))
+        self._dims = getattr(self, "_dims", {})
+        self._dims[sid.value] = (H, W)
+        return sid.value
+
+    def close_stream(self, sid):
+        call("yas_server_close_stream", self.h, sid)
+
+    def set_eviction(self, idle_ms):
+        call("yas_server_set_eviction", self.h, float(idle_ms))
+
+    STATES = ("free", "pending", "active", "hibernated")
+
+    def stream_state(self, sid):
+        v = ctypes.c_int()
+        call("yas_server_stream_state", self.h, sid, ctypes.byref(v))
+        return self.STATES[v.value]
 
     def set_gn_group(self, n):
         call("yas_server_set_gn_group", self.h, n)
