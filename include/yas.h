@@ -323,6 +323,20 @@ int yas_raycaster_destroy(yas_raycaster* rc);
 /* RayTrace(MapStruct*, SE3d) (RayTraceEngine.cu:412-450): the vertex map of
  * the volume seen from T_map_cam, in the camera frame. Asynchronous. */
 int yas_raycast(yas_raycaster* rc, yas_map* map, const double T_map_cam[16]);
+/* The ray cast of yas_raycast into caller memory, allocating nothing: no
+ * counterpart in the reference.  It is for callers that cannot allocate
+ * device memory: code under HIP graph capture, tight serving loops and
+ * harnesses that forbid hipMalloc.  The map (its visible-block list and
+ * pose) and the dense W*H float4 vertex map `vmap` (pitch = 16*W) are the
+ * caller's; intrinsics K = {fx, fy, cx, cy} as yas_raycaster_create.  The
+ * only scratch, the two 1/8-resolution z-range maps, is a static device
+ * array of 32768 cells per map (256 KiB per device, held until the process
+ * ends): (W/8)*(H/8) > 32768 is rejected with YAS_E_INVALID.  Every call on a
+ * device shares that array, so calls that overlap on one GPU (on different
+ * streams) race.  Asynchronous on the map's stream; results are those of
+ * yas_raycast with a ray caster of the same size and intrinsics. */
+int yas_raycast_into(yas_map* map, const double T_map_cam[16], const float K[4], int width, int height,
+                     float* vmap, size_t pitch);
 /* GetVMap() (RayTraceEngine.cu:452-454): device alias of the last result,
  * float4 per pixel, overwritten by the next yas_raycast. */
 int yas_raycaster_vmap(yas_raycaster* rc, float** vmap, size_t* pitch);

@@ -3,6 +3,7 @@
 //
 // The kernels' bodies are in yas_raycast_dev.hpp.
 #include <cstring>
+#include <mutex>
 
 #include "yas_internal.hpp"
 #include "yas_raycast_dev.hpp"
@@ -44,35 +45,74 @@ __global__ void __launch_bounds__(256) k_march(MapView m, const PosePair* pose, 
   march_at<VT>(m, pose, K, W, H, cols, zmin, zmax, vmap, vsinv, step_scale, blockIdx.x);
 }
 
-static int launch_raycast(yas_raycaster* rc, yas_map* map, hipEvent_t before_march = nullptr) {
-  hipStream_t s = rc->ctx->stream;
+// z-range tiles then the march (RayTraceEngine.cu:412-450) after the map's
+// visibility pass: z-range maps zmin/zmax of (W/8)*(H/8) cells, vertex map
+// `vmap` dense W*H float4.
+static int launch_raycast_to(hipStream_t s, yas_map* map, const Intr& K, int W, int H, unsigned* zmin,
+                             unsigned* zmax, float4* vmap, hipEvent_t before_march) {
   MapView v = yas_map_view(map);
   const PosePair* pose = yas_map_pose_dev(map);
-  const int ncell = rc->cols * rc->rows;
-  hipLaunchKernelGGL(k_zrange_init, dim3((ncell + 255) / 256), dim3(256), 0, s, rc->zmin, rc->zmax, ncell);
+  const int cols = W / 8, rows = H / 8, ncell = cols * rows;
+  hipLaunchKernelGGL(k_zrange_init, dim3((ncell + 255) / 256), dim3(256), 0, s, zmin, zmax, ncell);
   int g = (v.num_blocks + 255) / 256;
   if (g > 2048) g = 2048;
-  hipLaunchKernelGGL(k_zrange, dim3(g), dim3(256), 0, s, v, pose, rc->K, rc->cols, rc->rows, rc->zmin, rc->zmax);
+  hipLaunchKernelGGL(k_zrange, dim3(g), dim3(256), 0, s, v, pose, K, cols, rows, zmin, zmax);
   YAS_LAUNCH_CHECK();
   if (before_march) YAS_HIP(hipEventRecord(before_march, s));
   const float vsinv = (float)(1.0 / (double)v.voxel_size);
   const float step_scale = v.trunc / v.voxel_size;
-  dim3 grid(((rc->W + 15) / 16) * ((rc->H + 15) / 16));
+  dim3 grid(((W + 15) / 16) * ((H + 15) / 16));
   switch (v.format) {
     case YAS_VOXEL_I16_U8:
-      hipLaunchKernelGGL(k_march<VoxI16>, grid, dim3(256), 0, s, v, pose, rc->K, rc->W, rc->H, rc->cols,
-                         (const float*)rc->zmin, (const float*)rc->zmax, rc->vmap, vsinv, step_scale);
+      hipLaunchKernelGGL(k_march<VoxI16>, grid, dim3(256), 0, s, v, pose, K, W, H, cols, (const float*)zmin,
+                         (const float*)zmax, vmap, vsinv, step_scale);
       break;
     case YAS_VOXEL_F16_U16:
-      hipLaunchKernelGGL(k_march<VoxF16>, grid, dim3(256), 0, s, v, pose, rc->K, rc->W, rc->H, rc->cols,
-                         (const float*)rc->zmin, (const float*)rc->zmax, rc->vmap, vsinv, step_scale);
+      hipLaunchKernelGGL(k_march<VoxF16>, grid, dim3(256), 0, s, v, pose, K, W, H, cols, (const float*)zmin,
+                         (const float*)zmax, vmap, vsinv, step_scale);
       break;
     default:
-      hipLaunchKernelGGL(k_march<VoxF32>, grid, dim3(256), 0, s, v, pose, rc->K, rc->W, rc->H, rc->cols,
-                         (const float*)rc->zmin, (const float*)rc->zmax, rc->vmap, vsinv, step_scale);
+      hipLaunchKernelGGL(k_march<VoxF32>, grid, dim3(256), 0, s, v, pose, K, W, H, cols, (const float*)zmin,
+                         (const float*)zmax, vmap, vsinv, step_scale);
   }
   YAS_LAUNCH_CHECK();
+  return YAS_OK;
+}
+
+static int launch_raycast(yas_raycaster* rc, yas_map* map, hipEvent_t before_march = nullptr) {
+  int r = launch_raycast_to(rc->ctx->stream, map, rc->K, rc->W, rc->H, rc->zmin, rc->zmax, rc->vmap, before_march);
+  if (r) return r;
   rc->last_map = map;
+  return YAS_OK;
+}
+
+// ------------------------------------------------------------------ ray cast without allocation
+// yas_raycast_into keeps its only scratch, the two z-range maps, in this
+// file-scope device array: zmin in the first kScratchCells words, zmax in
+// the next (256 KiB, held on every device the library runs on until the
+// process ends).  The kernels get its device address, looked up once per
+// device with that device current; the array's name itself is a host-side
+// stand-in and is never passed to a kernel.
+constexpr int kScratchCells = 32768;
+static __device__ __align__(256) unsigned scratch[2 * kScratchCells];
+
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static unsigned* g_scratch_dev[kMaxDevices];
+
+static int scratch_on_device(int device, unsigned** out) {
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  YAS_REQUIRE(device >= 0 && device < kMaxDevices, "yas_raycast_into: device index out of range");
+  if (!g_scratch_dev[device]) {
+    void* p = nullptr;
+    hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+      yas_set_error("yas_raycast_into: hipGetSymbolAddress(scratch) failed: %s", hipGetErrorString(e));
+      return YAS_E_HIP;
+    }
+    g_scratch_dev[device] = (unsigned*)p;
+  }
+  *out = g_scratch_dev[device];
   return YAS_OK;
 }
 
@@ -151,6 +191,24 @@ int yas_raycast(yas_raycaster* rc, yas_map* map, const double T[16]) {
   int r = yas_map_visible_async(map, T, rc->W, rc->H);
   if (r) return r;
   return launch_raycast(rc, map);
+}
+
+int yas_raycast_into(yas_map* map, const double T[16], const float K[4], int W, int H, float* vmap, size_t pitch) {
+  YAS_REQUIRE(map && T && K && vmap, "yas_raycast_into: null argument");
+  YAS_REQUIRE(W >= 8 && H >= 8 && W % 8 == 0 && H % 8 == 0,
+              "yas_raycast_into: width and height must be multiples of 8");
+  YAS_REQUIRE((long long)(W / 8) * (H / 8) <= kScratchCells,
+              "yas_raycast_into: image exceeds the 32768 z-range cells of the scratch array "
+              "((width/8)*(height/8); e.g. 1280x960 fits, 2048x1152 does not)");
+  YAS_REQUIRE(pitch == sizeof(float4) * (size_t)W, "yas_raycast_into: the vertex map must be dense (pitch = 16 * width)");
+  YAS_REQUIRE_ACTIVE(map);
+  yas_ctx* ctx = yas_map_ctx(map);
+  YAS_HIP(hipSetDevice(ctx->device));
+  unsigned* zr = nullptr;
+  int r = scratch_on_device(ctx->device, &zr);
+  if (r) return r;
+  if ((r = yas_map_visible_async(map, T, W, H))) return r;
+  return launch_raycast_to(ctx->stream, map, make_intr(K), W, H, zr, zr + kScratchCells, (float4*)vmap, nullptr);
 }
 
 int yas_raycaster_vmap(yas_raycaster* rc, float** vmap, size_t* pitch) {

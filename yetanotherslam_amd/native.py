@@ -80,6 +80,7 @@ SIGNATURES = {
     "yas_raycast": (I, [P, P, P]),
     "yas_raycaster_vmap": (I, [P, PP, ctypes.POINTER(SZ)]),
     "yas_raycaster_depth": (I, [P, PP, ctypes.POINTER(SZ), P, SZ]),
+    "yas_raycast_into": (I, [P, P, P, I, I, P, SZ]),
     "yas_raycaster_zrange": (I, [P, PP, PP, PI, PI]),
     "yas_raycaster_num_visible": (I, [P, PI]),
     "yas_mesher_create": (I, [P, I, PP]),
