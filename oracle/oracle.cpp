@@ -6,7 +6,10 @@
 //
 // It is compiled with g++ -O2 -ffp-contract=off (no fast-math, no FMA
 // contraction) so that every float operation below is one IEEE operation in
-// the order written.  The HIP kernels evaluate the same operations in the same
+// the order written.  A second build with -fopenmp (liboracle_omp.so) runs
+// the per-pixel and per-block loops on all host cores; it exists only to time
+// the CPU baseline on every core (its pixel sums add in another order), and
+// the tests use the single-threaded build.  The HIP kernels evaluate the same operations in the same
 // order, which is what makes block sets, TSDF values and marching-cubes cases
 // bit-exact between the two.  Sums over pixels (J^T J, residual statistics)
 // are accumulated here in double and rounded to the working type once, which
@@ -43,6 +46,9 @@
 #include <memory>
 #include <unordered_map>
 #include <vector>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 
 namespace orc {
 
@@ -516,44 +522,66 @@ template <class V> struct Map : MapBase {
     return {vox_to_block(v[0]), vox_to_block(v[1]), vox_to_block(v[2])};
   }
 
-  // VoxelMap.cu:517-652 - allocate every block on the 3D Bresenham walk from
-  // the block at depth-mu/2 to the block at depth+mu/2 along each pixel ray.
+  // VoxelMap.cu:517-652 - every block on the 3D Bresenham walk from the
+  // block at depth-mu/2 to the block at depth+mu/2 along pixel (x, y)'s ray,
+  // passed to emit(bx, by, bz) in walk order.
+  template <class F>
+  void walk_pixel(const float* depth, int W, int x, int y, const Intr<R>& K, const Pose<R>& T, R half,
+                  F&& emit) const {
+    R d = (R)depth[(size_t)y * W + x];
+    if (is_nan(d) || d < dmin || d > dmax) return;
+    R dn = std::max(dmin, d - half);
+    R df = std::min(dmax, d + half);
+    if (dn >= df) return;
+    I3 b = block_of(x, y, dn, K, T);
+    I3 e = block_of(x, y, df, K, T);
+    int dir[3] = {e.x - b.x, e.y - b.y, e.z - b.z};
+    int inc[3] = {dir[0] < 0 ? -1 : 1, dir[1] < 0 ? -1 : 1, dir[2] < 0 ? -1 : 1};
+    int ad[3] = {std::abs(dir[0]), std::abs(dir[1]), std::abs(dir[2])};
+    int e2[3] = {ad[0] << 1, ad[1] << 1, ad[2] << 1};
+    int p[3] = {b.x, b.y, b.z};
+    // Dominant axis a, the two others walked in the reference's order
+    // (the z branch updates y before x, VoxelMap.cu:628-647).
+    int a, o1, o2;
+    if (ad[0] >= ad[1] && ad[0] >= ad[2]) { a = 0; o1 = 1; o2 = 2; }
+    else if (ad[1] >= ad[0] && ad[1] >= ad[2]) { a = 1; o1 = 0; o2 = 2; }
+    else { a = 2; o1 = 1; o2 = 0; }
+    int err1 = e2[o1] - 1, err2 = e2[o2] - 1;
+    emit(p[0], p[1], p[2]);
+    for (int i = 0; i < ad[a]; ++i) {
+      if (err1 > 0) { p[o1] += inc[o1]; err1 -= e2[a]; }
+      if (err2 > 0) { p[o2] += inc[o2]; err2 -= e2[a]; }
+      err1 += e2[o1];
+      err2 += e2[o2];
+      p[a] += inc[a];
+      emit(p[0], p[1], p[2]);
+    }
+  }
+
+  // VoxelMap.cu:517-652 - allocate every block on every pixel's walk.
   int allocate_impl(const float* depth, int W, int H, const Intr<R>& K, const Pose<R>& T) {
     int created = 0;
     walk_steps = 0;
     const R half = mu * (R)0.5;
-    for (int y = 0; y < H; ++y) {
-      for (int x = 0; x < W; ++x) {
-        R d = (R)depth[(size_t)y * W + x];
-        if (is_nan(d) || d < dmin || d > dmax) continue;
-        R dn = std::max(dmin, d - half);
-        R df = std::min(dmax, d + half);
-        if (dn >= df) continue;
-        I3 b = block_of(x, y, dn, K, T);
-        I3 e = block_of(x, y, df, K, T);
-        int dir[3] = {e.x - b.x, e.y - b.y, e.z - b.z};
-        int inc[3] = {dir[0] < 0 ? -1 : 1, dir[1] < 0 ? -1 : 1, dir[2] < 0 ? -1 : 1};
-        int ad[3] = {std::abs(dir[0]), std::abs(dir[1]), std::abs(dir[2])};
-        int e2[3] = {ad[0] << 1, ad[1] << 1, ad[2] << 1};
-        int p[3] = {b.x, b.y, b.z};
-        // Dominant axis a, the two others walked in the reference's order
-        // (the z branch updates y before x, VoxelMap.cu:628-647).
-        int a, o1, o2;
-        if (ad[0] >= ad[1] && ad[0] >= ad[2]) { a = 0; o1 = 1; o2 = 2; }
-        else if (ad[1] >= ad[0] && ad[1] >= ad[2]) { a = 1; o1 = 0; o2 = 2; }
-        else { a = 2; o1 = 1; o2 = 0; }
-        int err1 = e2[o1] - 1, err2 = e2[o2] - 1;
-        insert(p[0], p[1], p[2], created);
-        for (int i = 0; i < ad[a]; ++i) {
-          if (err1 > 0) { p[o1] += inc[o1]; err1 -= e2[a]; }
-          if (err2 > 0) { p[o2] += inc[o2]; err2 -= e2[a]; }
-          err1 += e2[o1];
-          err2 += e2[o2];
-          p[a] += inc[a];
-          insert(p[0], p[1], p[2], created);
-        }
-      }
+#ifdef _OPENMP
+    // Multi-threaded timing build: the walks run in parallel and their blocks
+    // are inserted afterwards (the allocated set does not depend on order).
+    std::vector<std::vector<I3>> walked(omp_get_max_threads());
+#pragma omp parallel
+    {
+      std::vector<I3>& mine = walked[omp_get_thread_num()];
+#pragma omp for schedule(static)
+      for (int y = 0; y < H; ++y)
+        for (int x = 0; x < W; ++x)
+          walk_pixel(depth, W, x, y, K, T, half, [&](int bx, int by, int bz) { mine.push_back({bx, by, bz}); });
     }
+    for (const auto& v : walked)
+      for (const I3& b : v) insert(b.x, b.y, b.z, created);
+#else
+    for (int y = 0; y < H; ++y)
+      for (int x = 0; x < W; ++x)
+        walk_pixel(depth, W, x, y, K, T, half, [&](int bx, int by, int bz) { insert(bx, by, bz, created); });
+#endif
     return created;
   }
 
@@ -573,11 +601,15 @@ template <class V> struct Map : MapBase {
   }
 
   std::vector<uint64_t> visible_impl(int W, int H, const Intr<R>& K, const Pose<R>& Tinv) const {
+    std::vector<uint64_t> keys;
+    keys.reserve(blocks.size());
+    for (const auto& kv : blocks) keys.push_back(kv.first);
+    std::vector<char> vis(keys.size());
+#pragma omp parallel for schedule(static)
+    for (long i = 0; i < (long)keys.size(); ++i) vis[i] = block_visible(unpack_key(keys[i]), Tinv, K, W, H);
     std::vector<uint64_t> out;
-    for (const auto& kv : blocks) {
-      I3 B = unpack_key(kv.first);
-      if (block_visible(B, Tinv, K, W, H)) out.push_back(kv.first);
-    }
+    for (size_t i = 0; i < keys.size(); ++i)
+      if (vis[i]) out.push_back(keys[i]);
     std::sort(out.begin(), out.end());
     return out;
   }
@@ -586,8 +618,10 @@ template <class V> struct Map : MapBase {
   // visible block.
   void integrate_impl(const float* depth, int W, int H, const Intr<R>& K, const Pose<R>& Tinv,
                       const std::vector<uint64_t>& vis) {
-    for (uint64_t key : vis) {
-      Block& blk = blocks[key];
+#pragma omp parallel for schedule(dynamic, 16)
+    for (long iv = 0; iv < (long)vis.size(); ++iv) {
+      const uint64_t key = vis[iv];
+      Block& blk = blocks.find(key)->second;
       I3 B = unpack_key(key);
       for (int lz = 0; lz < 8; ++lz)
         for (int ly = 0; ly < 8; ++ly)
@@ -809,6 +843,7 @@ template <class V> struct Map : MapBase {
     if (zmin_o)
       for (size_t i = 0; i < zmin.size(); ++i) { zmin_o[i] = zmin[i]; zmax_o[i] = zmax[i]; }
     int hits = 0;
+#pragma omp parallel for schedule(dynamic, 4) reduction(+ : hits)
     for (int y = 0; y < H; ++y)
       for (int x = 0; x < W; ++x) {
         R o[4];
@@ -942,6 +977,7 @@ template <class R> void pyr_down_vmap(const Img4<R>& s, Img4<R>& o, int W, int H
 // invz > 0 and z < cutoff; invalid pixels are (0,0,0,-1).
 template <class R> void vertex_map(const Img<R>& inv, Img4<R>& o, R invfx, R invfy, R cx, R cy, R cut) {
   o.resize(inv.w, inv.h);
+#pragma omp parallel for schedule(static)
   for (int y = 0; y < inv.h; ++y)
     for (int x = 0; x < inv.w; ++x) {
       R iz = inv.at(x, y);
@@ -954,6 +990,7 @@ template <class R> void vertex_map(const Img<R>& inv, Img4<R>& o, R invfx, R inv
 // normalize((v(y+1,x)-v) x (v(y,x+1)-v)); last row/column invalid.
 template <class R> void normal_map(const Img4<R>& v, Img4<R>& o) {
   o.resize(v.w, v.h);
+#pragma omp parallel for schedule(static)
   for (int y = 0; y < v.h; ++y)
     for (int x = 0; x < v.w; ++x) {
       auto& n = o.at(x, y);
@@ -978,6 +1015,7 @@ template <class R> void normal_map(const Img4<R>& v, Img4<R>& o) {
 template <class R> void gradient(const Img<R>& s, Img<R>& gx, Img<R>& gy) {
   gx.resize(s.w, s.h);
   gy.resize(s.w, s.h);
+#pragma omp parallel for schedule(static)
   for (int y = 1; y < s.h - 1; ++y)
     for (int x = 1; x < s.w - 1; ++x) {
       gx.at(x, y) = (s.at(x + 1, y) - s.at(x - 1, y)) * (R)0.5;
@@ -995,6 +1033,7 @@ inline int reflect101(int i, int n) {
 template <class R> void pyr_down_gauss(const Img<R>& s, Img<R>& o, int W, int H) {
   o.resize(W, H);
   const R k[5] = {(R)0.0625, (R)0.25, (R)0.375, (R)0.25, (R)0.0625};
+#pragma omp parallel for schedule(static)
   for (int y = 0; y < H; ++y)
     for (int x = 0; x < W; ++x) {
       R col[5];
@@ -1235,6 +1274,7 @@ template <class R> struct Tracker : TrackerBase {
     const R dist_th = (R)0.01;
     for (int i = 0; i < 29; ++i) s[i] = 0;
     const int cols = w[l], rows = h[l];
+#pragma omp parallel for schedule(static) reduction(+ : s[:29])
     for (int y = 0; y < rows; ++y)
       for (int x = 0; x < cols; ++x) {
         R row[7] = {0, 0, 0, 0, 0, 0, 0};
@@ -1287,6 +1327,7 @@ template <class R> struct Tracker : TrackerBase {
       t[i] = (R)Td[4 * i + 3];
     }
     const Img<R>& inv = ref_inv[l];
+#pragma omp parallel for schedule(static)
     for (int y = 0; y < h[l]; ++y)
       for (int x = 0; x < w[l]; ++x) {
         R iz = inv.at(x, y);
@@ -1307,6 +1348,7 @@ template <class R> struct Tracker : TrackerBase {
     transform_ref(l, Td);
     const int cols = w[l], rows = h[l];
     double sr = 0, cnt = 0;
+#pragma omp parallel for schedule(static) reduction(+ : sr, cnt)
     for (int y = 0; y < rows; ++y)
       for (int x = 0; x < cols; ++x) {
         const auto& p = warped[l].at(x, y);
@@ -1330,6 +1372,7 @@ template <class R> struct Tracker : TrackerBase {
     R rsum = (R)sr, n = (R)cnt;
     R mean = rsum / n;
     double sv = 0;
+#pragma omp parallel for schedule(static) reduction(+ : sv)
     for (int y = 0; y < rows; ++y)
       for (int x = 0; x < cols; ++x) {
         const auto& e = resid[l].at(x, y);
@@ -1338,6 +1381,7 @@ template <class R> struct Tracker : TrackerBase {
     R sigma = std::sqrt((R)sv / (n - (R)1));
     R th = (R)(4.685 * (double)sigma);
     for (int i = 0; i < 29; ++i) s[i] = 0;
+#pragma omp parallel for schedule(static) reduction(+ : s[:29])
     for (int y = 0; y < rows; ++y)
       for (int x = 0; x < cols; ++x) {
         const auto& e = resid[l].at(x, y);
@@ -1526,6 +1570,14 @@ int orc_map_raycast(void* m, int W, int H, const double K[4], const double T[16]
 long orc_map_mesh(void* m, int32_t* bkeys, int32_t* tcounts, int16_t* cases, double* tris, double* normals,
                   long tri_cap, int block_cap) {
   return ((MapBase*)m)->mesh(bkeys, tcounts, cases, tris, normals, tri_cap, block_cap);
+}
+// Threads the loops of this build use (1 without OpenMP).
+int orc_threads(void) {
+#ifdef _OPENMP
+  return omp_get_max_threads();
+#else
+  return 1;
+#endif
 }
 int orc_ref_hash(int x, int y, int z, int n) { return ref_hash(x, y, z, n); }
 int orc_vox_to_block(int v) { return vox_to_block(v); }

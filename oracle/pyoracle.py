@@ -15,17 +15,32 @@ LIB_PATH = os.path.join(_HERE, "_build", "liboracle.so")
 VOX_REF, VOX_F32, VOX_F16 = 0, 1, 2
 MODE_RGB, MODE_DEPTH, MODE_RGBD = 0, 1, 2
 
+LIB_OMP_PATH = os.path.join(_HERE, "_build", "liboracle_omp.so")
+
 _lib = None
+_lib_omp = None
 P = ctypes.c_void_p
 
 
 def lib():
     global _lib
-    if _lib is not None:
-        return _lib
-    if not os.path.exists(LIB_PATH):
+    if _lib is None:
+        _lib = _load(LIB_PATH)
+    return _lib
+
+
+def lib_omp():
+    """The OpenMP build (all host cores), for timing the CPU baseline only."""
+    global _lib_omp
+    if _lib_omp is None:
+        _lib_omp = _load(LIB_OMP_PATH)
+    return _lib_omp
+
+
+def _load(path):
+    if not os.path.exists(path):
         raise RuntimeError("oracle not built: run __graft_entry__.build()")
-    L = ctypes.CDLL(LIB_PATH)
+    L = ctypes.CDLL(path)
     sig = {
         "orc_map_new": (P, [ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_int,
                             ctypes.c_double, ctypes.c_double]),
@@ -39,6 +54,7 @@ def lib():
         "orc_map_raycast": (ctypes.c_int, [P, ctypes.c_int, ctypes.c_int, P, P, P, P, P]),
         "orc_map_mesh": (ctypes.c_long, [P, P, P, P, P, P, ctypes.c_long, ctypes.c_int]),
         "orc_ref_hash": (ctypes.c_int, [ctypes.c_int] * 4),
+        "orc_threads": (ctypes.c_int, []),
         "orc_vox_to_block": (ctypes.c_int, [ctypes.c_int]),
         "orc_vox_to_local": (ctypes.c_int, [ctypes.c_int]),
         "orc_pack_sdf": (ctypes.c_int16, [ctypes.c_float]),
@@ -74,7 +90,6 @@ def lib():
         f = getattr(L, name)
         f.restype = res
         f.argtypes = args
-    _lib = L
     return L
 
 
@@ -290,20 +305,22 @@ class OracleTracker:
 
 
 class OraclePipeline:
-    """Tracking::TrackRGBD replayed on one volume (see oracle.cpp)."""
+    """Tracking::TrackRGBD replayed on one volume (see oracle.cpp).
+    all_cores: run on the OpenMP build (timing only; its sums add in another
+    order)."""
 
     def __init__(self, W, H, K, voxel, trunc, levels, iters, mode, weights=(1, 1, 100, 10), precision=32,
-                 voxfmt=VOX_F32, max_weight=255, depth_scale=1000.0, gray_mode=0):
+                 voxfmt=VOX_F32, max_weight=255, depth_scale=1000.0, gray_mode=0, all_cores=False):
         it = np.array(list(iters) + [0] * (8 - len(iters)), np.int32)
         w = np.array(weights, np.float32)
-        self.h = lib().orc_pipe_new(precision, voxfmt, voxel, trunc, max_weight, W, H, _p(_f64(K)), levels,
-                                    _p(it), mode, _p(w), depth_scale, gray_mode)
+        self.L = lib_omp() if all_cores else lib()
+        self.h = self.L.orc_pipe_new(precision, voxfmt, voxel, trunc, max_weight, W, H, _p(_f64(K)), levels,
+                                     _p(it), mode, _p(w), depth_scale, gray_mode)
         self.W, self.H = W, H
 
     def __del__(self):
-        # at interpreter exit the module's globals may already be gone
-        if getattr(self, "h", None) and _lib is not None:
-            _lib.orc_pipe_free(self.h)
+        if getattr(self, "h", None) and getattr(self, "L", None) is not None:
+            self.L.orc_pipe_free(self.h)
             self.h = None
 
     def step(self, depth_u16, rgb):
@@ -311,5 +328,5 @@ class OraclePipeline:
         c = np.ascontiguousarray(rgb, np.uint8)
         T = np.zeros((4, 4), np.float64)
         times = np.zeros(4, np.float64)
-        ok = lib().orc_pipe_step(self.h, _p(d), _p(c), _p(T), _p(times))
+        ok = self.L.orc_pipe_step(self.h, _p(d), _p(c), _p(T), _p(times))
         return T, bool(ok), times

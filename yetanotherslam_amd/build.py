@@ -75,16 +75,21 @@ def build_synth(verbose=False):
 
 
 def build_oracle(verbose=False):
+    """liboracle.so (single thread: the checker) and liboracle_omp.so (the
+    same source with OpenMP: the all-cores CPU baseline's timing only)."""
     odir = os.path.join(REPO, "oracle", "_build")
     os.makedirs(odir, exist_ok=True)
-    out = os.path.join(odir, "liboracle.so")
-    tmp = out + ".tmp"
-    _run(["g++", "-O2", "-std=c++17", "-ffp-contract=off", "-fno-fast-math", "-fPIC", "-shared", "-o", tmp,
-          os.path.join(REPO, "oracle", "oracle.cpp")])
-    os.replace(tmp, out)
-    if verbose:
-        print("built", out)
-    return out
+    outs = []
+    for name, extra in (("liboracle.so", []), ("liboracle_omp.so", ["-fopenmp"])):
+        out = os.path.join(odir, name)
+        tmp = out + ".tmp"
+        _run(["g++", "-O2", "-std=c++17", "-ffp-contract=off", "-fno-fast-math", "-fPIC", "-shared"] + extra +
+             ["-o", tmp, os.path.join(REPO, "oracle", "oracle.cpp")])
+        os.replace(tmp, out)
+        outs.append(out)
+        if verbose:
+            print("built", out)
+    return outs[0]
 
 
 def build_facade_replay(verbose=False):
