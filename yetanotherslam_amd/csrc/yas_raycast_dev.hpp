@@ -160,6 +160,15 @@ struct VoxelReader {
   // vox_to_block except below INT32_MIN + 7 (a saturated coordinate), where
   // both lie outside the key range and the lookup finds nothing.
   __device__ static inline int block_of(int v) { return v >> 3; }
+  // The cached block answers every read alike: absent, or summarised as
+  // all-unobserved or all-free (read_at's early returns).
+  __device__ inline bool cached_uniform() const {
+    return cblk < 0 || cflags == kBlockFree || cflags == kBlockUnobserved;
+  }
+  __device__ inline bool in_cached_block(float x, float y, float z) const {
+    return block_of(voxel_of<kInRange>(x)) == cbx && block_of(voxel_of<kInRange>(y)) == cby &&
+           block_of(voxel_of<kInRange>(z)) == cbz;
+  }
   // Read of local voxel `l` of block (bx, by, bz) through the one-entry cache.
   __device__ inline float read_at(int bx, int by, int bz, int l, bool& valid) {
     if (bx != cbx || by != cby || bz != cbz) {
@@ -195,12 +204,25 @@ __device__ inline bool march_ray(const MapView& m, float& rx, float& ry, float& 
   while (dist_s < dist_e) {
     last = sdf;
     sdf = rd.read(rx, ry, rz, valid);
-    if (sdf <= 0.5f && sdf >= -0.5f) sdf = rd.interp_after_read(rx, ry, rz, valid);
+    const bool near = sdf <= 0.5f && sdf >= -0.5f;
+    if (near) sdf = rd.interp_after_read(rx, ry, rz, valid);
     if (sdf <= 0.0f) break;
     if (sdf >= 0.0f && last < 0.0f) return false;  // abandoned
     float step = valid ? fmaxf(sdf * step_scale, 1.0f) : 1.0f;
-    rx = rx + step * dx; ry = ry + step * dy; rz = rz + step * dz;
+    const float sdx = step * dx, sdy = step * dy, sdz = step * dz;
+    rx = rx + sdx; ry = ry + sdy; rz = rz + sdz;
     dist_s = dist_s + step;
+    // When this sample was answered by a block that answers every read with
+    // the same (1, valid) (no trilinear read, so the cache still holds the
+    // sample's block), each further sample inside it would repeat this
+    // iteration exactly (sdf 1, last 1, the same step): take those steps
+    // without the reads until the ray leaves the block or its range.
+    if (!near && rd.cached_uniform()) {
+      while (dist_s < dist_e && rd.in_cached_block(rx, ry, rz)) {
+        rx = rx + sdx; ry = ry + sdy; rz = rz + sdz;
+        dist_s = dist_s + step;
+      }
+    }
   }
   if (!(sdf <= 0.0f)) return false;
   float step = sdf * step_scale;
