@@ -29,6 +29,7 @@ from yetanotherslam_amd.workloads import (BLOCKS, BUCKETS, ITERS, LEVELS, THETA0
                                           H, W)
 
 HBM_PEAK_GBS = 8000.0
+DETAIL_EVERY = 4  # timed frames per frame sampled for stage and kernel times
 STAGES = ["convert", "visibility+zrange", "raymarch", "tracker_inputs", "gauss_newton", "alloc", "visible",
           "integrate"]
 
@@ -97,33 +98,42 @@ def walk_steps(stats):
     return float(d.sum() * len(c) / max(len(d), 1))
 
 
-def rooflines(stats, stage_ms, steps, traffic):
-    """Achieved algorithmic HBM bandwidth of the single-kernel stages (bytes
-    per launch as defined for each kernel below, divided by the kernel's
-    average duration measured with HIP events around its launches)."""
+def rooflines(stats, kernel_ms, stage_ms, steps, traffic):
+    """Achieved algorithmic HBM bandwidth of the single-kernel stages: bytes
+    per launch as defined for each kernel below (averaged over the timed
+    frames), divided by the kernel's average duration (kernel_ms, per launch)
+    from HIP events that each launch records at the start and the end of its
+    own execution on the sampled timed frames (yas_pipeline_kernel_timing).
+    The stage time between events recorded around the launch (stage_ms, per
+    frame), which adds the dispatch, is given beside it."""
     sv, se, sd = 8, 16, 4            # voxel bytes (f32 sdf + f32 weight), hash slot bytes, depth bytes
     nvis_fuse = stats[:, 1].astype(np.float64)
     nvis_ray = stats[:, 4].astype(np.float64)
     cells = (W // 8) * (H // 8)
+    nalloc = stats[:, 0].astype(np.float64)
     defs = {
         # every voxel of every visible block read and written, the depth
         # image, and the block's list entry + key + slot index
-        "k_integrate": (7, float(np.sum(nvis_fuse * (512 * 2 * sv + 4 + 8 + 4) + W * H * sd))),
+        "k_integrate": (3, 7, float(np.sum(nvis_fuse * (512 * 2 * sv + 4 + 8 + 4) + W * H * sd))),
         # compulsory ray-cast traffic: each visible block's voxels and hash
         # slot once, the vertex map written, the two z-range maps read
-        "k_march": (2, float(np.sum(nvis_ray * (512 * sv + se) + W * H * 16 + 2 * cells * 4))),
+        "k_march": (0, 2, float(np.sum(nvis_ray * (512 * sv + se) + W * H * 16 + 2 * cells * 4))),
         # the depth image read and one hash-slot probe per Bresenham visit
         # (the walk counter is cumulative and wraps at 2^32)
-        "k_alloc": (5, float(steps * W * H * sd + walk_steps(stats) * se)),
+        "k_alloc": (1, 5, float(steps * W * H * sd + walk_steps(stats) * se)),
+        # the dense block list (key) scanned, each visible block's index
+        # written
+        "k_visible": (2, 6, float(np.sum(nalloc * 8 + nvis_fuse * 4))),
     }
     out = {}
-    for name, (i, nbytes) in defs.items():
-        t = stage_ms[i] / 1e3
-        ach = nbytes / t / 1e9 if t > 0 else 0.0
+    for name, (k, i, nbytes) in defs.items():
+        per_launch = nbytes / steps
+        t = kernel_ms[k] / 1e3
+        ach = per_launch / t / 1e9 if t > 0 else 0.0
         out[name] = {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": traffic.get(name),
-                     "algorithmic_bytes_per_launch": round(nbytes / steps),
-                     "avg_launch_ms": round(stage_ms[i] / steps, 5)}
+                     "algorithmic_bytes_per_launch": round(per_launch),
+                     "avg_launch_ms": round(kernel_ms[k], 5), "avg_stage_ms": round(stage_ms[i], 5)}
     return out
 
 
@@ -281,9 +291,19 @@ def serving_bench(dev_depth, dev_rgb, nseq, frames_sat=60, sweep_loads=(1500, 30
 
     def plain_matched(hires, fmt, start):
         """The saturation run's frames through one single-stream pipeline per
-        stream (created before its timing starts)."""
+        stream (created before its timing starts).  Also the batched
+        kernels' algorithmic bytes over all of these frames, from each
+        pipeline's per-frame visible counts (a served stream is
+        bit-identical to its own pipeline, so the counts are the served
+        ones): the ray march reads each visible block's voxels and hash slot
+        and writes the vertex map (every frame after a stream's first), and
+        integration reads and writes each visible block's voxels plus its
+        list entry, key and slot index, and reads the depth image."""
         total = 0.0
+        sv_bytes = 8 if fmt == rf.VOXEL_F32_F32 else 4
+        nbytes = {"k_s_march": 0.0, "k_s_integrate": 0.0}
         for i in range(n_streams):
+            w, h, _ = dims(hires, i)
             pl = rf.DensePipeline(*dims(hires, i), map_params(fmt), tp)
             torch.cuda.synchronize()
             t0 = time.perf_counter()
@@ -291,8 +311,15 @@ def serving_bench(dev_depth, dev_rgb, nseq, frames_sat=60, sweep_loads=(1500, 30
                 pl.step(*frame(hires, i, int(start[i]) + k))
             torch.cuda.synchronize()
             total += time.perf_counter() - t0
+            st = np.zeros((frames_sat, 8), np.int32)
+            rf.native.call("yas_pipeline_frame_stats", pl.h, 0, frames_sat, st.ctypes.data)
+            cells = (w // 8) * (h // 8)
+            nbytes["k_s_march"] += float(np.sum(st[1:, 4].astype(np.float64) * (512 * sv_bytes + 16)
+                                                + w * h * 16 + 2 * cells * 4))
+            nbytes["k_s_integrate"] += float(np.sum(st[:, 1].astype(np.float64) * (512 * 2 * sv_bytes + 16)
+                                                    + w * h * 4))
             pl.close()
-        return n_streams * frames_sat / total
+        return n_streams * frames_sat / total, nbytes
 
     out = {"streams": n_streams, "streams_640x480": int(mixed.sum()), "streams_320x240": int((~mixed).sum())}
     start = wl.serving_starts(rng, nseq, frames_sat)
@@ -314,12 +341,14 @@ def serving_bench(dev_depth, dev_rgb, nseq, frames_sat=60, sweep_loads=(1500, 30
         st = sv.stats()
         sv.close()
         sat = len(res) / t_sat
-        plain = plain_matched(hires, fmt, start)
+        plain, nbytes = plain_matched(hires, fmt, start)
+        batches = max(st["batches_done"], 1)
         out[name] = {"frames": len(res), "frames_per_s": round(sat, 1),
                      "tracked": int(sum(r["tracked"] for r in res)),
-                     "mean_batch": round(st["frames_done"] / max(st["batches_done"], 1), 1),
+                     "mean_batch": round(st["frames_done"] / batches, 1),
                      "hbm_bytes_in_use": int(st["bytes_in_use"]),
-                     "plain_same_frames_per_s": round(plain, 1), "vs_plain_single_stream": round(sat / plain, 2)}
+                     "plain_same_frames_per_s": round(plain, 1), "vs_plain_single_stream": round(sat / plain, 2),
+                     "algorithmic_bytes_per_batch": {k: round(v / batches) for k, v in nbytes.items()}}
         log("serving", name, out[name])
     if runs and "poisson" not in runs:
         return out
@@ -443,7 +472,9 @@ def main():
         pl.step(dev_depth[t], dev_rgb[t])
     torch.cuda.synchronize()
 
-    lib.call("yas_pipeline_enable_timing", pl.h, 1)
+    # every frame's start and end, and every DETAIL-th frame's stage
+    # boundaries and single-kernel durations (fewer events, less overhead)
+    lib.call("yas_pipeline_enable_timing", pl.h, DETAIL_EVERY)
     torch.cuda.synchronize()
     log("timing %d frames" % args.steps)
     t0 = time.perf_counter()
@@ -456,14 +487,18 @@ def main():
     frame_ms = np.zeros(args.steps)
     nt = rf.ctypes.c_int()
     lib.call("yas_pipeline_timing", pl.h, stage.ctypes.data, frame_ms.ctypes.data, args.steps, rf.ctypes.byref(nt))
+    kernel = np.zeros(4)
+    ns = rf.ctypes.c_int()
+    lib.call("yas_pipeline_kernel_timing", pl.h, kernel.ctypes.data, rf.ctypes.byref(ns))
+    sampled = max(ns.value, 1)
     stats = np.zeros((args.steps, 8), np.int32)
     lib.call("yas_pipeline_frame_stats", pl.h, args.warmup + 1, args.steps, stats.ctypes.data)
     tracked = sum(pl.pose(t)[1] for t in range(args.warmup + 1, n))
 
     fps = args.steps / wall
-    per_stage = {k: round(v / args.steps, 4) for k, v in zip(STAGES, stage)}
+    per_stage = {k: round(v / sampled, 4) for k, v in zip(STAGES, stage)}
     traffic = TRAFFIC_BYTES.get((args.warmup, args.steps), {})
-    rl = rooflines(stats, stage, args.steps, traffic)
+    rl = rooflines(stats, kernel / sampled, stage / sampled, args.steps, traffic)
     dominant = max(rl, key=lambda k: rl[k]["avg_launch_ms"])
     log("stages ms/frame:", per_stage)
     log("visible blocks/frame: fuse %.0f, ray cast %.0f; allocated at end %d; tracked %d/%d" % (

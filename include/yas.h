@@ -493,17 +493,26 @@ int yas_pipeline_step(yas_pipeline* pl, const uint16_t* depth, size_t depth_pitc
  * (1) or rejected (0).  Synchronous. */
 int yas_pipeline_pose(yas_pipeline* pl, int frame, double T[16], int* tracked);
 int yas_pipeline_num_frames(yas_pipeline* pl, int* n);
-/* Per-stage device timing with HIP events on the context's stream (the
- * reference only brackets TrackRGBD with std::chrono, Examples/tum_rgbd.cpp:56-60).
- * While enabled, every frame after the first records its stage boundaries;
- * yas_pipeline_timing synchronises and accumulates them: stage_ms[8] =
- * {input conversion, visibility + z-range, ray march, tracker inputs
- * (model + frame pyramids), Gauss-Newton solve, block allocation, block
- * visibility, TSDF integration}, frame_ms = whole-frame time of each timed
- * frame (at most `capacity`), *num_frames = frames timed so far. */
-int yas_pipeline_enable_timing(yas_pipeline* pl, int enable);
+/* Device timing with HIP events on the context's stream (the reference only
+ * brackets TrackRGBD with std::chrono, Examples/tum_rgbd.cpp:56-60).  With
+ * detail_every = n > 0, every frame after the first records its start and
+ * end, and every n-th of them (the first included) also its stage
+ * boundaries and the kernel events below; 0 stops timing.
+ * yas_pipeline_timing synchronises and accumulates them: stage_ms[8] = sums
+ * over the sampled frames of {input conversion, visibility + z-range, ray
+ * march, tracker inputs (model + frame pyramids), Gauss-Newton solve, block
+ * allocation, block visibility, TSDF integration}, frame_ms = whole-frame
+ * time of each timed frame (at most `capacity`), *num_frames = frames timed
+ * so far. */
+int yas_pipeline_enable_timing(yas_pipeline* pl, int detail_every);
 int yas_pipeline_timing(yas_pipeline* pl, double stage_ms[8], double* frame_ms, int capacity, int* num_frames);
 int yas_pipeline_reset_timing(yas_pipeline* pl);
+/* Total duration, over the frames sampled for detail, of each of the four
+ * single-kernel stages' kernels {ray march, block allocation, block
+ * visibility, TSDF integration}, from events that each launch records at
+ * the start and the end of the kernel's own execution (hipExtLaunchKernel),
+ * and the number of sampled frames. Synchronous. */
+int yas_pipeline_kernel_timing(yas_pipeline* pl, double kernel_ms[4], int* frames_sampled);
 /* Counters of frames [first, first+count): 8 ints per frame {allocated
  * blocks, visible blocks at fusion, capacity flags, Bresenham walk steps
  * (cumulative), visible blocks at the ray cast, 0, 0, 0}. Synchronous. */

@@ -465,3 +465,53 @@ def test_plain_method_sequence_against_oracle():
         print("plain method: blocks", len(kg), name, len(ko), "common", len(common), "tsdf rms (voxels)", rms)
         assert len(common) >= 0.999 * max(len(kg), len(ko)) and len(common) > 5000
         assert rms <= 0.1, (name, rms)
+
+
+def test_depth_only_icp_sequence_against_oracle():
+    """Depth alone, where the ICP term (RGBDTracking.cu:305-406) does all the
+    work: 12 frames at 640x480 of noise-free depth, the camera moving 3 mm a
+    frame, 8 mm voxels, truncation 3 voxels, 3 levels with 10/5/3 iterations.
+    At the true motion two thirds of the valid level-0 pixels pass the 1 cm /
+    20 degree association gates (against 7 % with the default sensor noise,
+    whose per-pixel normals fail the angle gate), and the float64 oracle
+    stays within 1 mm of the generator's poses.  Every frame's motion and
+    chained pose on the GPU is within 1e-4 m / 1e-4 rad of the float32 and
+    of the float64 oracle."""
+    n, W, H = 12, 640, 480
+    vox, mu = 0.008, 0.024
+    p, K, fr = frames(n, W, H, step=0.003, noise_k=0.0)
+    tr = po.OracleTracker(W, H, K, 1, (1,), po.MODE_DEPTH, precision=64)
+    tr.set_reference_depth(po.depth_u16_to_f32(fr[0][0]))
+    tr.set_tracking_depth(po.depth_u16_to_f32(fr[1][0]))
+    true_dt = np.linalg.inv(np.linalg.inv(fr[0][2]) @ fr[1][2])
+    sums, _ = tr.step_sums(0, true_dt, 0)
+    valid = np.count_nonzero(fr[1][0])
+    print("icp sequence: inliers at the true motion", int(sums[28]), "of", valid, "valid pixels")
+    assert sums[28] >= 0.6 * valid
+
+    mp = rf.map_params(1 << 20, 1 << 17, vox, mu, rf.VOXEL_F32_F32)
+    tp = rf.tracker_params(3, (10, 5, 3), rf.TRACK_DEPTH)
+    pl = rf.DensePipeline(W, H, K, mp, tp)
+    o32 = po.OraclePipeline(W, H, K, vox, mu, 3, (10, 5, 3), po.MODE_DEPTH)
+    o64 = po.OraclePipeline(W, H, K, vox, mu, 3, (10, 5, 3), po.MODE_DEPTH, precision=64)
+    T0 = fr[0][2]
+    poses = []
+    for t in range(n):
+        d, rgb, Tt = fr[t]
+        pl.step(to_dev(d), to_dev(rgb))
+        T32, ok32, _ = o32.step(d, rgb)
+        T64, ok64, _ = o64.step(d, rgb)
+        Tg, okg = pl.pose(t)
+        assert okg and ok32 and ok64, t
+        assert pose_err(T64, np.linalg.inv(T0) @ Tt)[0] <= 1e-3, t
+        poses.append((Tg, T32, T64))
+    worst = 0.0
+    for t in range(1, n):
+        for j in (1, 2):
+            mg = np.linalg.inv(poses[t - 1][0]) @ poses[t][0]
+            mo = np.linalg.inv(poses[t - 1][j]) @ poses[t][j]
+            e = max(pose_err(mg, mo) + pose_err(poses[t][0], poses[t][j]))
+            worst = max(worst, e)
+            assert e <= 1e-4, (t, j, e)
+    print("icp sequence: worst gpu-oracle pose error %.2e" % worst)
+    pl.close()

@@ -151,3 +151,30 @@ def test_server_fuses_at_the_last_pose_after_nan_updates():
     assert_bits_equal(sg, so, "tsdf")
     assert_bits_equal(wg, wo, "weight")
     sv.close()
+
+
+def test_pipeline_timing_samples_stages_and_kernels():
+    """yas_pipeline_enable_timing(n): every timed frame's start and end, and
+    every n-th frame's stage boundaries and single-kernel durations."""
+    import ctypes
+    from yetanotherslam_amd import native
+    n, w, h = 7, 160, 120
+    p, K, seq = nan_sequence(n, w, h)
+    pl = rf.DensePipeline(w, h, K, rf.map_params(1 << 16, 1 << 14, 0.01, 0.03), rf.tracker_params())
+    pl.step(*(to_dev(a) for a in seq[0]))
+    native.call("yas_pipeline_enable_timing", pl.h, 2)
+    for d, c in seq[1:]:
+        pl.step(to_dev(d), to_dev(c))
+    stage = np.zeros(8)
+    frame = np.zeros(8)
+    nt = ctypes.c_int()
+    native.call("yas_pipeline_timing", pl.h, stage.ctypes.data, frame.ctypes.data, 8, ctypes.byref(nt))
+    kern = np.zeros(4)
+    ns = ctypes.c_int()
+    native.call("yas_pipeline_kernel_timing", pl.h, kern.ctypes.data, ctypes.byref(ns))
+    assert nt.value == n - 1 and ns.value == 3  # frames 1..6 timed, 1, 3, 5 sampled
+    assert np.all(frame[:n - 1] > 0) and np.all(stage > 0) and np.all(kern > 0)
+    # each kernel runs inside its stage
+    for k, i in ((0, 2), (1, 5), (2, 6), (3, 7)):
+        assert kern[k] <= stage[i] * 1.05 + 1e-3, (k, kern[k], stage[i])
+    pl.close()

@@ -166,9 +166,16 @@ __device__ inline void integrate_waves(const MapView& m, const PoseF& Ti, const 
   const int lx = lane & 7, ly = lane >> 3;
   const float vs = m.voxel_size, mu = m.trunc;
   typename VT::T* vox = (typename VT::T*)m.voxels;
+  // the next block's index and key are loaded while this one is updated
+  int nblk = wave < nvis ? m.visible[wave] : 0;
+  unsigned long long nkey = wave < nvis ? m.block_keys[nblk] : 0ull;
   for (int w = wave; w < nvis; w += nwaves) {
-    const int blk = m.visible[w];
-    const unsigned long long key = m.block_keys[blk];
+    const int blk = nblk;
+    const unsigned long long key = nkey;
+    if (w + nwaves < nvis) {
+      nblk = m.visible[w + nwaves];
+      nkey = m.block_keys[nblk];
+    }
     typename VT::T* bv = vox + (size_t)blk * 512;
     // the block's 8 slices are loaded together, then each pixel's depth is
     // read (at a clamped pixel, used only when in the image) before any update

@@ -10,6 +10,7 @@
 #pragma once
 
 #include <hip/hip_runtime.h>
+#include <hip/hip_ext.h>
 
 #include <atomic>
 #include <cstdint>
@@ -57,6 +58,16 @@ void yas_set_error(const char* fmt, ...);
       return YAS_E_INVALID;                                                                        \
     }                                                                                              \
   } while (0)
+
+// A kernel launch on stream s.  Given a pair of events, the launch records
+// them at the start and the end of the kernel's own execution
+// (hipExtLaunchKernel), which times the kernel without the dispatch gaps of
+// events recorded around it.
+template <typename F, typename... Args>
+inline void launch_timed(F kernel, dim3 grid, dim3 block, hipStream_t s, const hipEvent_t* ev, Args... args) {
+  if (ev) hipExtLaunchKernelGGL(kernel, grid, block, 0, s, ev[0], ev[1], 0, args...);
+  else hipLaunchKernelGGL(kernel, grid, block, 0, s, args...);
+}
 
 struct yas_ctx {
   int device;

@@ -200,19 +200,22 @@ static int grid_for(long n, int per, int cap) {
   return (int)g;
 }
 
-static int launch_integrate(yas_map* map, const float* depth, size_t pitch, int W, int H) {
+static int launch_integrate(yas_map* map, const float* depth, size_t pitch, int W, int H,
+                            const hipEvent_t* ev = nullptr) {
   hipStream_t s = map->ctx->stream;
-  // enough waves to cover a large visible set; excess waves exit at once
-  int grid = grid_for(map->P.num_blocks, 4, 2048);
+  // enough wavefronts for one visible block each up to 32768 visible blocks
+  // (those past the visible count return at once)
+  dim3 grid(grid_for(map->P.num_blocks, 4, 8192));
+  const PosePair* pose = map->pose_dev;
   switch (map->P.voxel_format) {
     case YAS_VOXEL_I16_U8:
-      hipLaunchKernelGGL(k_integrate<VoxI16>, dim3(grid), dim3(256), 0, s, map->v, map->pose_dev, map->K, depth, pitch, W, H);
+      launch_timed(k_integrate<VoxI16>, grid, dim3(256), s, ev, map->v, pose, map->K, depth, pitch, W, H);
       break;
     case YAS_VOXEL_F16_U16:
-      hipLaunchKernelGGL(k_integrate<VoxF16>, dim3(grid), dim3(256), 0, s, map->v, map->pose_dev, map->K, depth, pitch, W, H);
+      launch_timed(k_integrate<VoxF16>, grid, dim3(256), s, ev, map->v, pose, map->K, depth, pitch, W, H);
       break;
     default:
-      hipLaunchKernelGGL(k_integrate<VoxF32>, dim3(grid), dim3(256), 0, s, map->v, map->pose_dev, map->K, depth, pitch, W, H);
+      launch_timed(k_integrate<VoxF32>, grid, dim3(256), s, ev, map->v, pose, map->K, depth, pitch, W, H);
   }
   YAS_LAUNCH_CHECK();
   return YAS_OK;
@@ -222,21 +225,23 @@ static int launch_integrate(yas_map* map, const float* depth, size_t pitch, int 
 // The three fusion stages with the pose already in pose_dev.  A non-null
 // `gate` pointing at 0 turns allocation and visibility into no-ops on the
 // device (integration then sees no visible blocks).
-int yas_map_alloc_stage(yas_map* map, const float* depth, size_t pitch, int W, int H, const int* gate) {
+int yas_map_alloc_stage(yas_map* map, const float* depth, size_t pitch, int W, int H, const int* gate,
+                        const hipEvent_t* ev = nullptr) {
   dim3 g((W + 15) / 16, (H + 15) / 16);
-  hipLaunchKernelGGL(k_alloc, g, dim3(256), 0, map->ctx->stream, map->v, depth, pitch, W, H, map->K,
-                     (const PosePair*)map->pose_dev, gate);
+  launch_timed(k_alloc, g, dim3(256), map->ctx->stream, ev, map->v, depth, pitch, W, H, map->K,
+               (const PosePair*)map->pose_dev, gate);
   YAS_LAUNCH_CHECK();
   return YAS_OK;
 }
-int yas_map_visible_stage(yas_map* map, int W, int H, const int* gate) {
-  hipLaunchKernelGGL(k_visible, dim3(grid_for(map->P.num_blocks, 256, 4096)), dim3(256), 0, map->ctx->stream, map->v,
-                     (const PosePair*)map->pose_dev, map->K, W, H, gate);
+int yas_map_visible_stage(yas_map* map, int W, int H, const int* gate, const hipEvent_t* ev = nullptr) {
+  launch_timed(k_visible, dim3(grid_for(map->P.num_blocks, 256, 4096)), dim3(256), map->ctx->stream, ev, map->v,
+               (const PosePair*)map->pose_dev, map->K, W, H, gate);
   YAS_LAUNCH_CHECK();
   return YAS_OK;
 }
-int yas_map_integrate_stage(yas_map* map, const float* depth, size_t pitch, int W, int H) {
-  return launch_integrate(map, depth, pitch, W, H);
+int yas_map_integrate_stage(yas_map* map, const float* depth, size_t pitch, int W, int H,
+                            const hipEvent_t* ev = nullptr) {
+  return launch_integrate(map, depth, pitch, W, H, ev);
 }
 int yas_map_fuse_prepared(yas_map* map, const float* depth, size_t pitch, int W, int H, const int* gate) {
   int rc = yas_map_alloc_stage(map, depth, pitch, W, H, gate);

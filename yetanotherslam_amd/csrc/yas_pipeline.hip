@@ -17,10 +17,12 @@ int yas_map_fuse_prepared(yas_map* map, const float* depth, size_t pitch, int W,
 int yas_map_prepare_pose_dev(yas_map* map, const double* T_dev);
 int yas_convert_depth_u16(yas_ctx* ctx, const uint16_t* src, size_t sp, float* dst, size_t dp, int W, int H,
                           float depth_scale);
-int yas_raycast_dev(yas_raycaster* rc, yas_map* map, const double* T_dev, hipEvent_t before_march);
-int yas_map_alloc_stage(yas_map* map, const float* depth, size_t pitch, int W, int H, const int* gate);
-int yas_map_visible_stage(yas_map* map, int W, int H, const int* gate);
-int yas_map_integrate_stage(yas_map* map, const float* depth, size_t pitch, int W, int H);
+int yas_raycast_dev(yas_raycaster* rc, yas_map* map, const double* T_dev, hipEvent_t before_march,
+                    const hipEvent_t* march_ev);
+int yas_map_alloc_stage(yas_map* map, const float* depth, size_t pitch, int W, int H, const int* gate,
+                        const hipEvent_t* ev);
+int yas_map_visible_stage(yas_map* map, int W, int H, const int* gate, const hipEvent_t* ev);
+int yas_map_integrate_stage(yas_map* map, const float* depth, size_t pitch, int W, int H, const hipEvent_t* ev);
 MapView yas_map_view(yas_map* map);
 float4* yas_raycaster_vmap_dev(yas_raycaster* rc);
 int yas_tracker_solve_async(yas_tracker* t);
@@ -67,15 +69,24 @@ __global__ void k_set_identity(double* T) {
 
 }  // namespace
 
-// Stage boundaries recorded per frame while timing is on.
+// Events of a timed frame: its start and end (ev[0], ev[kStages]); on the
+// frames sampled for detail also the stage boundaries in between and a
+// start/stop pair recorded by each single-kernel stage's own launch.
 constexpr int kStages = 8;  // conversion, visibility+z-range, march, tracker inputs, solve, alloc, visible, integrate
+constexpr int kTimedKernels = 4;  // k_march, k_alloc, k_visible, k_integrate
+constexpr int kEvents = kStages + 1 + 2 * kTimedKernels;
 
 struct yas_pipeline {
   yas_ctx* ctx;
   bool timing;
-  std::vector<std::vector<hipEvent_t>> pending;  // kStages + 1 events per timed frame
+  std::vector<std::vector<hipEvent_t>> pending;  // kEvents events per timed frame
+  std::vector<bool> pending_detail;              // whether that frame was sampled for detail
   std::vector<std::vector<hipEvent_t>> spare;
+  int detail_every;        // every n-th timed frame records the stage and kernel events
+  long long timed_count;   // frames timed since timing was enabled
+  int sampled;             // frames whose stage and kernel times are in stage_ms / kernel_ms
   double stage_ms[kStages];
+  double kernel_ms[kTimedKernels];
   std::vector<double> frame_ms;
   int* frame_stats;  // [cap][8]: allocated, visible (fuse), flags, walk steps, visible (ray cast)
   int W, H;
@@ -164,7 +175,11 @@ int yas_pipeline_create(yas_ctx* ctx, int W, int H, const float K[4], const yas_
   yas_pipeline* p = new yas_pipeline();
   p->ctx = yas_ctx_retain(ctx);
   p->timing = false;
+  p->detail_every = 1;
+  p->timed_count = 0;
+  p->sampled = 0;
   std::memset(p->stage_ms, 0, sizeof(p->stage_ms));
+  std::memset(p->kernel_ms, 0, sizeof(p->kernel_ms));
   p->map = nullptr; p->rc = nullptr; p->trk = nullptr;
   p->depth = p->gray = nullptr;
   p->T_last = p->poses = nullptr;
@@ -232,22 +247,29 @@ int yas_pipeline_step(yas_pipeline* p, const uint16_t* depth, size_t dpitch, con
   const size_t fp = (size_t)W * sizeof(float);
   hipStream_t s = p->ctx->stream;
   // events of this frame (timed frames after the first only)
-  hipEvent_t ev[kStages + 1] = {};
+  hipEvent_t ev[kEvents] = {};
   const bool timed = p->timing && p->frames > 0;
+  bool detail = false;
   if (timed) {
+    detail = p->timed_count % p->detail_every == 0;
+    p->timed_count++;
     std::vector<hipEvent_t> set;
     if (!p->spare.empty()) {
       set = std::move(p->spare.back());
       p->spare.pop_back();
     } else {
-      set.resize(kStages + 1);
+      set.resize(kEvents);
       for (auto& e : set) YAS_HIP(hipEventCreate(&e));
     }
-    for (int i = 0; i <= kStages; ++i) ev[i] = set[i];
+    for (int i = 0; i < kEvents; ++i) ev[i] = set[i];
     p->pending.push_back(std::move(set));
+    p->pending_detail.push_back(detail);
   }
+  // start/stop pairs of the kernels timed by their own launch
+  const hipEvent_t* kev = detail ? ev + kStages + 1 : nullptr;
+  auto kpair = [&](int k) -> const hipEvent_t* { return kev ? kev + 2 * k : nullptr; };
   auto mark = [&](int i) -> int {
-    if (timed) YAS_HIP(hipEventRecord(ev[i], s));
+    if (timed && (detail || i == 0 || i == kStages)) YAS_HIP(hipEventRecord(ev[i], s));
     return YAS_OK;
   };
   if ((rc = mark(0))) return rc;
@@ -266,7 +288,7 @@ int yas_pipeline_step(yas_pipeline* p, const uint16_t* depth, size_t dpitch, con
     YAS_LAUNCH_CHECK();
   } else {
     // TrackRGBD (Tracking.cpp:162-197)
-    if ((rc = yas_raycast_dev(p->rc, p->map, p->T_last, ev[2]))) return rc;
+    if ((rc = yas_raycast_dev(p->rc, p->map, p->T_last, detail ? ev[2] : nullptr, kpair(0)))) return rc;
     if ((rc = mark(3))) return rc;
     if ((rc = yas_tracker_prepare_frame_dev(p->trk, (const float*)yas_raycaster_vmap_dev(p->rc), (size_t)W * 16,
                                             p->gray, fp, p->depth, fp)))
@@ -286,11 +308,11 @@ int yas_pipeline_step(yas_pipeline* p, const uint16_t* depth, size_t dpitch, con
     gate = p->gate;
   }
   if ((rc = mark(5))) return rc;
-  if ((rc = yas_map_alloc_stage(p->map, p->depth, fp, W, H, gate))) return rc;
+  if ((rc = yas_map_alloc_stage(p->map, p->depth, fp, W, H, gate, kpair(1)))) return rc;
   if ((rc = mark(6))) return rc;
-  if ((rc = yas_map_visible_stage(p->map, W, H, gate))) return rc;
+  if ((rc = yas_map_visible_stage(p->map, W, H, gate, kpair(2)))) return rc;
   if ((rc = mark(7))) return rc;
-  if ((rc = yas_map_integrate_stage(p->map, p->depth, fp, W, H))) return rc;
+  if ((rc = yas_map_integrate_stage(p->map, p->depth, fp, W, H, kpair(3)))) return rc;
   if ((rc = mark(8))) return rc;
   hipLaunchKernelGGL(k_frame_stats, dim3(1), dim3(64), 0, s, (const int*)yas_map_view(p->map).ctr,
                      p->frame_stats + 8 * (size_t)p->frames);
@@ -309,29 +331,39 @@ int yas_pipeline_pose(yas_pipeline* p, int frame, double T[16], int* tracked) {
   return YAS_OK;
 }
 
-int yas_pipeline_enable_timing(yas_pipeline* p, int enable) {
-  YAS_REQUIRE(p, "null pipeline");
-  p->timing = enable != 0;
+int yas_pipeline_enable_timing(yas_pipeline* p, int detail_every) {
+  YAS_REQUIRE(p && detail_every >= 0, "yas_pipeline_enable_timing: bad argument");
+  p->timing = detail_every != 0;
+  if (detail_every) p->detail_every = detail_every;
+  p->timed_count = 0;
   return YAS_OK;
 }
 
 int yas_pipeline_timing(yas_pipeline* p, double stage_ms[8], double* frame_ms, int capacity, int* num_frames) {
   YAS_REQUIRE(p, "null pipeline");
   YAS_HIP(hipStreamSynchronize(p->ctx->stream));
-  for (auto& set : p->pending) {
-    double total = 0.0;
-    for (int i = 0; i < kStages; ++i) {
-      float ms = 0.0f;
-      YAS_HIP(hipEventElapsedTime(&ms, set[i], set[i + 1]));
-      p->stage_ms[i] += ms;
+  for (size_t f = 0; f < p->pending.size(); ++f) {
+    auto& set = p->pending[f];
+    if (p->pending_detail[f]) {
+      for (int i = 0; i < kStages; ++i) {
+        float ms = 0.0f;
+        YAS_HIP(hipEventElapsedTime(&ms, set[i], set[i + 1]));
+        p->stage_ms[i] += ms;
+      }
+      for (int k = 0; k < kTimedKernels; ++k) {
+        float ms = 0.0f;
+        YAS_HIP(hipEventElapsedTime(&ms, set[kStages + 1 + 2 * k], set[kStages + 2 + 2 * k]));
+        p->kernel_ms[k] += ms;
+      }
+      p->sampled++;
     }
     float whole = 0.0f;
     YAS_HIP(hipEventElapsedTime(&whole, set[0], set[kStages]));
-    total = whole;
-    p->frame_ms.push_back(total);
+    p->frame_ms.push_back(whole);
     p->spare.push_back(std::move(set));
   }
   p->pending.clear();
+  p->pending_detail.clear();
   if (stage_ms) std::memcpy(stage_ms, p->stage_ms, sizeof(p->stage_ms));
   int n = (int)p->frame_ms.size();
   if (frame_ms)
@@ -346,7 +378,18 @@ int yas_pipeline_reset_timing(yas_pipeline* p) {
   int rc = yas_pipeline_timing(p, tmp, nullptr, 0, nullptr);
   if (rc) return rc;
   std::memset(p->stage_ms, 0, sizeof(p->stage_ms));
+  std::memset(p->kernel_ms, 0, sizeof(p->kernel_ms));
+  p->sampled = 0;
   p->frame_ms.clear();
+  return YAS_OK;
+}
+
+int yas_pipeline_kernel_timing(yas_pipeline* p, double kernel_ms[4], int* frames_sampled) {
+  YAS_REQUIRE(p && kernel_ms, "null argument");
+  int rc = yas_pipeline_timing(p, nullptr, nullptr, 0, nullptr);
+  if (rc) return rc;
+  std::memcpy(kernel_ms, p->kernel_ms, sizeof(p->kernel_ms));
+  if (frames_sampled) *frames_sampled = p->sampled;
   return YAS_OK;
 }
 

@@ -38,6 +38,15 @@ __global__ void __launch_bounds__(256) k_zrange(MapView m, const PosePair* pose,
   zrange_at(m, pose, K, cols, rows, zmin, zmax, blockIdx.x, gridDim.x);
 }
 
+__global__ void __launch_bounds__(256) k_zrange_lds(MapView m, const PosePair* pose, Intr K, int cols, int rows,
+                                                    unsigned* zmin, unsigned* zmax) {
+  extern __shared__ unsigned lz[];
+  zrange_lds_at(m, pose, K, cols, rows, zmin, zmax, lz, blockIdx.x, gridDim.x);
+}
+
+// Workgroups of the LDS-privatised z-range: one per CU.
+constexpr int kZrangeLdsGroups = 256;
+
 template <class VT>
 __global__ void __launch_bounds__(256) k_march(MapView m, const PosePair* pose, Intr K, int W, int H, int cols,
                                                const float* zmin, const float* zmax, float4* vmap, float vsinv,
@@ -49,38 +58,48 @@ __global__ void __launch_bounds__(256) k_march(MapView m, const PosePair* pose, 
 // visibility pass: z-range maps zmin/zmax of (W/8)*(H/8) cells, vertex map
 // `vmap` dense W*H float4.
 static int launch_raycast_to(hipStream_t s, yas_map* map, const Intr& K, int W, int H, unsigned* zmin,
-                             unsigned* zmax, float4* vmap, hipEvent_t before_march) {
+                             unsigned* zmax, float4* vmap, hipEvent_t before_march,
+                             const hipEvent_t* march_ev = nullptr) {
   MapView v = yas_map_view(map);
   const PosePair* pose = yas_map_pose_dev(map);
   const int cols = W / 8, rows = H / 8, ncell = cols * rows;
   hipLaunchKernelGGL(k_zrange_init, dim3((ncell + 255) / 256), dim3(256), 0, s, zmin, zmax, ncell);
-  int g = (v.num_blocks + 255) / 256;
-  if (g > 2048) g = 2048;
-  hipLaunchKernelGGL(k_zrange, dim3(g), dim3(256), 0, s, v, pose, K, cols, rows, zmin, zmax);
+  if (ncell <= kZrangeLdsCells) {
+    hipLaunchKernelGGL(k_zrange_lds, dim3(kZrangeLdsGroups), dim3(256), 2 * ncell * sizeof(unsigned), s, v, pose, K,
+                       cols, rows, zmin, zmax);
+  } else {
+    int g = (v.num_blocks + 255) / 256;
+    if (g > 2048) g = 2048;
+    hipLaunchKernelGGL(k_zrange, dim3(g), dim3(256), 0, s, v, pose, K, cols, rows, zmin, zmax);
+  }
   YAS_LAUNCH_CHECK();
   if (before_march) YAS_HIP(hipEventRecord(before_march, s));
   const float vsinv = (float)(1.0 / (double)v.voxel_size);
   const float step_scale = v.trunc / v.voxel_size;
   dim3 grid(((W + 15) / 16) * ((H + 15) / 16));
+  const float* zn = (const float*)zmin;
+  const float* zf = (const float*)zmax;
   switch (v.format) {
     case YAS_VOXEL_I16_U8:
-      hipLaunchKernelGGL(k_march<VoxI16>, grid, dim3(256), 0, s, v, pose, K, W, H, cols, (const float*)zmin,
-                         (const float*)zmax, vmap, vsinv, step_scale);
+      launch_timed(k_march<VoxI16>, grid, dim3(256), s, march_ev, v, pose, K, W, H, cols, zn, zf, vmap, vsinv,
+                   step_scale);
       break;
     case YAS_VOXEL_F16_U16:
-      hipLaunchKernelGGL(k_march<VoxF16>, grid, dim3(256), 0, s, v, pose, K, W, H, cols, (const float*)zmin,
-                         (const float*)zmax, vmap, vsinv, step_scale);
+      launch_timed(k_march<VoxF16>, grid, dim3(256), s, march_ev, v, pose, K, W, H, cols, zn, zf, vmap, vsinv,
+                   step_scale);
       break;
     default:
-      hipLaunchKernelGGL(k_march<VoxF32>, grid, dim3(256), 0, s, v, pose, K, W, H, cols, (const float*)zmin,
-                         (const float*)zmax, vmap, vsinv, step_scale);
+      launch_timed(k_march<VoxF32>, grid, dim3(256), s, march_ev, v, pose, K, W, H, cols, zn, zf, vmap, vsinv,
+                   step_scale);
   }
   YAS_LAUNCH_CHECK();
   return YAS_OK;
 }
 
-static int launch_raycast(yas_raycaster* rc, yas_map* map, hipEvent_t before_march = nullptr) {
-  int r = launch_raycast_to(rc->ctx->stream, map, rc->K, rc->W, rc->H, rc->zmin, rc->zmax, rc->vmap, before_march);
+static int launch_raycast(yas_raycaster* rc, yas_map* map, hipEvent_t before_march = nullptr,
+                          const hipEvent_t* march_ev = nullptr) {
+  int r = launch_raycast_to(rc->ctx->stream, map, rc->K, rc->W, rc->H, rc->zmin, rc->zmax, rc->vmap, before_march,
+                            march_ev);
   if (r) return r;
   rc->last_map = map;
   return YAS_OK;
@@ -118,10 +137,11 @@ static int scratch_on_device(int device, unsigned** out) {
 
 // Ray cast with the pose held in device memory (pipeline).
 // `before_march` (optional) is recorded between the z-range and march kernels.
-int yas_raycast_dev(yas_raycaster* rc, yas_map* map, const double* T_dev, hipEvent_t before_march) {
+int yas_raycast_dev(yas_raycaster* rc, yas_map* map, const double* T_dev, hipEvent_t before_march,
+                    const hipEvent_t* march_ev) {
   int r = yas_map_visible_async_dev(map, T_dev, rc->W, rc->H);
   if (r) return r;
-  return launch_raycast(rc, map, before_march);
+  return launch_raycast(rc, map, before_march, march_ev);
 }
 
 float4* yas_raycaster_vmap_dev(yas_raycaster* rc) { return rc->vmap; }

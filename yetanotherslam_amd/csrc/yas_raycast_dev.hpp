@@ -22,55 +22,121 @@
 
 namespace yas {
 
+// Cells hold this zmin until a block covers them.
+constexpr float kZrangeFar = 100.0f;
+
 __device__ inline void zrange_init_at(unsigned* zmin, unsigned* zmax, int n, int bx, int nb) {
   for (int i = bx * blockDim.x + threadIdx.x; i < n; i += nb * blockDim.x) {
-    zmin[i] = __float_as_uint(100.0f);
+    zmin[i] = __float_as_uint(kZrangeFar);
     zmax[i] = 0u;
   }
 }
 
-// RayTraceEngine.cu:80-144 (createRenderingBlock) + 184-214 (fill).
+// RayTraceEngine.cu:80-144 (createRenderingBlock).  One block's cells and
+// depth range: the 8 corners projected onto the 1/8 grid, clipped to the
+// image; false when the block covers no cell or lies wholly in front of
+// depth_min.  `a`, `b` are the float bits of the range.
+struct CellBox {
+  int x0, y0, x1, y1;
+  unsigned a, b;
+};
+
+__device__ inline bool block_cells(const MapView& m, const PoseF& Ti, const Intr& K, int cols, int rows,
+                                   unsigned long long key, CellBox& cb) {
+  const float scale = m.voxel_size * 8.0f;
+  int bx, by, bz;
+  unpack_key(key, bx, by, bz);
+  short ul0 = (short)cols, ul1 = (short)rows, lr0 = -1, lr1 = -1;
+  float zr0 = m.depth_max, zr1 = m.depth_min;
+#pragma unroll
+  for (int c = 0; c < 8; ++c) {
+    float x = (float)(bx + (c & 1)) * scale;
+    float y = (float)(by + ((c >> 1) & 1)) * scale;
+    float z = (float)(bz + ((c >> 2) & 1)) * scale;
+    float qx, qy, qz;
+    xform(Ti, x, y, z, qx, qy, qz);
+    float pu = (K.fx * qx / qz + K.cx) / 8.0f;
+    float pv = (K.fy * qy / qz + K.cy) / 8.0f;
+    if ((float)ul0 > floorf(pu)) ul0 = (short)floor_sat(pu);
+    if ((float)lr0 < ceilf(pu)) lr0 = (short)ceil_sat(pu);
+    if ((float)ul1 > floorf(pv)) ul1 = (short)floor_sat(pv);
+    if ((float)lr1 < ceilf(pv)) lr1 = (short)ceil_sat(pv);
+    if (zr0 > qz) zr0 = qz;
+    if (zr1 < qz) zr1 = qz;
+  }
+  if (ul0 < 0) ul0 = 0;
+  if (ul1 < 0) ul1 = 0;
+  if (lr0 >= cols) lr0 = (short)(cols - 1);
+  if (lr1 >= rows) lr1 = (short)(rows - 1);
+  if (ul0 > lr0 || ul1 > lr1) return false;
+  if (zr0 < m.depth_min) zr0 = m.depth_min;
+  if (zr1 < m.depth_min) return false;
+  cb.x0 = ul0;
+  cb.y0 = ul1;
+  cb.x1 = lr0;
+  cb.y1 = lr1;
+  cb.a = __float_as_uint(zr0);
+  cb.b = __float_as_uint(zr1);
+  return true;
+}
+
+// LDS-privatised form for grids of at most kZrangeLdsCells cells: workgroup
+// `bx0` of `nb` takes a contiguous slice of the visible list, reduces it
+// into its own copy of the grid in LDS (`lz`: zmin then zmax, 2*cols*rows
+// words), then merges every cell it touched into the global grid with one
+// atomic per cell and bound.  Neighbouring entries of the visible list were
+// allocated together and cover the same cells, so most updates meet in LDS.
+// Min and max are order-free, so the grid equals zrange_at's bit for bit.
+constexpr int kZrangeLdsCells = 8192;
+
+__device__ inline void zrange_lds_at(const MapView& m, const PosePair* pose, const Intr& K, int cols, int rows,
+                                     unsigned* zmin, unsigned* zmax, unsigned* lz, int bx0, int nb) {
+  const int ncell = cols * rows;
+  const unsigned far = __float_as_uint(kZrangeFar);
+  for (int c = threadIdx.x; c < ncell; c += blockDim.x) {
+    lz[c] = far;
+    lz[ncell + c] = 0u;
+  }
+  __syncthreads();
+  const int nv = m.ctr[kCtrVisible];
+  const int nvis = nv < m.num_blocks ? nv : m.num_blocks;
+  const int per = (nvis + nb - 1) / nb;
+  const int i0 = bx0 * per;
+  const int i1 = i0 + per < nvis ? i0 + per : nvis;
+  const PoseF Ti = pose->Tinv;
+  for (int i = i0 + threadIdx.x; i < i1; i += blockDim.x) {
+    CellBox cb;
+    if (!block_cells(m, Ti, K, cols, rows, m.block_keys[m.visible[i]], cb)) continue;
+    for (int y = cb.y0; y <= cb.y1; ++y)
+      for (int x = cb.x0; x <= cb.x1; ++x) {
+        atomicMin(&lz[y * cols + x], cb.a);
+        atomicMax(&lz[ncell + y * cols + x], cb.b);
+      }
+  }
+  __syncthreads();
+  for (int c = threadIdx.x; c < ncell; c += blockDim.x) {
+    const unsigned a = lz[c], b = lz[ncell + c];
+    if (a < far) atomicMin(&zmin[c], a);
+    if (b != 0u) atomicMax(&zmax[c], b);
+  }
+}
+
+// RayTraceEngine.cu:184-214 (fill), straight into the global grid.
 // kReadFirst: read a cell before each atomic and skip the atomic when the
 // read already shows it would change nothing (a cell's zmin only falls and
 // its zmax only rises, so a stale read errs towards doing the atomic).  That
-// pays where many maps update cells at once (the batched launches); alone,
-// the atomics without a return value are cheaper than the read.
+// pays where many maps update cells at once (the batched launches).
 template <bool kReadFirst = false>
 __device__ inline void zrange_at(const MapView& m, const PosePair* pose, const Intr& K, int cols, int rows,
                                  unsigned* zmin, unsigned* zmax, int bx0, int nb) {
   const int nv = m.ctr[kCtrVisible];
   const int nvis = nv < m.num_blocks ? nv : m.num_blocks;
   const PoseF Ti = pose->Tinv;
-  const float scale = m.voxel_size * 8.0f;
   for (int i = bx0 * blockDim.x + threadIdx.x; i < nvis; i += nb * blockDim.x) {
-    int bx, by, bz;
-    unpack_key(m.block_keys[m.visible[i]], bx, by, bz);
-    short ul0 = (short)cols, ul1 = (short)rows, lr0 = -1, lr1 = -1;
-    float zr0 = m.depth_max, zr1 = m.depth_min;
-#pragma unroll
-    for (int c = 0; c < 8; ++c) {
-      float x = (float)(bx + (c & 1)) * scale;
-      float y = (float)(by + ((c >> 1) & 1)) * scale;
-      float z = (float)(bz + ((c >> 2) & 1)) * scale;
-      float qx, qy, qz;
-      xform(Ti, x, y, z, qx, qy, qz);
-      float pu = (K.fx * qx / qz + K.cx) / 8.0f;
-      float pv = (K.fy * qy / qz + K.cy) / 8.0f;
-      if ((float)ul0 > floorf(pu)) ul0 = (short)floor_sat(pu);
-      if ((float)lr0 < ceilf(pu)) lr0 = (short)ceil_sat(pu);
-      if ((float)ul1 > floorf(pv)) ul1 = (short)floor_sat(pv);
-      if ((float)lr1 < ceilf(pv)) lr1 = (short)ceil_sat(pv);
-      if (zr0 > qz) zr0 = qz;
-      if (zr1 < qz) zr1 = qz;
-    }
-    if (ul0 < 0) ul0 = 0;
-    if (ul1 < 0) ul1 = 0;
-    if (lr0 >= cols) lr0 = (short)(cols - 1);
-    if (lr1 >= rows) lr1 = (short)(rows - 1);
-    if (ul0 > lr0 || ul1 > lr1) continue;
-    if (zr0 < m.depth_min) zr0 = m.depth_min;
-    if (zr1 < m.depth_min) continue;
-    const unsigned a = __float_as_uint(zr0), b = __float_as_uint(zr1);
+    CellBox cb;
+    if (!block_cells(m, Ti, K, cols, rows, m.block_keys[m.visible[i]], cb)) continue;
+    const int ul0 = cb.x0, ul1 = cb.y0, lr0 = cb.x1, lr1 = cb.y1;
+    const unsigned a = cb.a, b = cb.b;
     for (int y = ul1; y <= lr1; ++y)
       for (int x = ul0; x <= lr0; ++x) {
         unsigned* pn = &zmin[y * cols + x];
@@ -165,13 +231,14 @@ struct VoxelReader {
   __device__ inline bool cached_uniform() const {
     return cblk < 0 || cflags == kBlockFree || cflags == kBlockUnobserved;
   }
+  // One comparison, no short-circuit branches.
   __device__ inline bool in_cached_block(float x, float y, float z) const {
-    return block_of(voxel_of<kInRange>(x)) == cbx && block_of(voxel_of<kInRange>(y)) == cby &&
-           block_of(voxel_of<kInRange>(z)) == cbz;
+    return ((block_of(voxel_of<kInRange>(x)) ^ cbx) | (block_of(voxel_of<kInRange>(y)) ^ cby) |
+            (block_of(voxel_of<kInRange>(z)) ^ cbz)) == 0;
   }
   // Read of local voxel `l` of block (bx, by, bz) through the one-entry cache.
   __device__ inline float read_at(int bx, int by, int bz, int l, bool& valid) {
-    if (bx != cbx || by != cby || bz != cbz) {
+    if ((bx ^ cbx) | (by ^ cby) | (bz ^ cbz)) {
       cbx = bx; cby = by; cbz = bz;
       cflags = 0;
       cblk = find_block_xyz(m, bx, by, bz, &cflags);
@@ -218,9 +285,11 @@ __device__ inline bool march_ray(const MapView& m, float& rx, float& ry, float& 
     // iteration exactly (sdf 1, last 1, the same step): take those steps
     // without the reads until the ray leaves the block or its range.
     if (!near && rd.cached_uniform()) {
-      while (dist_s < dist_e && rd.in_cached_block(rx, ry, rz)) {
+      bool more = (dist_s < dist_e) & rd.in_cached_block(rx, ry, rz);
+      while (more) {
         rx = rx + sdx; ry = ry + sdy; rz = rz + sdz;
         dist_s = dist_s + step;
+        more = (dist_s < dist_e) & rd.in_cached_block(rx, ry, rz);
       }
     }
   }

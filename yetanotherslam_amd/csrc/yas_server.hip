@@ -127,6 +127,18 @@ __global__ void __launch_bounds__(256) k_s_zrange(const StreamDev* streams, cons
   zrange_at<true>(S.m, S.pose, S.K, S.cols, S.rows, S.zmin, S.zmax, blockIdx.x, S.zr_groups);
 }
 
+// LDS-privatised z-range of every slot (grids of at most kZrangeLdsCells
+// cells); gridDim.x workgroups share each stream's visible list.
+__global__ void __launch_bounds__(256) k_s_zrange_lds(const StreamDev* streams, const Table* tab) {
+  SLOT_OR_RETURN;
+  if (sl.first) return;
+  extern __shared__ unsigned lz[];
+  zrange_lds_at(S.m, S.pose, S.K, S.cols, S.rows, S.zmin, S.zmax, lz, blockIdx.x, gridDim.x);
+}
+
+// Workgroups per stream of the batched LDS z-range.
+constexpr int kZrangeLdsGroupsPerSlot = 64;
+
 __global__ void __launch_bounds__(256) k_s_march_order(const StreamDev* streams, const Table* tab, MarchOrder mo) {
   SLOT_OR_RETURN;
   if (sl.first) return;
@@ -440,7 +452,12 @@ static int launch_batch_kernels(yas_server* sv, hipStream_t s, int nz) {
   const int tiles = ((W + 15) / 16) * ((H + 15) / 16);
   const MarchOrder mo{sv->march_tiles, sv->march_counts, kMaxSlots * tiles};
   hipLaunchKernelGGL(k_s_zrange_init, dim3(((W / 8) * (H / 8) + 255) / 256, 1, nz), dim3(256), 0, s, S, T, mo);
-  hipLaunchKernelGGL(k_s_zrange, dim3(2048, 1, nz), dim3(256), 0, s, S, T);
+  const int ncell = (W / 8) * (H / 8);
+  if (ncell <= kZrangeLdsCells)
+    hipLaunchKernelGGL(k_s_zrange_lds, dim3(kZrangeLdsGroupsPerSlot, 1, nz), dim3(256), 2 * ncell * sizeof(unsigned),
+                       s, S, T);
+  else
+    hipLaunchKernelGGL(k_s_zrange, dim3(2048, 1, nz), dim3(256), 0, s, S, T);
   hipLaunchKernelGGL(k_s_march_order, dim3((tiles + 255) / 256, 1, nz), dim3(256), 0, s, S, T, mo);
   const dim3 mg(tiles, 1, nz);  // one workgroup per tile of every image, in the order's sequence
   switch (sv->voxel_format) {

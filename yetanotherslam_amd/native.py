@@ -160,6 +160,7 @@ SIGNATURES = {
     "yas_pipeline_enable_timing": (I, [P, I]),
     "yas_pipeline_timing": (I, [P, P, P, I, PI]),
     "yas_pipeline_reset_timing": (I, [P]),
+    "yas_pipeline_kernel_timing": (I, [P, P, PI]),
     "yas_pipeline_frame_stats": (I, [P, I, I, P]),
     "yas_pipeline_map": (I, [P, PP]),
     "yas_pipeline_raycaster": (I, [P, PP]),
