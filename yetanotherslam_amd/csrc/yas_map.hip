@@ -183,10 +183,13 @@ __global__ void __launch_bounds__(256) k_visible(MapView m, const PosePair* pose
   visible_scan(m, pose->Tinv, K, W, H, blockIdx.x * blockDim.x, gridDim.x * blockDim.x);
 }
 
-// K5 (VoxelMap.cu:719-786).
+// K5 (VoxelMap.cu:719-786).  `stats` (optional) receives the map's first
+// four counters (allocated, visible, flags, walk steps), final once
+// allocation and visibility have run.
 template <class VT>
 __global__ void __launch_bounds__(256) k_integrate(MapView m, const PosePair* pose, Intr K, const float* depth,
-                                                   size_t pitch, int W, int H) {
+                                                   size_t pitch, int W, int H, int* stats) {
+  if (stats && blockIdx.x == 0 && threadIdx.x < 4) stats[threadIdx.x] = m.ctr[threadIdx.x];
   integrate_waves<VT>(m, pose->Tinv, K, DepthF32{depth, pitch}, W, H,
                       blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6), gridDim.x * (blockDim.x >> 6));
 }
@@ -201,7 +204,7 @@ static int grid_for(long n, int per, int cap) {
 }
 
 static int launch_integrate(yas_map* map, const float* depth, size_t pitch, int W, int H,
-                            const hipEvent_t* ev = nullptr) {
+                            const hipEvent_t* ev = nullptr, int* stats = nullptr) {
   hipStream_t s = map->ctx->stream;
   // enough wavefronts for one visible block each up to 32768 visible blocks
   // (those past the visible count return at once)
@@ -209,13 +212,13 @@ static int launch_integrate(yas_map* map, const float* depth, size_t pitch, int 
   const PosePair* pose = map->pose_dev;
   switch (map->P.voxel_format) {
     case YAS_VOXEL_I16_U8:
-      launch_timed(k_integrate<VoxI16>, grid, dim3(256), s, ev, map->v, pose, map->K, depth, pitch, W, H);
+      launch_timed(k_integrate<VoxI16>, grid, dim3(256), s, ev, map->v, pose, map->K, depth, pitch, W, H, stats);
       break;
     case YAS_VOXEL_F16_U16:
-      launch_timed(k_integrate<VoxF16>, grid, dim3(256), s, ev, map->v, pose, map->K, depth, pitch, W, H);
+      launch_timed(k_integrate<VoxF16>, grid, dim3(256), s, ev, map->v, pose, map->K, depth, pitch, W, H, stats);
       break;
     default:
-      launch_timed(k_integrate<VoxF32>, grid, dim3(256), s, ev, map->v, pose, map->K, depth, pitch, W, H);
+      launch_timed(k_integrate<VoxF32>, grid, dim3(256), s, ev, map->v, pose, map->K, depth, pitch, W, H, stats);
   }
   YAS_LAUNCH_CHECK();
   return YAS_OK;
@@ -240,8 +243,8 @@ int yas_map_visible_stage(yas_map* map, int W, int H, const int* gate, const hip
   return YAS_OK;
 }
 int yas_map_integrate_stage(yas_map* map, const float* depth, size_t pitch, int W, int H,
-                            const hipEvent_t* ev = nullptr) {
-  return launch_integrate(map, depth, pitch, W, H, ev);
+                            const hipEvent_t* ev = nullptr, int* stats = nullptr) {
+  return launch_integrate(map, depth, pitch, W, H, ev, stats);
 }
 int yas_map_fuse_prepared(yas_map* map, const float* depth, size_t pitch, int W, int H, const int* gate) {
   int rc = yas_map_alloc_stage(map, depth, pitch, W, H, gate);
@@ -267,6 +270,7 @@ static int prepare_pose(yas_map* map, const double T[16], bool reset_visible) {
 
 MapView yas_map_view(yas_map* map) { return map->v; }
 const PosePair* yas_map_pose_dev(yas_map* map) { return map->pose_dev; }
+PosePair* yas_map_pose_dev_mut(yas_map* map) { return map->pose_dev; }
 Intr yas_map_intr(yas_map* map) { return map->K; }
 yas_ctx* yas_map_ctx(yas_map* map) { return map->ctx; }
 
@@ -274,6 +278,14 @@ yas_ctx* yas_map_ctx(yas_map* map) { return map->ctx; }
 int yas_map_visible_async(yas_map* map, const double T[16], int W, int H) {
   int rc = prepare_pose(map, T, true);
   if (rc) return rc;
+  hipLaunchKernelGGL(k_visible, dim3(grid_for(map->P.num_blocks, 256, 4096)), dim3(256), 0, map->ctx->stream,
+                     map->v, (const PosePair*)map->pose_dev, map->K, W, H, (const int*)nullptr);
+  YAS_LAUNCH_CHECK();
+  return YAS_OK;
+}
+// Visibility with the pose pair already in pose_dev and the visible counter
+// already cleared.
+int yas_map_visible_prepared(yas_map* map, int W, int H) {
   hipLaunchKernelGGL(k_visible, dim3(grid_for(map->P.num_blocks, 256, 4096)), dim3(256), 0, map->ctx->stream,
                      map->v, (const PosePair*)map->pose_dev, map->K, W, H, (const int*)nullptr);
   YAS_LAUNCH_CHECK();

@@ -9,24 +9,24 @@
 #include <vector>
 
 #include "yas_internal.hpp"
+#include "yas_raycast_dev.hpp"
 #include "yas_server_dev.hpp"
 
 using namespace yas;
 
 int yas_map_fuse_prepared(yas_map* map, const float* depth, size_t pitch, int W, int H, const int* gate);
-int yas_map_prepare_pose_dev(yas_map* map, const double* T_dev);
-int yas_convert_depth_u16(yas_ctx* ctx, const uint16_t* src, size_t sp, float* dst, size_t dp, int W, int H,
-                          float depth_scale);
-int yas_raycast_dev(yas_raycaster* rc, yas_map* map, const double* T_dev, hipEvent_t before_march,
-                    const hipEvent_t* march_ev);
 int yas_map_alloc_stage(yas_map* map, const float* depth, size_t pitch, int W, int H, const int* gate,
                         const hipEvent_t* ev);
 int yas_map_visible_stage(yas_map* map, int W, int H, const int* gate, const hipEvent_t* ev);
-int yas_map_integrate_stage(yas_map* map, const float* depth, size_t pitch, int W, int H, const hipEvent_t* ev);
+int yas_map_integrate_stage(yas_map* map, const float* depth, size_t pitch, int W, int H, const hipEvent_t* ev,
+                            int* stats);
 MapView yas_map_view(yas_map* map);
+PosePair* yas_map_pose_dev_mut(yas_map* map);
+int yas_raycast_prepared(yas_raycaster* rc, yas_map* map, hipEvent_t before_march, const hipEvent_t* march_ev);
 float4* yas_raycaster_vmap_dev(yas_raycaster* rc);
+void yas_raycaster_zrange_dev(yas_raycaster* rc, unsigned** zmin, unsigned** zmax, int* ncell);
+TrackState* yas_tracker_state_dev(yas_tracker* t);
 int yas_tracker_solve_async(yas_tracker* t);
-int yas_tracker_init_identity_async(yas_tracker* t);
 const double* yas_tracker_result_dev(yas_tracker* t);
 const int* yas_tracker_good_dev(yas_tracker* t);
 int yas_tracker_set_tracking_image_dev(yas_tracker* t, const float* g, size_t pitch);
@@ -34,28 +34,76 @@ int yas_tracker_prepare_frame_dev(yas_tracker* t, const float* vmap, size_t vpit
                                   const float* d, size_t dpitch);
 int yas_tracker_set_tracking_depth_dev(yas_tracker* t, const float* d, size_t pitch);
 int yas_tracker_unswap_copies(yas_tracker* t, RefRestore& r);
-int yas_rgb_to_gray_async(yas_ctx* ctx, const uint8_t* rgb, size_t sp, float* dst, size_t dp, int W, int H, int mode);
 
 namespace {
 
-// The frame's outcome (one thread), and after a NaN update the reference
-// pyramids put back as they were before the host-side swap (all threads).
+// The start of a frame in one launch of 64x4-pixel workgroups: depth in
+// metres and the gray image; from the first workgroup the map's float pose
+// pair of T_last with its visible counter cleared (for the ray cast, or the
+// first frame's fusion) and the tracker's state reset to identity; from the
+// first ncell threads the ray caster's z-range cells reset.
+struct FrameBegin {
+  const uint16_t* depth_in;
+  size_t dpitch;
+  const uint8_t* rgb;
+  size_t cpitch;
+  float* depth;  // pitch fpitch
+  float* gray;   // pitch fpitch
+  size_t fpitch;
+  float scale;
+  int W, H, gray_mode;
+  const double* T_last;
+  PosePair* pose;
+  int* ctr;
+  TrackState* st;
+  unsigned* zmin;
+  unsigned* zmax;
+  int ncell;
+};
+
+__global__ void __launch_bounds__(256) k_frame_begin(FrameBegin a) {
+  const int gid = (blockIdx.y * gridDim.x + blockIdx.x) * blockDim.x + threadIdx.x;
+  if (gid < a.ncell) {
+    a.zmin[gid] = __float_as_uint(kZrangeFar);
+    a.zmax[gid] = 0u;
+  }
+  if (blockIdx.x == 0 && blockIdx.y == 0) {
+    if (threadIdx.x == 0) {
+      PosePair p;
+      pose_pair_from(a.T_last, p);
+      *a.pose = p;
+      a.ctr[kCtrVisible] = 0;
+    }
+    track_init_identity_at(a.st);
+  }
+  const int x = blockIdx.x * 64 + (threadIdx.x & 63), y = blockIdx.y * 4 + (threadIdx.x >> 6);
+  if (x < a.W && y < a.H) {
+    const uint16_t d = ((const uint16_t*)((const char*)a.depth_in + (size_t)y * a.dpitch))[x];
+    ((float*)((char*)a.depth + (size_t)y * a.fpitch))[x] = (float)d * a.scale;
+  }
+  rgb_to_gray_at(a.rgb, a.cpitch, a.gray, a.fpitch, a.W, a.H, a.gray_mode, blockIdx.x, blockIdx.y);
+}
+
+// The frame's outcome (one thread), which then writes the map's pose pair
+// of the (possibly new) T_last for the fusion and clears its visible
+// counter, after saving the ray cast's visible count; and after a NaN
+// update the reference pyramids put back as they were before the host-side
+// swap (all threads).
 __global__ void __launch_bounds__(256) k_post_track(const double* DT, const int* good, double* T_last,
-                                                    double* pose_out, int* tracked, int* gate, const int* ray_visible,
-                                                    int* ray_visible_out, RefRestore rr) {
+                                                    double* pose_out, int* tracked, int* gate, int* ctr,
+                                                    int* ray_visible_out, PosePair* pose, RefRestore rr) {
   if (blockIdx.x == 0 && threadIdx.x == 0) {
-    *ray_visible_out = *ray_visible;
+    *ray_visible_out = ctr[kCtrVisible];
     post_track_at(DT, good, T_last, pose_out, tracked, gate);
+    PosePair p;
+    pose_pair_from(T_last, p);
+    *pose = p;
+    ctr[kCtrVisible] = 0;
   }
   if (*good) return;
   const int stride = gridDim.x * blockDim.x;
   for (int c = 0; c < rr.count; ++c)
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < rr.n[c]; i += stride) rr.dst[c][i] = rr.src[c][i];
-}
-
-// Per-frame counters of the map: allocated, visible, flags, walk steps.
-__global__ void k_frame_stats(const int* ctr, int* out) {
-  if (threadIdx.x < 4) out[threadIdx.x] = ctr[threadIdx.x];
 }
 
 __global__ void k_record_first(const double* T_last, double* pose_out, int* tracked) {
@@ -273,38 +321,55 @@ int yas_pipeline_step(yas_pipeline* p, const uint16_t* depth, size_t dpitch, con
     return YAS_OK;
   };
   if ((rc = mark(0))) return rc;
-  if ((rc = yas_convert_depth_u16(p->ctx, depth, dpitch, p->depth, fp, W, H, p->depth_scale))) return rc;
-  if ((rc = yas_rgb_to_gray_async(p->ctx, rgb, cpitch, p->gray, fp, W, H, p->gray_mode))) return rc;
+  {
+    FrameBegin a;
+    a.depth_in = depth;
+    a.dpitch = dpitch;
+    a.rgb = rgb;
+    a.cpitch = cpitch;
+    a.depth = p->depth;
+    a.gray = p->gray;
+    a.fpitch = fp;
+    a.scale = (float)(1.0 / (double)p->depth_scale);
+    a.W = W;
+    a.H = H;
+    a.gray_mode = p->gray_mode;
+    a.T_last = p->T_last;
+    a.pose = yas_map_pose_dev_mut(p->map);
+    a.ctr = yas_map_view(p->map).ctr;
+    a.st = yas_tracker_state_dev(p->trk);
+    yas_raycaster_zrange_dev(p->rc, &a.zmin, &a.zmax, &a.ncell);
+    hipLaunchKernelGGL(k_frame_begin, dim3((W + 63) / 64, (H + 3) / 4), dim3(256), 0, s, a);
+    YAS_LAUNCH_CHECK();
+  }
   if ((rc = mark(1))) return rc;
   double* pose_out = p->poses + 16 * (size_t)p->frames;
   int* tracked_out = p->tracked + p->frames;
   const int* gate = nullptr;
   if (p->frames == 0) {
-    // StereoInitialization (Tracking.cpp:139-155)
+    // StereoInitialization (Tracking.cpp:139-155); the map's pose pair is
+    // already that of T_last
     if ((rc = yas_tracker_set_reference_image(p->trk, p->gray, fp))) return rc;
     if ((rc = yas_tracker_set_reference_depth(p->trk, p->depth, fp))) return rc;
-    if ((rc = yas_map_prepare_pose_dev(p->map, p->T_last))) return rc;
     hipLaunchKernelGGL(k_record_first, dim3(1), dim3(64), 0, s, (const double*)p->T_last, pose_out, tracked_out);
     YAS_LAUNCH_CHECK();
   } else {
     // TrackRGBD (Tracking.cpp:162-197)
-    if ((rc = yas_raycast_dev(p->rc, p->map, p->T_last, detail ? ev[2] : nullptr, kpair(0)))) return rc;
+    if ((rc = yas_raycast_prepared(p->rc, p->map, detail ? ev[2] : nullptr, kpair(0)))) return rc;
     if ((rc = mark(3))) return rc;
     if ((rc = yas_tracker_prepare_frame_dev(p->trk, (const float*)yas_raycaster_vmap_dev(p->rc), (size_t)W * 16,
                                             p->gray, fp, p->depth, fp)))
       return rc;
     if ((rc = mark(4))) return rc;
-    if ((rc = yas_tracker_init_identity_async(p->trk))) return rc;
     if ((rc = yas_tracker_solve_async(p->trk))) return rc;
     yas_tracker_swap_frame_buffer(p->trk);
     RefRestore rr;
     yas_tracker_unswap_copies(p->trk, rr);
     hipLaunchKernelGGL(k_post_track, dim3(32), dim3(256), 0, s, yas_tracker_result_dev(p->trk),
                        yas_tracker_good_dev(p->trk), p->T_last, pose_out, tracked_out, p->gate,
-                       (const int*)yas_map_view(p->map).ctr + kCtrVisible, p->frame_stats + 8 * (size_t)p->frames + 4,
-                       rr);
+                       yas_map_view(p->map).ctr, p->frame_stats + 8 * (size_t)p->frames + 4,
+                       yas_map_pose_dev_mut(p->map), rr);
     YAS_LAUNCH_CHECK();
-    if ((rc = yas_map_prepare_pose_dev(p->map, p->T_last))) return rc;
     gate = p->gate;
   }
   if ((rc = mark(5))) return rc;
@@ -312,11 +377,9 @@ int yas_pipeline_step(yas_pipeline* p, const uint16_t* depth, size_t dpitch, con
   if ((rc = mark(6))) return rc;
   if ((rc = yas_map_visible_stage(p->map, W, H, gate, kpair(2)))) return rc;
   if ((rc = mark(7))) return rc;
-  if ((rc = yas_map_integrate_stage(p->map, p->depth, fp, W, H, kpair(3)))) return rc;
+  if ((rc = yas_map_integrate_stage(p->map, p->depth, fp, W, H, kpair(3), p->frame_stats + 8 * (size_t)p->frames)))
+    return rc;
   if ((rc = mark(8))) return rc;
-  hipLaunchKernelGGL(k_frame_stats, dim3(1), dim3(64), 0, s, (const int*)yas_map_view(p->map).ctr,
-                     p->frame_stats + 8 * (size_t)p->frames);
-  YAS_LAUNCH_CHECK();
   p->frames++;
   return YAS_OK;
 }

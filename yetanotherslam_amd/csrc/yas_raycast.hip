@@ -16,6 +16,7 @@ const PosePair* yas_map_pose_dev(yas_map* map);
 yas_ctx* yas_map_ctx(yas_map* map);
 int yas_map_visible_async(yas_map* map, const double T[16], int W, int H);
 int yas_map_visible_async_dev(yas_map* map, const double* T_dev, int W, int H);
+int yas_map_visible_prepared(yas_map* map, int W, int H);
 
 struct yas_raycaster {
   yas_ctx* ctx;
@@ -56,14 +57,16 @@ __global__ void __launch_bounds__(256) k_march(MapView m, const PosePair* pose, 
 
 // z-range tiles then the march (RayTraceEngine.cu:412-450) after the map's
 // visibility pass: z-range maps zmin/zmax of (W/8)*(H/8) cells, vertex map
-// `vmap` dense W*H float4.
+// `vmap` dense W*H float4.  `cells_reset`: the caller has already reset the
+// z-range cells on this stream.
 static int launch_raycast_to(hipStream_t s, yas_map* map, const Intr& K, int W, int H, unsigned* zmin,
                              unsigned* zmax, float4* vmap, hipEvent_t before_march,
-                             const hipEvent_t* march_ev = nullptr) {
+                             const hipEvent_t* march_ev = nullptr, bool cells_reset = false) {
   MapView v = yas_map_view(map);
   const PosePair* pose = yas_map_pose_dev(map);
   const int cols = W / 8, rows = H / 8, ncell = cols * rows;
-  hipLaunchKernelGGL(k_zrange_init, dim3((ncell + 255) / 256), dim3(256), 0, s, zmin, zmax, ncell);
+  if (!cells_reset)
+    hipLaunchKernelGGL(k_zrange_init, dim3((ncell + 255) / 256), dim3(256), 0, s, zmin, zmax, ncell);
   if (ncell <= kZrangeLdsCells) {
     hipLaunchKernelGGL(k_zrange_lds, dim3(kZrangeLdsGroups), dim3(256), 2 * ncell * sizeof(unsigned), s, v, pose, K,
                        cols, rows, zmin, zmax);
@@ -97,9 +100,9 @@ static int launch_raycast_to(hipStream_t s, yas_map* map, const Intr& K, int W, 
 }
 
 static int launch_raycast(yas_raycaster* rc, yas_map* map, hipEvent_t before_march = nullptr,
-                          const hipEvent_t* march_ev = nullptr) {
+                          const hipEvent_t* march_ev = nullptr, bool cells_reset = false) {
   int r = launch_raycast_to(rc->ctx->stream, map, rc->K, rc->W, rc->H, rc->zmin, rc->zmax, rc->vmap, before_march,
-                            march_ev);
+                            march_ev, cells_reset);
   if (r) return r;
   rc->last_map = map;
   return YAS_OK;
@@ -144,7 +147,21 @@ int yas_raycast_dev(yas_raycaster* rc, yas_map* map, const double* T_dev, hipEve
   return launch_raycast(rc, map, before_march, march_ev);
 }
 
+// Ray cast whose set-up the caller has already enqueued on this stream: the
+// map's pose pair written, its visible counter cleared and this ray caster's
+// z-range cells reset (the pipeline does all three in its first launch).
+int yas_raycast_prepared(yas_raycaster* rc, yas_map* map, hipEvent_t before_march, const hipEvent_t* march_ev) {
+  int r = yas_map_visible_prepared(map, rc->W, rc->H);
+  if (r) return r;
+  return launch_raycast(rc, map, before_march, march_ev, true);
+}
+
 float4* yas_raycaster_vmap_dev(yas_raycaster* rc) { return rc->vmap; }
+void yas_raycaster_zrange_dev(yas_raycaster* rc, unsigned** zmin, unsigned** zmax, int* ncell) {
+  *zmin = rc->zmin;
+  *zmax = rc->zmax;
+  *ncell = rc->cols * rc->rows;
+}
 
 // Buffers and launch constants of this ray caster for the serving engine.
 int yas_raycaster_fill(yas_raycaster* rc, const MapView& v, StreamDev& d) {

@@ -381,6 +381,8 @@ const yas_tracker_params& yas_tracker_params_of(yas_tracker* t) { return t->P; }
 // Device pointers of the solver's result (camera pose change and status).
 const double* yas_tracker_result_dev(yas_tracker* t) { return t->st->last_ok; }
 const int* yas_tracker_good_dev(yas_tracker* t) { return &t->st->good; }
+// The state a solve starts from (the pipeline resets it in its first launch).
+TrackState* yas_tracker_state_dev(yas_tracker* t) { return t->st; }
 
 int yas_tracker_set_tracking_image_dev(yas_tracker* t, const float* g, size_t pitch) {
   return image_pyramid(t, g, pitch, t->cur_int, true);
