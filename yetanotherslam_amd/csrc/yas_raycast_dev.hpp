@@ -196,6 +196,22 @@ struct VoxelReader {
   __device__ inline float interp_after_read(float x, float y, float z, bool& valid) {
     return interp_rest(x, y, z, rx0, ry0, rz0, rs, rv, valid);
   }
+  // How a block answers a corner read: its pool index when the voxel must
+  // be loaded, else -1 for (1, valid) (all free) or -2 for (1, invalid)
+  // (absent or all unobserved) -- read_at's cases.
+  __device__ static inline int corner_code(int blk, int flags) {
+    if (blk < 0 || flags == kBlockUnobserved) return -2;
+    return flags == kBlockFree ? -1 : blk;
+  }
+  // The 7 remaining corners: corner c is (c&1 ? x1 : x0, c&2 ? y1 : y0,
+  // c&4 ? z1 : z0).  When at most one axis's two corners straddle a block
+  // face (nearly every call), the corners lie in corner 0's block, which is
+  // cached, and at most one other, looked up once; then the voxel loads of
+  // all 7 corners are issued before any is used.  Read one by one through
+  // the one-entry cache they made a chain of dependent loads, with a lookup
+  // per read when the corners alternate between two blocks.  Corners
+  // spread over more blocks take the reads in turn.  The values and the
+  // arithmetic order are those of 8 reads in the reference's order.
   __device__ inline float interp_rest(float x, float y, float z, int x0, int y0, int z0, float s0, bool vp,
                                       bool& valid) {
     const float fx = x - floorf(x), fy = y - floorf(y), fz = z - floorf(z);
@@ -203,22 +219,56 @@ struct VoxelReader {
     const int bx0 = block_of(x0), bx1 = block_of(x1), by0 = block_of(y0), by1 = block_of(y1);
     const int bz0 = block_of(z0), bz1 = block_of(z1);
     const int lx0 = vox_to_local(x0), lx1 = vox_to_local(x1);
-    const int r00 = (vox_to_local(z0) * 8 + vox_to_local(y0)) * 8, r01 = (vox_to_local(z0) * 8 + vox_to_local(y1)) * 8;
-    const int r10 = (vox_to_local(z1) * 8 + vox_to_local(y0)) * 8, r11 = (vox_to_local(z1) * 8 + vox_to_local(y1)) * 8;
-    bool v;
-    float s1 = read_at(bx1, by0, bz0, r00 + lx1, v); vp &= v;
-    float r0 = (1.0f - fx) * s0 + fx * s1;
-    s0 = read_at(bx0, by1, bz0, r01 + lx0, v); vp &= v;
-    s1 = read_at(bx1, by1, bz0, r01 + lx1, v); vp &= v;
-    float r1 = (1.0f - fx) * s0 + fx * s1;
-    float r2 = (1.0f - fy) * r0 + fy * r1;
-    s0 = read_at(bx0, by0, bz1, r10 + lx0, v); vp &= v;
-    s1 = read_at(bx1, by0, bz1, r10 + lx1, v); vp &= v;
-    r0 = (1.0f - fx) * s0 + fx * s1;
-    s0 = read_at(bx0, by1, bz1, r11 + lx0, v); vp &= v;
-    s1 = read_at(bx1, by1, bz1, r11 + lx1, v); vp &= v;
-    r1 = (1.0f - fx) * s0 + fx * s1;
-    float r3 = (1.0f - fy) * r0 + fy * r1;
+    const int mask = (int)(bx1 != bx0) | ((int)(by1 != by0) << 1) | ((int)(bz1 != bz0) << 2);
+    float sv[8];
+    sv[0] = s0;
+    if ((mask & (mask - 1)) == 0) {
+      const int code_a = corner_code(cblk, cflags);
+      int code_b = code_a;
+      if (mask) {
+        int fb = 0;
+        const int ib = find_block_xyz(m, (mask & 1) ? bx1 : bx0, (mask & 2) ? by1 : by0, (mask & 4) ? bz1 : bz0, &fb);
+        code_b = corner_code(ib, fb);
+      }
+      const int ly[2] = {vox_to_local(y0), vox_to_local(y1)};
+      const int lz[2] = {vox_to_local(z0), vox_to_local(z1)};
+      typename VT::T raw[8];
+#pragma unroll
+      for (int c = 1; c < 8; ++c) {
+        const int code = (c & mask) ? code_b : code_a;
+        const size_t at =
+            code >= 0 ? (size_t)code * 512 + (lz[c >> 2] * 8 + ly[(c >> 1) & 1]) * 8 + ((c & 1) ? lx1 : lx0) : 0;
+        raw[c] = vox[at];
+      }
+#pragma unroll
+      for (int c = 1; c < 8; ++c) {
+        const int code = (c & mask) ? code_b : code_a;
+        bool v = code == -1;
+        sv[c] = 1.0f;
+        if (code >= 0 && VT::observed(raw[c])) {
+          v = true;
+          sv[c] = VT::sdf(raw[c]);
+        }
+        vp &= v;
+      }
+    } else {
+      const int r00 = (vox_to_local(z0) * 8 + vox_to_local(y0)) * 8, r01 = (vox_to_local(z0) * 8 + vox_to_local(y1)) * 8;
+      const int r10 = (vox_to_local(z1) * 8 + vox_to_local(y0)) * 8, r11 = (vox_to_local(z1) * 8 + vox_to_local(y1)) * 8;
+      bool v;
+      sv[1] = read_at(bx1, by0, bz0, r00 + lx1, v); vp &= v;
+      sv[2] = read_at(bx0, by1, bz0, r01 + lx0, v); vp &= v;
+      sv[3] = read_at(bx1, by1, bz0, r01 + lx1, v); vp &= v;
+      sv[4] = read_at(bx0, by0, bz1, r10 + lx0, v); vp &= v;
+      sv[5] = read_at(bx1, by0, bz1, r10 + lx1, v); vp &= v;
+      sv[6] = read_at(bx0, by1, bz1, r11 + lx0, v); vp &= v;
+      sv[7] = read_at(bx1, by1, bz1, r11 + lx1, v); vp &= v;
+    }
+    float r0 = (1.0f - fx) * sv[0] + fx * sv[1];
+    float r1 = (1.0f - fx) * sv[2] + fx * sv[3];
+    const float r2 = (1.0f - fy) * r0 + fy * r1;
+    r0 = (1.0f - fx) * sv[4] + fx * sv[5];
+    r1 = (1.0f - fx) * sv[6] + fx * sv[7];
+    const float r3 = (1.0f - fy) * r0 + fy * r1;
     valid = vp;
     return (1.0f - fz) * r2 + fz * r3;
   }
