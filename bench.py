@@ -86,8 +86,8 @@ def cpu_baseline(p, K, depth, rgb, budget_s=12.0, max_frames=60, all_cores=False
 # carried here, keyed by the run they were measured on: (warmup, steps),
 # i.e. the timed frames warmup+1 .. warmup+steps of the sequence.  Other runs
 # report no traffic figure.
-TRAFFIC_BYTES = {(10, 290): {"k_march": 55.6e6, "k_integrate": 111.7e6, "k_alloc": 4.5e6},
-                 (5, 20): {"k_march": 39.31e6, "k_integrate": 89.22e6, "k_alloc": 4.42e6}}
+TRAFFIC_BYTES = {(10, 290): {"k_march": 54.49e6, "k_integrate": 111.31e6, "k_alloc": 4.45e6},
+                 (5, 20): {"k_march": 38.37e6, "k_integrate": 89.15e6, "k_alloc": 4.42e6}}
 
 
 def walk_steps(stats):
