@@ -59,18 +59,19 @@ def cpu_model():
     return "unknown"
 
 
-def cpu_baseline(p, K, depth, rgb, budget_s=12.0, max_frames=60, all_cores=False):
-    """The scalar oracle on the first frames of the same sequence (frame 0
-    initialises the volume and is not timed): single-threaded, or its OpenMP
-    build over the host cores OpenMP is given (all_cores).  Returns
-    (frames/s, frames, seconds, threads)."""
+def cpu_baseline(p, K, depth, rgb, budget_s=12.0, all_cores=False):
+    """The scalar oracle on the frames of the same sequence from frame 1 on
+    (frame 0 initialises the volume and is not timed), until `budget_s` of
+    work or the end of the sequence: single-threaded, or its OpenMP build
+    over the host cores OpenMP is given (all_cores).  Returns (frames/s,
+    frames, seconds, threads)."""
     from oracle import pyoracle as po
     op = po.OraclePipeline(W, H, K, VOXEL, TRUNC, LEVELS, ITERS, po.MODE_RGBD, WEIGHTS, all_cores=all_cores)
     threads = (po.lib_omp() if all_cores else po.lib()).orc_threads()
     op.step(depth[0], rgb[0])
     t0 = time.perf_counter()
     n = 0
-    for t in range(1, min(max_frames, len(depth))):
+    for t in range(1, len(depth)):
         op.step(depth[t], rgb[t])
         n += 1
         if time.perf_counter() - t0 > budget_s:
