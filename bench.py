@@ -239,8 +239,8 @@ def ping_pong(k, n):
     return k if k < n else 2 * n - 2 - k
 
 
-def serving_bench(dev_depth, dev_rgb, nseq, frames_sat=60, sweep_loads=(1500, 3000, 4500, 6000), sweep_s=2.0,
-                  runs=None):
+def serving_bench(dev_depth, dev_rgb, nseq, frames_sat=60, sweep_loads=(1500, 3000, 4500, 5000, 5700, 6500),
+                  sweep_s=2.0, runs=None):
     """Serving form: 32 streams (workloads), each a segment of the room
     sequence from its own seeded start frame, 640x480 or 320x240, voxel 5 mm,
     2^20 buckets and a 2^16-block pool per stream, f32 storage (and, for one

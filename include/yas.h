@@ -260,6 +260,14 @@ int yas_server_destroy(yas_server* sv);
  * tracker's per-level images of the batch resident in the last-level cache
  * across Gauss-Newton iterations. */
 int yas_server_set_max_batch(yas_server* sv, int max_batch);
+/* Batches queued on the device at once, 1..3 (default 2), and the frames
+ * that must be waiting before a batch is queued behind one that is already
+ * on the device (default 16, at most the batch cap).  Frames that arrive
+ * while a batch runs gather into the next one, so holding batches back makes
+ * them larger, paying their fixed launch cost over more frames; queueing
+ * ahead keeps the device busy between batches.  No reference counterpart:
+ * serving is this library's own form. */
+int yas_server_set_max_in_flight(yas_server* sv, int batches, int min_ahead);
 /* Run the Gauss-Newton iterations on sub-ranges of `slots` batch slots in
  * turn (0, the default: the whole batch at once).  Results are unchanged. */
 int yas_server_set_gn_group(yas_server* sv, int slots);

@@ -281,3 +281,32 @@ def test_idle_streams_are_hibernated_to_admit_others_and_resume_exactly():
                 np.testing.assert_array_equal(arr, ref)
     print("hibernated first:", hib, "stats:", st)
     sv.close()
+
+
+@pytest.mark.parametrize("in_flight,ahead", [(1, 1), (2, 16), (3, 1)], ids=["one", "two_ahead16", "three"])
+def test_queueing_policy_changes_batches_not_results(in_flight, ahead):
+    """Batches held back (one in flight), queued ahead only when enough frames
+    wait (the default), or always queued: the batches differ, every stream's
+    poses equal its own pipeline's.  Three streams submit four frames each
+    with a pump after each round."""
+    n = 4
+    ss = [stream_frames(160, 120, th, n) for th in (0.3, 1.1, 2.4)]
+    sv = rf.Server(160, 120, 4, tracker_params())
+    sv.set_max_in_flight(in_flight, ahead)
+    ids = [sv.add_stream(160, 120, s[1], map_params()) for s in ss]
+    torch.cuda.synchronize()
+    for t in range(n):
+        for sid, s in zip(ids, ss):
+            sv.submit(sid, *s[2][t])
+        sv.pump()
+    sv.drain()
+    by = results_by_stream(sv.poll())
+    st = sv.stats()
+    sv.close()
+    assert st["frames_done"] == 3 * n
+    for sid, s in zip(ids, ss):
+        check_against_single(by[sid], s)
+    bad = rf.Server(160, 120, 4, tracker_params())
+    with pytest.raises(Exception):
+        bad.set_max_in_flight(4, 1)
+    bad.close()

@@ -415,8 +415,10 @@ struct yas_server {
   int maxW, maxH;
   size_t budget, used;
   int max_streams;
-  int max_batch;  // frames per batch (<= kMaxSlots)
-  int gn_group;   // slots per Gauss-Newton sub-range (0: the whole batch)
+  int max_batch;      // frames per batch (<= kMaxSlots)
+  int max_in_flight;  // batches queued on the device at once (<= kInFlight)
+  int min_ahead;      // frames that must wait before a batch queues behind a running one
+  int gn_group;       // slots per Gauss-Newton sub-range (0: the whole batch)
   std::vector<HostStream> streams;  // slot per stream id
   double evict_idle_ms;  // < 0: never hibernate; else idle streams older than this may be paged out
   long long next_order;
@@ -582,6 +584,12 @@ int yas_server_create(yas_ctx* ctx, int max_width, int max_height, int max_strea
   sv->used = 0;
   sv->max_streams = max_streams;
   sv->max_batch = kMaxSlots;
+  // Measured with 32 mixed-resolution streams under Poisson load: two in
+  // flight, the second only once 16 frames wait, gives the lowest p99
+  // latency from 3k to 6.5k frames/s offered (p99 8.6 ms at 4.4k against
+  // 11.0 ms with three always queued), and the same saturation rate.
+  sv->max_in_flight = 2;
+  sv->min_ahead = 16;
   sv->gn_group = 0;
   sv->streams.resize(max_streams);
   sv->next_batch = 0;
@@ -878,6 +886,14 @@ int yas_server_set_max_batch(yas_server* sv, int max_batch) {
   return YAS_OK;
 }
 
+int yas_server_set_max_in_flight(yas_server* sv, int batches, int min_ahead) {
+  YAS_REQUIRE(sv && batches >= 1 && batches <= kInFlight && min_ahead >= 1,
+              "yas_server_set_max_in_flight: 1..3 batches, min_ahead >= 1");
+  sv->max_in_flight = batches;
+  sv->min_ahead = min_ahead;
+  return YAS_OK;
+}
+
 int yas_server_set_gn_group(yas_server* sv, int slots) {
   YAS_REQUIRE(sv && slots >= 0 && slots <= kMaxSlots, "yas_server_set_gn_group: 0..64");
   if (slots == sv->gn_group) return YAS_OK;
@@ -932,11 +948,14 @@ int yas_server_pump(yas_server* sv, int* launched) {
   for (;;) {
     Batch& b = sv->batches[sv->next_batch];
     if (b.busy) break;  // kInFlight batches queued already
+    int busy = 0;
+    for (int k = 0; k < kInFlight; ++k) busy += sv->batches[k].busy ? 1 : 0;
+    if (busy >= sv->max_in_flight) break;
     // size of the next batch, and its graph, before any frame leaves its queue
     int want = 0;
     for (int i = 0; i < sv->max_streams && want < sv->max_batch; ++i)
       if (sv->streams[i].state == kActive && !sv->streams[i].waiting.empty()) ++want;
-    if (want == 0) break;
+    if (want == 0 || (busy > 0 && want < std::min(sv->min_ahead, sv->max_batch))) break;
     hipGraphExec_t ex = nullptr;
     if ((rc = graph_for(sv, want, &ex))) return rc;
     int count = 0;

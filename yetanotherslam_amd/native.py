@@ -102,6 +102,7 @@ SIGNATURES = {
     "yas_server_stream_pipeline": (I, [P, I, P]),
     "yas_server_set_max_batch": (I, [P, I]),
     "yas_server_set_gn_group": (I, [P, I]),
+    "yas_server_set_max_in_flight": (I, [P, I, I]),
     "yas_server_prepare": (I, [P]),
     "yas_server_submit": (I, [P, I, P, ctypes.c_size_t, P, ctypes.c_size_t, P]),
     "yas_server_pump": (I, [P, PI]),

@@ -346,6 +346,9 @@ class Server(_Native):
     def set_gn_group(self, n):
         call("yas_server_set_gn_group", self.h, n)
 
+    def set_max_in_flight(self, n, min_ahead=1):
+        call("yas_server_set_max_in_flight", self.h, n, min_ahead)
+
     def set_max_batch(self, n):
         call("yas_server_set_max_batch", self.h, n)
 
