@@ -126,6 +126,31 @@ def test_reserve_hibernate_and_file_round_trips_are_exact(tmp_path):
         assert_bits_equal(w, we, "weight after further fusion")
 
 
+def test_load_into_a_large_pool_keeps_every_voxel(tmp_path):
+    """A map file read into a 2^19-block f32 pool (2 GiB): creating the map
+    clears the whole pool on the library's stream (about 0.5 ms), and the
+    file's blocks are uploaded after that on the same stream, so none of
+    them is cleared again (yas_map_load)."""
+    p, K, fr = frames(2)
+    gt = synth.relative_poses(p, 2)
+    m = rf.MapStruct(K.astype(np.float32), 1 << 20, 1 << 19, VOX, MU, voxel_format=rf.VOXEL_F32_F32)
+    for t in range(2):
+        m.Fuse(to_dev(po.depth_u16_to_f32(fr[t][0])), gt[t])
+    k0, s0, w0 = m.export_sorted()
+    path = tmp_path / "big.yas"
+    m.SaveToFile(path)
+    m.close()
+    l = rf.MapStruct.ReadFromFile(path)
+    P = rf.MapParams()
+    native.call("yas_map_get_params", l.h, rf.ctypes.byref(P), None)
+    assert P.num_blocks == 1 << 19 and len(k0) > 1000
+    k, sv, w = l.export_sorted()
+    np.testing.assert_array_equal(k, k0)
+    assert_bits_equal(sv, s0, "sdf after loading into a large pool")
+    assert_bits_equal(w, w0, "weight after loading into a large pool")
+    l.close()
+
+
 def test_ply_file_holds_the_indexed_mesh(tmp_path):
     p, K, fr = frames(2)
     gt = synth.relative_poses(p, 2)
