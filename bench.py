@@ -141,24 +141,35 @@ def rooflines(stats, kernel_ms, stage_ms, steps, traffic):
 def mesh_bench(pl, dev_depth, K, nframes, reps=3):
     """Marching cubes over the fused 5 mm volume (float32 sdf, the timed
     run's own map) and over the same frames fused at the tracked poses into a
-    float16 sdf + uint16 weight map: indexed output (float32 vertices, int32
-    triangles), and the reference's triangle soup beside it.  Wall time of
-    the synchronous extraction call (kernels + one 8-byte readback), median
-    of `reps` after one untimed call."""
+    float16 sdf + uint16 weight map, then over a volume of the size marching
+    cubes is quoted on (workloads.wall_frames: about 267k blocks) in both
+    storages: indexed output (float32 vertices, int32 triangles), and the
+    reference's triangle soup beside it.  Wall time of the synchronous
+    extraction call (kernels + one 8-byte readback), median of `reps` after
+    one untimed call."""
     import ctypes
     import torch
     from yetanotherslam_amd import rgbdfusion as rf
+    from yetanotherslam_amd import workloads as wl
 
     f16 = rf.MapStruct(np.asarray(K, np.float32), BUCKETS, BLOCKS, VOXEL, TRUNC, voxel_format=rf.VOXEL_F16_U16)
     for t in range(nframes):
         T, ok = pl.pose(t)
         if ok:
             f16.FuseU16(dev_depth[t], T)
+    wall = {}
+    for name, fmt in (("wall_f32", rf.VOXEL_F32_F32), ("wall_f16u16", rf.VOXEL_F16_U16)):
+        wall[name] = rf.MapStruct(np.asarray(K, np.float32), BUCKETS, BLOCKS, VOXEL, TRUNC, voxel_format=fmt)
+    for d, T in wl.wall_frames(K):
+        dd = torch.from_numpy(d).cuda()
+        for m in wall.values():
+            m.Fuse(dd, T)
     torch.cuda.synchronize()
-    me = rf.MeshEngine(8_000_000)
+    me = rf.MeshEngine(32_000_000)
     nv, nt, nb = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
     out = {}
-    for name, h, sv in (("f32", pl.map_handle(), 8), ("f16u16", f16.h, 4)):
+    for name, h, sv in (("f32", pl.map_handle(), 8), ("f16u16", f16.h, 4), ("wall_f32", wall["wall_f32"].h, 8),
+                        ("wall_f16u16", wall["wall_f16u16"].h, 4)):
         ms = {}
         for kind in ("indexed", "soup"):
             ts = []
@@ -175,6 +186,10 @@ def mesh_bench(pl, dev_depth, K, nframes, reps=3):
         out[name] = {"blocks": nb.value, "triangles": nt.value, "vertices": nv.value,
                      "ms_indexed": round(ms["indexed"], 3), "ms_soup": round(ms["soup"], 3),
                      "voxel_read_GBps_indexed": round(2 * nb.value * 512 * sv / (ms["indexed"] * 1e-3) / 1e9, 1)}
+    me.close()
+    f16.close()
+    for m in wall.values():
+        m.close()
     return out
 
 

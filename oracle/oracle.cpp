@@ -226,6 +226,15 @@ template <> struct Vox<VRef> {
   typedef float R;
   static float sdf(const VRef& v) { return (float)v.s / 32767.0f; }
   static float wt(const VRef& v) { return (float)v.w; }
+  // the voxel whose decoded values are (s, w), for values read back from
+  // the same storage (the nearest int16 of s * 32767 is the stored one)
+  static VRef make(double s, double w) {
+    VRef v;
+    v.s = (int16_t)std::lround(s * 32767.0);
+    v.w = (uint8_t)w;
+    v.pad = 0;
+    return v;
+  }
   static int16_t pack(float f) {
     int i = trunc_sat(f * 32767.0f);
     return (int16_t)std::max(-32768, std::min(32767, i));
@@ -253,6 +262,7 @@ template <> struct Vox<VF32> {
   typedef float R;
   static float sdf(const VF32& v) { return v.s; }
   static float wt(const VF32& v) { return v.w; }
+  static VF32 make(double s, double w) { return VF32{(float)s, (float)w}; }
   static void update(VF32& v, float sdf, int maxw) {
     float w = v.w;
     if (w == 0.0f) {
@@ -273,6 +283,7 @@ template <> struct Vox<VF16> {
   typedef float R;
   static float sdf(const VF16& v) { return f16_to_f32(v.s); }
   static float wt(const VF16& v) { return (float)v.w; }
+  static VF16 make(double s, double w) { return VF16{f32_to_f16((float)s), (uint16_t)w}; }
   static void update(VF16& v, float sdf, int maxw) {
     int w = v.w;
     if (w == 0) {
@@ -295,6 +306,7 @@ template <> struct Vox<VF64> {
   typedef double R;
   static double sdf(const VF64& v) { return v.s; }
   static double wt(const VF64& v) { return v.w; }
+  static VF64 make(double s, double w) { return VF64{s, w}; }
   static void update(VF64& v, double sdf, int maxw) {
     double w = v.w;
     if (w == 0.0) {
@@ -399,6 +411,7 @@ struct MapBase {
   virtual int visible(int W, int H, const double K[4], const double T[16], int32_t* keys_out, int cap) = 0;
   virtual int num_blocks() const = 0;
   virtual int export_blocks(int32_t* keys, double* sdf, double* wt, int cap) const = 0;
+  virtual void import_blocks(const int32_t* keys, const double* sdf, const double* wt, int n) = 0;
   virtual int raycast(int W, int H, const double K[4], const double T[16], double* vmap, double* zmin,
                       double* zmax) = 0;
   virtual long mesh(int32_t* block_keys, int32_t* tri_counts, int16_t* cases, double* tris, double* normals,
@@ -688,6 +701,17 @@ template <class V> struct Map : MapBase {
         for (int j = 0; j < 512; ++j) wt[(size_t)i * 512 + j] = (double)Vox<V>::wt((*v[i].second)[j]);
     }
     return n;
+  }
+
+  // Test set-up: blocks given by keys and decoded voxel values (as
+  // export_blocks gives them), replacing any block of the same key.  Lets
+  // the read-only functions (ray cast, marching cubes) run on a volume
+  // built elsewhere.
+  void import_blocks(const int32_t* keys, const double* sdf, const double* wt, int n) override {
+    for (int i = 0; i < n; ++i) {
+      Block& b = blocks[pack_key(keys[3 * i], keys[3 * i + 1], keys[3 * i + 2])];
+      for (int j = 0; j < 512; ++j) b[j] = Vox<V>::make(sdf[(size_t)i * 512 + j], wt[(size_t)i * 512 + j]);
+    }
   }
 
   // ------------------------------------------------------------ raycast
@@ -1562,6 +1586,9 @@ int orc_map_fuse_map(void* dst, void* src, const double Tdst[16], const double T
 }
 int orc_map_export(void* m, int32_t* keys, double* sdf, double* wt, int cap) {
   return ((MapBase*)m)->export_blocks(keys, sdf, wt, cap);
+}
+void orc_map_import(void* m, const int32_t* keys, const double* sdf, const double* wt, int n) {
+  ((MapBase*)m)->import_blocks(keys, sdf, wt, n);
 }
 int orc_map_raycast(void* m, int W, int H, const double K[4], const double T[16], double* vmap, double* zmin,
                     double* zmax) {

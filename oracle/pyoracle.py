@@ -51,6 +51,7 @@ def _load(path):
         "orc_map_num_blocks": (ctypes.c_int, [P]),
         "orc_map_fuse_map": (ctypes.c_int, [P, P, P, P]),
         "orc_map_export": (ctypes.c_int, [P, P, P, P, ctypes.c_int]),
+        "orc_map_import": (None, [P, P, P, P, ctypes.c_int]),
         "orc_map_raycast": (ctypes.c_int, [P, ctypes.c_int, ctypes.c_int, P, P, P, P, P]),
         "orc_map_mesh": (ctypes.c_long, [P, P, P, P, P, P, ctypes.c_long, ctypes.c_int]),
         "orc_ref_hash": (ctypes.c_int, [ctypes.c_int] * 4),
@@ -149,6 +150,16 @@ class OracleMap:
         if with_voxels:
             return keys[:n], sdf[:n], wt[:n]
         return keys[:n]
+
+    def import_blocks(self, keys, sdf, wt):
+        """Blocks (n, 3) int32 keys with (n, 512) decoded sdf and weight
+        values, as export gives them (test set-up for the read-only
+        functions on a volume built elsewhere)."""
+        k = np.ascontiguousarray(keys, np.int32)
+        s = np.ascontiguousarray(sdf, np.float64)
+        w = np.ascontiguousarray(wt, np.float64)
+        assert k.shape[0] == s.shape[0] == w.shape[0] and s.shape[1] == w.shape[1] == 512
+        lib().orc_map_import(self.h, _p(k), _p(s), _p(w), k.shape[0])
 
     def raycast(self, W, H, K, T):
         vmap = np.zeros((H, W, 4), np.float64)

@@ -113,6 +113,37 @@ def test_mesh_of_the_300_frame_room_bit_exact(fused_room, fmt):
     me.close()
 
 
+def test_mesh_of_a_c3_sized_volume_bit_exact():
+    """Marching cubes at the size c3 names (200k-400k blocks at 5 mm): the
+    wavy wall of workloads.wall_frames (a 7 x 6 grid of camera positions
+    2.4 m x 1.8 m apart), in float16 sdf + uint16 weight storage.  The oracle meshes the same volume
+    (imported from the device): the block set, every case index and
+    triangle count, and every vertex and normal are bit-exact."""
+    K = synth.intrinsics(wl.synth_params())
+    m = rf.MapStruct(K.astype(np.float32), wl.BUCKETS, wl.BLOCKS, wl.VOXEL, wl.TRUNC, voxel_format=rf.VOXEL_F16_U16)
+    for d, T in wl.wall_frames(K):
+        m.Fuse(to_dev(d), T)
+    torch.cuda.synchronize()
+    keys, sdf, wt = m.export_sorted()
+    assert len(keys) >= 200_000, len(keys)
+    om = po.OracleMap(wl.VOXEL, wl.TRUNC, precision=32, voxfmt=po.VOX_F16)
+    om.import_blocks(keys, sdf.astype(np.float64), wt.astype(np.float64))
+    del sdf, wt
+    mo = om.mesh()
+    me = rf.MeshEngine(len(mo["tris"]) + 1)
+    mg = me.Meshify(m, with_cases=True)
+    gk, counts, cases, tris, normals = rf.canonical_mesh(mg)
+    print("c3-sized volume: %d blocks, %d triangles" % (len(keys), mg["num_triangles"]))
+    np.testing.assert_array_equal(mo["keys"], gk)
+    np.testing.assert_array_equal(mo["cases"], cases)
+    np.testing.assert_array_equal(mo["counts"], counts)
+    assert mg["num_triangles"] == len(mo["tris"])
+    assert_bits_equal(mo["tris"], tris, "vertices")
+    assert_bits_equal(mo["normals"], normals, "normals")
+    me.close()
+    m.close()
+
+
 # ------------------------------------------------------------------ batching
 
 def test_64_submaps_batched_equal_one_at_a_time_and_the_oracle(hi_frames, lo_frames):

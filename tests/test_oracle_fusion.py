@@ -275,3 +275,27 @@ def test_map_merge_weights_average_two_maps():
     np.testing.assert_array_equal(wa2[merged], 4.0)
     np.testing.assert_allclose(sa2[merged], sa[merged], rtol=0, atol=1e-6)
     assert np.all(wa2[~merged] == wa[~merged]) and np.all(sa2[~merged] == sa[~merged])
+
+
+@pytest.mark.parametrize("voxfmt", [po.VOX_F32, po.VOX_F16, po.VOX_REF], ids=["f32", "f16u16", "i16u8"])
+def test_import_rebuilds_an_exported_volume(voxfmt):
+    """import_blocks (test set-up for meshing a volume built on the device)
+    restores exactly what export gives, in every storage: the same blocks and
+    voxels, and so the same mesh."""
+    p = synth.params(width=160, height=120)
+    K = synth.intrinsics(p)
+    gt = synth.relative_poses(p, 2)
+    src = po.OracleMap(0.01, 0.03, precision=32, voxfmt=voxfmt)
+    for t in range(2):
+        src.fuse(po.depth_u16_to_f32(synth.frame(p, t)[0]), K, gt[t])
+    k, s, w = src.export()
+    dst = po.OracleMap(0.01, 0.03, precision=32, voxfmt=voxfmt)
+    dst.import_blocks(k, s, w)
+    k2, s2, w2 = dst.export()
+    np.testing.assert_array_equal(k2, k)
+    np.testing.assert_array_equal(s2, s)
+    np.testing.assert_array_equal(w2, w)
+    m1, m2 = src.mesh(), dst.mesh()
+    assert len(k) > 500 and len(m1["tris"]) > 1000
+    for name in ("keys", "cases", "counts", "tris", "normals"):
+        np.testing.assert_array_equal(m2[name], m1[name])

@@ -10,6 +10,9 @@ them, so that what is timed is what is tested.
 * serving: 32 streams, each a segment of the room sequence from its own
   seeded start frame, at 640x480 or 320x240 (seeded), 2^20 buckets and a
   2^16-block pool per stream.
+* a volume of the size marching cubes is quoted on (200k-400k blocks at
+  5 mm; the room holds about 100k): a wavy wall fused from a 7 x 6 grid of
+  camera positions, about 267k blocks.
 """
 import math
 
@@ -28,6 +31,32 @@ BATCH_MAPS, BATCH_SEED = 64, 1
 BATCH_BUCKETS = 1 << 18
 SERVE_STREAMS, SERVE_SEED = 32, 2
 SERVE_BUCKETS, SERVE_BLOCKS = 1 << 20, 1 << 16
+
+
+WALL_GRID, WALL_STEP = (7, 6), (2.4, 1.8)   # camera positions and their spacing (m)
+
+
+def wavy_wall_depth(K, cx, cy, width=W, height=H, iters=12):
+    """Depth (float32 metres) of the wall z = 2 + 0.15 sin(2 pi x / 0.7)
+    cos(2 pi y / 0.9) m seen by a camera at (cx, cy, 0) looking along +z, by
+    fixed-point iteration along each pixel's ray."""
+    u, v = np.meshgrid(np.arange(width, dtype=np.float64), np.arange(height, dtype=np.float64))
+    dx, dy = (u - K[2]) / K[0], (v - K[3]) / K[1]
+    t = np.full((height, width), 2.0)
+    for _ in range(iters):
+        t = 2.0 + 0.15 * np.sin(2 * np.pi * (cx + t * dx) / 0.7) * np.cos(2 * np.pi * (cy + t * dy) / 0.9)
+    return t.astype(np.float32)
+
+
+def wall_frames(K):
+    """(depth, camera-to-world pose) of every camera position of the wall."""
+    out = []
+    for i in range(WALL_GRID[0]):
+        for j in range(WALL_GRID[1]):
+            T = np.eye(4)
+            T[0, 3], T[1, 3] = WALL_STEP[0] * i, WALL_STEP[1] * j
+            out.append((wavy_wall_depth(K, T[0, 3], T[1, 3]), T))
+    return out
 
 
 def synth_params(width=W, height=H):
