@@ -114,12 +114,13 @@ __global__ void __launch_bounds__(256) k_step_pass1(Level L, TrackBufs B, StepAr
                                                     float* part_res, float angle_th, float dist_th) {
   pass1_at<ICP, RGB, true, true>(L, B, nullptr, part_icp, part_res, angle_th, dist_th, blockIdx.x, gridDim.x, &a);
 }
-// A frame's tracker inputs in two launches per level instead of seven:
-// first the pointwise products of level l from level l - 1 (or from the
+// A frame's tracker inputs in levels + 1 launches instead of seven per
+// level: the pointwise products of level l from level l - 1 (or from the
 // frame at level 0): intensity, inverse depth and vertices of the current
-// frame, and the model's vertices; then the neighbourhood products of level
-// l: gradients and both ICP point maps (vertex and normal).  Each pixel gets
-// the value the separate kernels give it.
+// frame, and the model's vertices; and the neighbourhood products of level
+// l: gradients and both ICP point maps (vertex and normal), which need the
+// products of level l but nothing of level l + 1, so they share a launch
+// with those.  Each pixel gets the value the separate kernels give it.
 struct FrameLevel {
   int l, w, h, pw, ph;  // level, its size, the size of level l - 1
   float invfx, invfy, cx, cy;
@@ -141,8 +142,7 @@ struct FrameLevel {
   float4* cur_p;  // ICP point maps
   float4* ref_p;
 };
-__global__ void __launch_bounds__(256) k_frame_level(FrameLevel a) {
-  const int x = blockIdx.x * 64 + (threadIdx.x & 63), y = blockIdx.y * 4 + (threadIdx.x >> 6);
+__device__ inline void frame_level_px(const FrameLevel& a, int x, int y) {
   if (x >= a.w || y >= a.h) return;
   const size_t k = (size_t)y * a.w + x;
   float in, iv;
@@ -161,8 +161,7 @@ __global__ void __launch_bounds__(256) k_frame_level(FrameLevel a) {
   a.cur_v[k] = vertex_of(iv, x, y, a.invfx, a.invfy, a.cx, a.cy, 3.0f);
   a.ref_v[k] = rv;
 }
-__global__ void __launch_bounds__(256) k_frame_level_nb(FrameLevel a) {
-  const int x = blockIdx.x * 64 + (threadIdx.x & 63), y = blockIdx.y * 4 + (threadIdx.x >> 6);
+__device__ inline void frame_level_nb_px(const FrameLevel& a, int x, int y) {
   if (x >= a.w || y >= a.h) return;
   const size_t k = (size_t)y * a.w + x;
   float gxv, gyv;
@@ -171,6 +170,26 @@ __global__ void __launch_bounds__(256) k_frame_level_nb(FrameLevel a) {
   a.gy[k] = gyv;
   store_icp_point(a.cur_p, a.w * a.h, (int)k, icp_point_px(a.cur_v, (size_t)a.w * 16, a.w, a.h, x, y));
   store_icp_point(a.ref_p, a.w * a.h, (int)k, icp_point_px(a.ref_v, (size_t)a.w * 16, a.w, a.h, x, y));
+}
+__global__ void __launch_bounds__(256) k_frame_level(FrameLevel a) {
+  frame_level_px(a, blockIdx.x * 64 + (threadIdx.x & 63), blockIdx.y * 4 + (threadIdx.x >> 6));
+}
+__global__ void __launch_bounds__(256) k_frame_level_nb(FrameLevel a) {
+  frame_level_nb_px(a, blockIdx.x * 64 + (threadIdx.x & 63), blockIdx.y * 4 + (threadIdx.x >> 6));
+}
+// Level l's pointwise products and level l - 1's neighbourhood products,
+// which do not depend on each other, in one launch: the first `nprod`
+// workgroups take the products' 64x4 tiles, the others the neighbourhood
+// products'.
+__global__ void __launch_bounds__(256) k_frame_level_pair(FrameLevel prod, FrameLevel nb, int nprod) {
+  const int b = blockIdx.x;
+  if (b < nprod) {
+    const int tx = (prod.w + 63) >> 6;
+    frame_level_px(prod, (b % tx) * 64 + (threadIdx.x & 63), (b / tx) * 4 + (threadIdx.x >> 6));
+  } else {
+    const int tx = (nb.w + 63) >> 6, c = b - nprod;
+    frame_level_nb_px(nb, (c % tx) * 64 + (threadIdx.x & 63), (c / tx) * 4 + (threadIdx.x >> 6));
+  }
 }
 __global__ void k_track_init(TrackState* st, Mat44 T) { track_init_at(st, T.m); }
 __global__ void k_track_init_identity(TrackState* st) { track_init_identity_at(st); }
@@ -416,10 +435,17 @@ int yas_tracker_prepare_frame_dev(yas_tracker* t, const float* vmap, size_t vpit
     f.refv_prev = l > 0 ? t->ref_v[l - 1] : nullptr;
     f.cur_int = t->cur_int[l]; f.cur_inv = t->cur_inv[l]; f.cur_v = t->cur_v[l]; f.ref_v = t->ref_v[l];
     f.gx = t->gx[l]; f.gy = t->gy[l]; f.cur_p = t->cur_p[l]; f.ref_p = t->ref_p[l];
-    hipLaunchKernelGGL(k_frame_level, grid64x4(v.w, v.h), dim3(256), 0, s, f);
   }
-  for (int l = 0; l < t->P.levels; ++l)
-    hipLaunchKernelGGL(k_frame_level_nb, grid64x4(t->lv[l].w, t->lv[l].h), dim3(256), 0, s, a[l]);
+  // level 0's products; then level l's products beside level l - 1's
+  // neighbourhood products; then the coarsest level's neighbourhood products
+  hipLaunchKernelGGL(k_frame_level, grid64x4(a[0].w, a[0].h), dim3(256), 0, s, a[0]);
+  for (int l = 1; l < t->P.levels; ++l) {
+    const dim3 gp = grid64x4(a[l].w, a[l].h), gn = grid64x4(a[l - 1].w, a[l - 1].h);
+    const int nprod = (int)(gp.x * gp.y);
+    hipLaunchKernelGGL(k_frame_level_pair, dim3(nprod + gn.x * gn.y), dim3(256), 0, s, a[l], a[l - 1], nprod);
+  }
+  hipLaunchKernelGGL(k_frame_level_nb, grid64x4(a[t->P.levels - 1].w, a[t->P.levels - 1].h), dim3(256), 0, s,
+                     a[t->P.levels - 1]);
   YAS_LAUNCH_CHECK();
   return YAS_OK;
 }
