@@ -87,8 +87,17 @@ def cpu_baseline(p, K, depth, rgb, budget_s=12.0, all_cores=False):
 # carried here, keyed by the run they were measured on: (warmup, steps),
 # i.e. the timed frames warmup+1 .. warmup+steps of the sequence.  Other runs
 # report no traffic figure.
-TRAFFIC_BYTES = {(10, 290): {"k_march": 54.49e6, "k_integrate": 111.31e6, "k_alloc": 4.45e6},
-                 (5, 20): {"k_march": 38.37e6, "k_integrate": 89.15e6, "k_alloc": 4.42e6}}
+TRAFFIC_BYTES = {(10, 290): {"k_march": 54.49e6, "k_integrate": 111.32e6, "k_alloc": 4.45e6},
+                 (5, 20): {"k_march": 38.43e6, "k_integrate": 89.15e6, "k_alloc": 4.42e6}}
+
+# VALU instructions (wavefront instructions summed over the GPU) per launch
+# of the ray march, from a rocprofv3 SQ_INSTS_VALU pass over this script's
+# own run (profiles/k_march_640x480_sq_counters.md), keyed as above.  The
+# march is bound by the latency of its slowest rays' dependent reads, so
+# neither HBM nor VALU issue saturates; both fractions are reported.
+VALU_INSTS = {(10, 290): {"k_march": 22.68e6}, (5, 20): {"k_march": 16.21e6}}
+# one wave64 VALU instruction per SIMD every 4 cycles: 256 CUs x 4 SIMDs x 2.4 GHz / 4
+VALU_PEAK_G = 614.4
 
 
 def walk_steps(stats):
@@ -136,6 +145,19 @@ def rooflines(stats, kernel_ms, stage_ms, steps, traffic):
                      "algorithmic_bytes_per_launch": round(per_launch),
                      "avg_launch_ms": round(kernel_ms[k], 5), "avg_stage_ms": round(stage_ms[i], 5)}
     return out
+
+
+def valu_issue(kernel, avg_launch_ms, run):
+    """The kernel's VALU issue rate against MI355X's peak: measured
+    instructions per launch (VALU_INSTS, for the frame range it was counted
+    on) over this run's average launch time; None for other ranges."""
+    n = VALU_INSTS.get(run, {}).get(kernel)
+    if n is None or avg_launch_ms <= 0:
+        return None
+    ach = n / (avg_launch_ms * 1e-3) / 1e9
+    return {"achieved": round(ach, 1), "peak": VALU_PEAK_G, "unit": "G wavefront instructions/s",
+            "frac": round(ach / VALU_PEAK_G, 4), "instructions_per_launch": n,
+            "source": "rocprofv3 SQ_INSTS_VALU of this run's kernels (frames %d-%d)" % (run[0] + 1, run[0] + run[1])}
 
 
 def mesh_bench(pl, dev_depth, K, nframes, reps=3):
@@ -548,7 +570,8 @@ def main():
         "roofline": dict(rl[dominant], kernel=dominant,
                          traffic_source=("rocprofv3 FETCH_SIZE + WRITE_SIZE of this run's kernels (frames %d-%d), "
                                          "copy-calibrated" % (args.warmup + 1, args.warmup + args.steps))
-                         if traffic else "not profiled for this frame range"),
+                         if traffic else "not profiled for this frame range",
+                         valu=valu_issue(dominant, rl[dominant]["avg_launch_ms"], (args.warmup, args.steps))),
         "roofline_by_kernel": rl,
     }
     if not args.no_serving:

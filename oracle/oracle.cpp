@@ -854,8 +854,12 @@ template <class V> struct Map : MapBase {
     }
   }
 
+  // The z-range grid has (W/8)·(H/8) cells and RayTraceEngine.cu:340-344
+  // reads cell (x/8, y/8) for every pixel, past the grid when W or H is not
+  // a multiple of 8; such sizes are refused (-1), as the library does.
   int raycast(int W, int H, const double Kd[4], const double T[16], double* vmap, double* zmin_o,
               double* zmax_o) override {
+    if (W < 8 || H < 8 || W % 8 != 0 || H % 8 != 0) return -1;
     Intr<R> K = intr_from<R>(Kd[0], Kd[1], Kd[2], Kd[3]);
     Pose<R> Tf = pose_from<R>(T);
     double Ti[16];
@@ -1694,6 +1698,7 @@ void orc_rgb_to_intensity(const uint8_t* rgb, float* g, long n, int mode) { rgb_
 // Pipeline: map params (voxel, trunc, maxw, voxfmt), tracker params.
 void* orc_pipe_new(int precision, int voxfmt, double voxel, double trunc, int maxw, int W, int H, const double K[4],
                    int L, const int* iters, int mode, const float wts[4], double depth_scale, int gray_mode) {
+  if (W < 8 || H < 8 || W % 8 != 0 || H % 8 != 0) return nullptr;  // the ray cast's z-range grid
   Pipeline* p = new Pipeline();
   p->map.reset((MapBase*)orc_map_new(precision, voxfmt, voxel, trunc, maxw, 0.1, 3.0));
   p->trk.reset((TrackerBase*)orc_trk_new(precision, W, H, K, L, iters, mode, wts));

@@ -165,7 +165,8 @@ class OracleMap:
         vmap = np.zeros((H, W, 4), np.float64)
         zmin = np.zeros((H // 8, W // 8), np.float64)
         zmax = np.zeros((H // 8, W // 8), np.float64)
-        lib().orc_map_raycast(self.h, W, H, _p(_f64(K)), _p(_f64(T)), _p(vmap), _p(zmin), _p(zmax))
+        if lib().orc_map_raycast(self.h, W, H, _p(_f64(K)), _p(_f64(T)), _p(vmap), _p(zmin), _p(zmax)) < 0:
+            raise ValueError("raycast: width and height must be multiples of 8, got %dx%d" % (W, H))
         return vmap, zmin, zmax
 
     def mesh(self, tri_cap=None):
@@ -327,6 +328,8 @@ class OraclePipeline:
         self.L = lib_omp() if all_cores else lib()
         self.h = self.L.orc_pipe_new(precision, voxfmt, voxel, trunc, max_weight, W, H, _p(_f64(K)), levels,
                                      _p(it), mode, _p(w), depth_scale, gray_mode)
+        if not self.h:
+            raise ValueError("pipeline: width and height must be multiples of 8, got %dx%d" % (W, H))
         self.W, self.H = W, H
 
     def __del__(self):

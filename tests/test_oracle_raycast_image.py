@@ -32,8 +32,8 @@ def test_raycast_plane_hits_the_analytic_surface():
 
 
 def test_raycast_sphere_hits_the_analytic_surface():
-    W, H = 80, 60
-    K = np.array([70.0, 70.0, 39.5, 29.5])
+    W, H = 80, 64
+    K = np.array([70.0, 70.0, 39.5, 31.5])
     c, r = np.array([0.05, -0.03, 1.0]), 0.25
     ys, xs = np.mgrid[0:H, 0:W]
     d = np.stack([(xs - K[2]) / K[0], (ys - K[3]) / K[1], np.ones_like(xs, float)], -1)
@@ -51,6 +51,18 @@ def test_raycast_sphere_hits_the_analytic_surface():
     assert hit.sum() > 0.8 * (z > 0).sum()
     dist = np.abs(np.linalg.norm(v[..., :3][hit] - c, axis=1) - r)
     assert np.percentile(dist, 99) <= 0.5 * 0.005
+
+
+def test_raycast_refuses_sizes_the_zrange_grid_does_not_cover():
+    # (W/8)*(H/8) cells: the reference reads cell (x/8, y/8) for every pixel,
+    # past the grid for the last H % 8 rows; the oracle refuses such sizes,
+    # as the library does, instead of reading there.
+    om, K, _, _ = plane_map()
+    for W, H in ((64, 60), (60, 48), (4, 8)):
+        with pytest.raises(ValueError):
+            om.raycast(W, H, K, np.eye(4))
+    with pytest.raises(ValueError):
+        po.OraclePipeline(80, 60, K, 0.01, 0.03, 1, (1,), po.MODE_DEPTH)
 
 
 def test_zrange_tiles_equal_numpy_brute_force():
