@@ -152,6 +152,26 @@ __device__ inline void visible_scan(const MapView& m, const PoseF& Ti, const Int
   }
 }
 
+// The first visible block of this thread's wavefront in integrate_waves,
+// for workgroup b of `groups` (the wavefronts then stride by groups times
+// the wavefronts per workgroup).  When every visible block gets a wavefront
+// of its own, XCD j (which runs workgroups b with b % 8 == j) takes the j-th
+// eighth of the visible list: its blocks, listed in allocation order,
+// project near one another, so each XCD's L2 holds part of the depth image
+// rather than all of it.  Idle wavefronts get an index past the list.
+__device__ inline int integrate_wave(const MapView& m, int b, int groups) {
+  const int wpg = blockDim.x >> 6;
+  const int nv0 = m.ctr[kCtrVisible];
+  const int nv = nv0 < m.num_blocks ? nv0 : m.num_blocks;
+  const int per = ((nv + wpg - 1) / wpg + 7) >> 3;  // workgroups with a block, per XCD
+  int g = b;
+  if (per * 8 <= groups) {
+    const int j = b & 7, k = b >> 3;
+    g = k < per ? j * per + k : groups;
+  }
+  return g * wpg + (threadIdx.x >> 6);
+}
+
 // K5 (VoxelMap.cu:719-786): one wavefront per visible block, lane = (x, y),
 // looping over the 8 z-slices; each slice is 64 consecutive voxels.  After
 // the update the wavefront re-derives the block's summary flags from the

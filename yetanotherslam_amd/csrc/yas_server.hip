@@ -328,12 +328,14 @@ __global__ void __launch_bounds__(256) k_s_alloc(const StreamDev* streams, const
   if (threadIdx.x == 0 && walked) atomicAdd(&S.m.ctr[kCtrWalk], walked);
 }
 
+// At least six wavefronts per SIMD, as k_integrate.
 template <class VT>
-__global__ void __launch_bounds__(256) k_s_integrate(const StreamDev* streams, const Table* tab) {
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6)))
+k_s_integrate(const StreamDev* streams, const Table* tab) {
   SLOT_OR_RETURN;
   if ((int)blockIdx.x >= S.int_groups) return;
   integrate_waves<VT>(S.m, S.pose->Tinv, S.K, DepthF32{S.depth, (size_t)S.W * 4}, S.W, S.H,
-                      blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6), S.int_groups * (blockDim.x >> 6));
+                      integrate_wave(S.m, blockIdx.x, S.int_groups), S.int_groups * (blockDim.x >> 6));
 }
 
 inline dim3 g64x4(int W, int H, int z) { return dim3((W + 63) / 64, (H + 3) / 4, z); }
