@@ -151,6 +151,50 @@ def test_load_into_a_large_pool_keeps_every_voxel(tmp_path):
     l.close()
 
 
+def read_map_file(path, fmt):
+    """The saved volume: (keys (n, 3) int32, summary flags (n,) int32, sdf
+    and weight (n, 512) float32) in pool order.  The header is 224 bytes
+    with the block count at byte 24; then a (key, flags) record of 4 int32
+    per block; then the voxels."""
+    data = path.read_bytes()
+    n = int(np.frombuffer(data, np.int32, 1, 24)[0])
+    vb = 8 if fmt == rf.VOXEL_F32_F32 else 4
+    assert len(data) == 224 + 16 * n + 512 * vb * n
+    rec = np.frombuffer(data, np.int32, 4 * n, 224).reshape(n, 4)
+    raw = np.frombuffer(data, np.uint8, 512 * vb * n, 224 + 16 * n).reshape(n, 512 * vb)
+    sdf, wt = rf.decode_voxels(raw.copy(), fmt)
+    return rec[:, :3], rec[:, 3], sdf, wt
+
+
+@pytest.mark.parametrize("fmt", [f for f, _ in FMTS], ids=["i16u8", "f32", "f16u16"])
+def test_block_summaries_equal_a_recount(tmp_path, fmt):
+    """Each block's summary (all unobserved / all free), which the ray cast
+    trusts in place of the voxels, is kept by allocation and integration and
+    carried over by a growth of the table.  After frames fused before and
+    after a growth, every block's summary in the saved file equals a recount
+    over its 512 voxels, and both kinds of summary occur."""
+    n = 14
+    p, K, fr = frames(n)
+    gt = synth.relative_poses(p, n)
+    m = rf.MapStruct(K.astype(np.float32), 1 << 16, 1 << 14, VOX, MU, voxel_format=fmt)
+    for t in range(n):
+        if t == 8:
+            m.Reserve(1 << 17, 1 << 15)
+        m.Fuse(to_dev(po.depth_u16_to_f32(fr[t][0])), gt[t])
+    torch.cuda.synchronize()
+    path = tmp_path / "summaries.yasmap"
+    m.SaveToFile(path)
+    keys, flags, sdf, wt = read_map_file(path, fmt)
+    unobserved = (wt == 0).all(axis=1)
+    free = ((wt > 0) & (sdf == 1.0)).all(axis=1)
+    expect = unobserved.astype(np.int32) | (free.astype(np.int32) << 1)
+    bad = np.nonzero(flags != expect)[0]
+    assert bad.size == 0, "%d of %d blocks: first %s flags %d, recount %d" % (
+        bad.size, len(flags), keys[bad[0]], flags[bad[0]], expect[bad[0]])
+    assert unobserved.any() and free.any() and (expect == 0).any()
+    m.close()
+
+
 def test_ply_file_holds_the_indexed_mesh(tmp_path):
     p, K, fr = frames(2)
     gt = synth.relative_poses(p, 2)
