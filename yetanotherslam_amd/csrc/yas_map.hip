@@ -187,8 +187,8 @@ __global__ void __launch_bounds__(256) k_visible(MapView m, const PosePair* pose
 // four counters (allocated, visible, flags, walk steps), final once
 // allocation and visibility have run.
 template <class VT>
-// At least six wavefronts per SIMD (73 registers, no spills; unbounded the
-// kernel took 84 and five).
+// At least six wavefronts per SIMD (the float storage's kernel takes 72
+// registers, seven wavefronts, no spills; unbounded it took 84 and five).
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6)))
 k_integrate(MapView m, const PosePair* pose, Intr K, const float* depth, size_t pitch, int W, int H, int* stats) {
   if (stats && blockIdx.x == 0 && threadIdx.x < 4) stats[threadIdx.x] = m.ctr[threadIdx.x];
