@@ -88,6 +88,49 @@ def test_raycast_bit_exact():
     assert_bits_equal(vo[valid][:, :3], vg[valid][:, :3], "vertex")
 
 
+def test_raycast_bit_exact_with_partial_tiles():
+    """200x136 is a multiple of 8 but not of 16, so the march's last column
+    and row of 16x16 tiles lie half outside the image (their outside 8x8
+    cells do nothing): z-ranges and vertex maps still equal the oracle's."""
+    p, K, fr, gt, om, gm = fuse_both(FMTS[1], width=200, height=136)
+    rc = rf.RayTraceEngine(p.width, p.height, K.astype(np.float32))
+    T = gt[2]
+    rc.RayTrace(gm, T)
+    vg = rc.GetVMap()
+    zmin_g, zmax_g = rc.GetZRange()
+    vo, zmin_o, zmax_o = om.raycast(p.width, p.height, K, T)
+    assert vg.shape == (136, 200, 4) and zmin_g.shape == (17, 25)
+    assert_bits_equal(zmin_o, zmin_g, "zmin")
+    assert_bits_equal(zmax_o, zmax_g, "zmax")
+    np.testing.assert_array_equal(vo[..., 3], vg[..., 3])
+    valid = vo[..., 3] > 0
+    assert valid[:, 192:].any() and valid[128:, :].any()
+    assert_bits_equal(vo[valid][:, :3], vg[valid][:, :3], "vertex")
+
+
+def test_nothing_to_see_gives_misses_and_no_triangles():
+    """A map with no blocks, and a populated map seen from where none of its
+    blocks is in view: every pixel a miss (w = -1 and x = NaN, as the
+    reference writes first), every z-range cell at its initial 100 / 0, as
+    the oracle's; marching cubes of the empty map gives no triangles."""
+    p, K, fr = frames(1)
+    W, H, K32 = p.width, p.height, K.astype(np.float32)
+    empty = rf.MapStruct(K32, 1 << 14, 1 << 12, 0.008, 0.024)
+    _, _, _, gt, om, full = fuse_both(FMTS[1], n=1)
+    away = np.eye(4)
+    away[:3, :3] = np.diag([-1.0, 1.0, -1.0])  # turned round about the y axis
+    rc = rf.RayTraceEngine(W, H, K32)
+    for m, T in ((empty, np.eye(4)), (full, gt[0] @ away)):
+        rc.RayTrace(m, T)
+        v = rc.GetVMap()
+        zmin, zmax = rc.GetZRange()
+        assert (v[..., 3] == -1.0).all() and np.isnan(v[..., 0]).all()
+        assert (zmin == 100.0).all() and (zmax == 0.0).all()
+    vo, zo_min, zo_max = om.raycast(W, H, K, gt[0] @ away)
+    assert (vo[..., 3] == -1.0).all() and (zo_min == 100.0).all() and (zo_max == 0.0).all()
+    assert rf.MeshEngine(1000).Meshify(empty)["num_triangles"] == 0
+
+
 def test_image_ops_bit_exact():
     p, K, fr = frames(1)
     d, rgb, _ = fr[0]
