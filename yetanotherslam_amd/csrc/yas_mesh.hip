@@ -1,10 +1,12 @@
 // Marching-cubes mesh extraction: the reference's MeshEngine
 // (RGBDFusion/src/MeshEngine.cu:7-339, tables RGBDFusion/utils/TriangleTable.h).
 //
-// MI355X design: one 512-thread workgroup per allocated block, one thread per
-// voxel.  The +1 corners of a block's voxels live in up to seven neighbour
-// blocks; their pool indices are resolved once per workgroup (seven hash
-// probes) into LDS instead of eight probes per voxel.  Extraction is a count
+// MI355X design: one 128-thread workgroup per allocated block, four voxels
+// per thread.  The +1 corners of a block's voxels live in up to seven
+// neighbour blocks; their pool indices are resolved once per workgroup
+// (seven hash probes) instead of eight probes per voxel, and the block with
+// its +1 halo is staged in LDS, so each voxel's eight corners are LDS reads.
+// Extraction is a count
 // pass, a three-phase device scan of the per-block counts and an emit pass,
 // so every block's output is contiguous and ordered by voxel index, then
 // table order: the output is deterministic, unlike the reference's atomic
@@ -74,23 +76,81 @@ __device__ inline void edge_offset(int e, float t, float& ox, float& oy, float& 
 __constant__ int kOwner[12][4] = {{0, 0, 0, 0}, {1, 0, 0, 1}, {0, 1, 0, 0}, {0, 0, 0, 1}, {0, 0, 1, 0}, {1, 0, 1, 1},
                                   {0, 1, 1, 0}, {0, 0, 1, 1}, {0, 0, 0, 2}, {1, 0, 0, 2}, {1, 1, 0, 2}, {0, 1, 0, 2}};
 
-// The 8 corner values of this thread's voxel and a bit mask of the corners
-// that are observed (in an allocated block with weight > 0).  nb[k] = pool
-// index of the block at offset (k&1, k>>1&1, k>>2&1), -1 if absent.
+// A workgroup of kMcThreads threads takes one block; thread t owns voxels
+// l = r * kMcThreads + t (r < kMcRounds), l = z*64 + y*8 + x, so the owned
+// voxels of a round are consecutive and the rounds in order are the voxel
+// order.  The block's 512 voxels and its +1 halo (217 voxels of up to seven
+// neighbour blocks) are staged in LDS as a 9x9x9 cube of (sdf, observed),
+// index (z*9 + y)*9 + x, once per block: the corner reads of the cubes then
+// come from LDS (eight per voxel), and a workgroup of 128 threads lets four
+// times as many blocks wait on memory at once as one thread per voxel did.
+constexpr int kMcThreads = 128;
+constexpr int kMcRounds = 512 / kMcThreads;
+constexpr int kMcCube = 9 * 9 * 9;
+constexpr int kMcHalo = kMcCube - 512;
+
+struct McCube {
+  float v[kMcCube];          // sdf, 0 where not observed (absent block or weight 0)
+  unsigned char o[kMcCube];  // observed
+};
+
+// Pool indices of the block and its neighbours at offset (k&1, k>>1&1,
+// k>>2&1) in nb[k] (-1 if absent), and the staged cube.  The block's own
+// voxels are loaded while threads 0-7 probe the neighbours.
 template <class VT>
-__device__ inline int read_corners(const typename VT::T* vox, const int* nb, int lx, int ly, int lz, float* s) {
+__device__ inline void stage_cube(const MapView& m, int blk, int* nb, McCube& c, int& bx, int& by, int& bz) {
+  const typename VT::T* vox = (const typename VT::T*)m.voxels;
+  const int t = threadIdx.x;
+  typename VT::T own[kMcRounds];
+#pragma unroll
+  for (int r = 0; r < kMcRounds; ++r) own[r] = vox[(size_t)blk * 512 + r * kMcThreads + t];
+  unpack_key(m.block_keys[blk], bx, by, bz);
+  if (t < 8) nb[t] = t == 0 ? blk : find_block_xyz(m, bx + (t & 1), by + ((t >> 1) & 1), bz + ((t >> 2) & 1));
+#pragma unroll
+  for (int r = 0; r < kMcRounds; ++r) {
+    const int l = r * kMcThreads + t, x = l & 7, y = (l >> 3) & 7, z = l >> 6;
+    const int i = (z * 9 + y) * 9 + x;
+    const bool obs = VT::observed(own[r]);
+    c.v[i] = obs ? VT::sdf(own[r]) : 0.0f;
+    c.o[i] = obs ? 1 : 0;
+  }
+  __syncthreads();
+  for (int h = t; h < kMcHalo; h += kMcThreads) {
+    // the x = 8 face (81), then y = 8 with x < 8 (72), then z = 8 with x, y < 8 (64)
+    int x, y, z;
+    if (h < 81) {
+      x = 8; y = h % 9; z = h / 9;
+    } else if (h < 153) {
+      x = (h - 81) & 7; y = 8; z = (h - 81) >> 3;
+    } else {
+      x = (h - 153) & 7; y = (h - 153) >> 3; z = 8;
+    }
+    const int b = nb[(x >> 3) | ((y >> 3) << 1) | ((z >> 3) << 2)];
+    float v = 0.0f;
+    unsigned char o = 0;
+    if (b >= 0) {
+      const typename VT::T w = vox[(size_t)b * 512 + ((z & 7) * 8 + (y & 7)) * 8 + (x & 7)];
+      if (VT::observed(w)) {
+        v = VT::sdf(w);
+        o = 1;
+      }
+    }
+    const int i = (z * 9 + y) * 9 + x;
+    c.v[i] = v;
+    c.o[i] = o;
+  }
+  __syncthreads();
+}
+
+// The 8 corner values of voxel (lx, ly, lz) and a bit mask of the corners
+// that are observed (in an allocated block with weight > 0), from the cube.
+__device__ inline int read_corners(const McCube& c, int lx, int ly, int lz, float* s) {
   int valid = 0;
 #pragma unroll
-  for (int c = 0; c < 8; ++c) {
-    int x = lx + kCorner[c][0], y = ly + kCorner[c][1], z = lz + kCorner[c][2];
-    int k = (x >> 3) | ((y >> 3) << 1) | ((z >> 3) << 2);
-    int blk = nb[k];
-    s[c] = 0.0f;
-    if (blk < 0) continue;
-    typename VT::T v = vox[(size_t)blk * 512 + ((z & 7) * 8 + (y & 7)) * 8 + (x & 7)];
-    if (!VT::observed(v)) continue;
-    s[c] = VT::sdf(v);
-    valid |= 1 << c;
+  for (int k = 0; k < 8; ++k) {
+    const int i = ((lz + kCorner[k][2]) * 9 + (ly + kCorner[k][1])) * 9 + (lx + kCorner[k][0]);
+    s[k] = c.v[i];
+    valid |= (int)c.o[i] << k;
   }
   return valid;
 }
@@ -118,15 +178,6 @@ __device__ inline int owned_edges(int valid, const float* s) {
   return m;
 }
 
-__device__ inline void resolve_neighbours(const MapView& m, int blk, int* nb, int& bx, int& by, int& bz) {
-  unpack_key(m.block_keys[blk], bx, by, bz);
-  if (threadIdx.x < 8) {
-    int k = threadIdx.x;
-    nb[k] = k == 0 ? blk : find_block_xyz(m, bx + (k & 1), by + ((k >> 1) & 1), bz + ((k >> 2) & 1));
-  }
-  __syncthreads();
-}
-
 // Exclusive prefix of v over the threads of a workgroup (thread order), and
 // the workgroup total.  `wsum` holds one int per wavefront.
 __device__ inline int block_excl_scan(int v, int* wsum, int& total) {
@@ -151,28 +202,36 @@ __device__ inline int block_excl_scan(int v, int* wsum, int& total) {
 }
 
 // Count pass: per block, triangles and owned vertices; per voxel, the cube
-// index and (local vertex prefix << 3 | owned edge mask).
+// index and (local vertex prefix << 3 | owned edge mask).  One scan a round
+// of the packed counts (triangles << 16 | vertices: a block has at most 2560
+// and 1536).
 template <class VT>
-__global__ void __launch_bounds__(512) k_mc_count(MapView m, int nblocks, int2* counts, int16_t* cases,
-                                                  uint16_t* vinfo) {
+__global__ void __launch_bounds__(kMcThreads) k_mc_count(MapView m, int nblocks, int2* counts, int16_t* cases,
+                                                         uint16_t* vinfo) {
   __shared__ int nb[8];
-  __shared__ int wsum[8];
+  __shared__ int wsum[kMcThreads / 64];
+  __shared__ McCube c;
   const int blk = blockIdx.x;
   if (blk >= nblocks) return;
   int bx, by, bz;
-  resolve_neighbours(m, blk, nb, bx, by, bz);
-  const int l = threadIdx.x, lx = l & 7, ly = (l >> 3) & 7, lz = l >> 6;
-  float s[8];
-  const int valid = read_corners<VT>((const typename VT::T*)m.voxels, nb, lx, ly, lz, s);
-  const int cube = cube_index(valid, s);
-  const int om = owned_edges(valid, s);
-  cases[(size_t)blk * 512 + l] = (int16_t)cube;
-  const int nt = cube > 0 ? (int)(kTriTable[cube] & 0xF) : 0;
-  int ttot, vtot;
-  block_excl_scan(nt, wsum, ttot);
-  const int vpre = block_excl_scan(__popc(om), wsum, vtot);
-  vinfo[(size_t)blk * 512 + l] = (uint16_t)((vpre << 3) | om);
-  if (l == 0) counts[blk] = make_int2(ttot, vtot);
+  stage_cube<VT>(m, blk, nb, c, bx, by, bz);
+  int tri = 0, vert = 0;  // totals of the rounds so far
+#pragma unroll 1
+  for (int r = 0; r < kMcRounds; ++r) {
+    const int l = r * kMcThreads + threadIdx.x, lx = l & 7, ly = (l >> 3) & 7, lz = l >> 6;
+    float s[8];
+    const int valid = read_corners(c, lx, ly, lz, s);
+    const int cube = cube_index(valid, s);
+    const int om = owned_edges(valid, s);
+    cases[(size_t)blk * 512 + l] = (int16_t)cube;
+    const int nt = cube > 0 ? (int)(kTriTable[cube] & 0xF) : 0;
+    int total;
+    const int pre = block_excl_scan((nt << 16) | __popc(om), wsum, total);
+    vinfo[(size_t)blk * 512 + l] = (uint16_t)(((vert + (pre & 0xFFFF)) << 3) | om);
+    tri += total >> 16;
+    vert += total & 0xFFFF;
+  }
+  if (threadIdx.x == 0) counts[blk] = make_int2(tri, vert);
 }
 
 // Three-phase exclusive scan of int2 counts: 1024-element chunks scanned in
@@ -231,46 +290,51 @@ __device__ inline void edge_vertex(int e, const float* s, float px, float py, fl
 // Emit, reference form: triangle slot = block offset + exclusive prefix (in
 // voxel index order) of the per-voxel counts.
 template <class VT>
-__global__ void __launch_bounds__(512) k_mc_emit(MapView m, int nblocks, const int2* offsets, float* tris,
-                                                 float* normals, int cap) {
+__global__ void __launch_bounds__(kMcThreads) k_mc_emit(MapView m, int nblocks, const int2* offsets, float* tris,
+                                                        float* normals, int cap) {
   __shared__ int nb[8];
-  __shared__ int wsum[8];
+  __shared__ int wsum[kMcThreads / 64];
+  __shared__ McCube c;
   const int blk = blockIdx.x;
   if (blk >= nblocks) return;
   int bx, by, bz;
-  resolve_neighbours(m, blk, nb, bx, by, bz);
-  const int l = threadIdx.x, lx = l & 7, ly = (l >> 3) & 7, lz = l >> 6;
-  float s[8];
-  const int cube = cube_index(read_corners<VT>((const typename VT::T*)m.voxels, nb, lx, ly, lz, s), s);
-  const uint64_t word = cube > 0 ? kTriTable[cube] : 0ull;
-  const int nt = (int)(word & 0xF);
-  int total;
-  int slot = offsets[blk].x + block_excl_scan(nt, wsum, total);
+  stage_cube<VT>(m, blk, nb, c, bx, by, bz);
   const float vs = m.voxel_size;
-  const float px = (float)(bx * 8 + lx), py = (float)(by * 8 + ly), pz = (float)(bz * 8 + lz);
-  for (int i = 0; i < nt; ++i, ++slot) {
-    if (slot >= cap) return;
-    float a[3], b[3], c[3];
-    edge_vertex((int)((word >> (4 + 12 * i)) & 0xF), s, px, py, pz, vs, a);
-    edge_vertex((int)((word >> (8 + 12 * i)) & 0xF), s, px, py, pz, vs, b);
-    edge_vertex((int)((word >> (12 + 12 * i)) & 0xF), s, px, py, pz, vs, c);
-    float u0 = b[0] - a[0], u1 = b[1] - a[1], u2 = b[2] - a[2];
-    float w0 = c[0] - a[0], w1 = c[1] - a[1], w2 = c[2] - a[2];
-    float n0 = u1 * w2 - u2 * w1, n1 = u2 * w0 - u0 * w2, n2 = u0 * w1 - u1 * w0;
-    float q = n0 * n0 + n1 * n1 + n2 * n2;
-    if (q > 0.0f) {
-      float nn = sqrtf(q);
-      n0 = n0 / nn; n1 = n1 / nn; n2 = n2 / nn;
-    }
-    float* T = tris + (size_t)slot * 9;
-    T[0] = a[0]; T[1] = a[1]; T[2] = a[2];
-    T[3] = b[0]; T[4] = b[1]; T[5] = b[2];
-    T[6] = c[0]; T[7] = c[1]; T[8] = c[2];
-    if (normals) {
-      float* N = normals + (size_t)slot * 9;
-      N[0] = N[3] = N[6] = n0;
-      N[1] = N[4] = N[7] = n1;
-      N[2] = N[5] = N[8] = n2;
+  int base = offsets[blk].x;
+#pragma unroll 1
+  for (int r = 0; r < kMcRounds; ++r) {
+    const int l = r * kMcThreads + threadIdx.x, lx = l & 7, ly = (l >> 3) & 7, lz = l >> 6;
+    float s[8];
+    const int cube = cube_index(read_corners(c, lx, ly, lz, s), s);
+    const uint64_t word = cube > 0 ? kTriTable[cube] : 0ull;
+    const int nt = (int)(word & 0xF);
+    int total;
+    int slot = base + block_excl_scan(nt, wsum, total);
+    base += total;
+    const float px = (float)(bx * 8 + lx), py = (float)(by * 8 + ly), pz = (float)(bz * 8 + lz);
+    for (int i = 0; i < nt && slot < cap; ++i, ++slot) {
+      float a[3], b[3], cc[3];
+      edge_vertex((int)((word >> (4 + 12 * i)) & 0xF), s, px, py, pz, vs, a);
+      edge_vertex((int)((word >> (8 + 12 * i)) & 0xF), s, px, py, pz, vs, b);
+      edge_vertex((int)((word >> (12 + 12 * i)) & 0xF), s, px, py, pz, vs, cc);
+      float u0 = b[0] - a[0], u1 = b[1] - a[1], u2 = b[2] - a[2];
+      float w0 = cc[0] - a[0], w1 = cc[1] - a[1], w2 = cc[2] - a[2];
+      float n0 = u1 * w2 - u2 * w1, n1 = u2 * w0 - u0 * w2, n2 = u0 * w1 - u1 * w0;
+      float q = n0 * n0 + n1 * n1 + n2 * n2;
+      if (q > 0.0f) {
+        float nn = sqrtf(q);
+        n0 = n0 / nn; n1 = n1 / nn; n2 = n2 / nn;
+      }
+      float* T = tris + (size_t)slot * 9;
+      T[0] = a[0]; T[1] = a[1]; T[2] = a[2];
+      T[3] = b[0]; T[4] = b[1]; T[5] = b[2];
+      T[6] = cc[0]; T[7] = cc[1]; T[8] = cc[2];
+      if (normals) {
+        float* N = normals + (size_t)slot * 9;
+        N[0] = N[3] = N[6] = n0;
+        N[1] = N[4] = N[7] = n1;
+        N[2] = N[5] = N[8] = n2;
+      }
     }
   }
 }
@@ -278,54 +342,59 @@ __global__ void __launch_bounds__(512) k_mc_emit(MapView m, int nblocks, const i
 // Emit, indexed form: the voxel's owned vertices, then its triangles as
 // indices of the owners' vertices.
 template <class VT>
-__global__ void __launch_bounds__(512) k_mc_emit_indexed(MapView m, int nblocks, const int2* offsets,
-                                                         const uint16_t* vinfo, float* verts, int vcap, int* tris,
-                                                         int tcap) {
+__global__ void __launch_bounds__(kMcThreads) k_mc_emit_indexed(MapView m, int nblocks, const int2* offsets,
+                                                                const uint16_t* vinfo, float* verts, int vcap,
+                                                                int* tris, int tcap) {
   __shared__ int nb[8];
-  __shared__ int wsum[8];
+  __shared__ int wsum[kMcThreads / 64];
   __shared__ int vbase[8];
+  __shared__ McCube c;
   const int blk = blockIdx.x;
   if (blk >= nblocks) return;
   int bx, by, bz;
-  resolve_neighbours(m, blk, nb, bx, by, bz);
+  stage_cube<VT>(m, blk, nb, c, bx, by, bz);
   if (threadIdx.x < 8) vbase[threadIdx.x] = nb[threadIdx.x] >= 0 ? offsets[nb[threadIdx.x]].y : 0;
-  const int l = threadIdx.x, lx = l & 7, ly = (l >> 3) & 7, lz = l >> 6;
-  float s[8];
-  const int valid = read_corners<VT>((const typename VT::T*)m.voxels, nb, lx, ly, lz, s);
-  const int cube = cube_index(valid, s);
-  const uint64_t word = cube > 0 ? kTriTable[cube] : 0ull;
-  const int nt = (int)(word & 0xF);
-  int total;
-  // (the scan's barriers also publish vbase)
-  int slot = offsets[blk].x + block_excl_scan(nt, wsum, total);
   const float vs = m.voxel_size;
-  const float px = (float)(bx * 8 + lx), py = (float)(by * 8 + ly), pz = (float)(bz * 8 + lz);
-  // owned vertices, interpolated from this voxel towards +x, +y, +z
-  const int own = vinfo[(size_t)blk * 512 + l];
-  int vid = vbase[0] + (own >> 3);
+  int base = offsets[blk].x;
   const int far[3] = {1, 3, 4};
+#pragma unroll 1
+  for (int r = 0; r < kMcRounds; ++r) {
+    const int l = r * kMcThreads + threadIdx.x, lx = l & 7, ly = (l >> 3) & 7, lz = l >> 6;
+    float s[8];
+    const int valid = read_corners(c, lx, ly, lz, s);
+    const int cube = cube_index(valid, s);
+    const uint64_t word = cube > 0 ? kTriTable[cube] : 0ull;
+    const int nt = (int)(word & 0xF);
+    int total;
+    // (the scan's barriers also publish vbase)
+    int slot = base + block_excl_scan(nt, wsum, total);
+    base += total;
+    const float px = (float)(bx * 8 + lx), py = (float)(by * 8 + ly), pz = (float)(bz * 8 + lz);
+    // owned vertices, interpolated from this voxel towards +x, +y, +z
+    const int own = vinfo[(size_t)blk * 512 + l];
+    int vid = vbase[0] + (own >> 3);
 #pragma unroll
-  for (int a = 0; a < 3; ++a) {
-    if (!(own & (1 << a))) continue;
-    const float t = edge_t(s[0], s[far[a]]);
-    if (vid < vcap) {
-      float* V = verts + (size_t)vid * 3;
-      V[0] = (px + (a == 0 ? t : 0.0f)) * vs;
-      V[1] = (py + (a == 1 ? t : 0.0f)) * vs;
-      V[2] = (pz + (a == 2 ? t : 0.0f)) * vs;
+    for (int a = 0; a < 3; ++a) {
+      if (!(own & (1 << a))) continue;
+      const float t = edge_t(s[0], s[far[a]]);
+      if (vid < vcap) {
+        float* V = verts + (size_t)vid * 3;
+        V[0] = (px + (a == 0 ? t : 0.0f)) * vs;
+        V[1] = (py + (a == 1 ? t : 0.0f)) * vs;
+        V[2] = (pz + (a == 2 ? t : 0.0f)) * vs;
+      }
+      ++vid;
     }
-    ++vid;
-  }
-  for (int i = 0; i < nt; ++i, ++slot) {
-    if (slot >= tcap) return;
+    for (int i = 0; i < nt && slot < tcap; ++i, ++slot) {
 #pragma unroll
-    for (int k = 0; k < 3; ++k) {
-      const int e = (int)((word >> (4 + 12 * i + 4 * k)) & 0xF);
-      const int x = lx + kOwner[e][0], y = ly + kOwner[e][1], z = lz + kOwner[e][2];
-      const int nbk = (x >> 3) | ((y >> 3) << 1) | ((z >> 3) << 2);
-      const int ob = nb[nbk];  // present: every corner of an emitting cube is observed
-      const int oi = vinfo[(size_t)ob * 512 + ((z & 7) * 8 + (y & 7)) * 8 + (x & 7)];
-      tris[(size_t)slot * 3 + k] = vbase[nbk] + (oi >> 3) + __popc((oi & 7) & ((1 << kOwner[e][3]) - 1));
+      for (int k = 0; k < 3; ++k) {
+        const int e = (int)((word >> (4 + 12 * i + 4 * k)) & 0xF);
+        const int x = lx + kOwner[e][0], y = ly + kOwner[e][1], z = lz + kOwner[e][2];
+        const int nbk = (x >> 3) | ((y >> 3) << 1) | ((z >> 3) << 2);
+        const int ob = nb[nbk];  // present: every corner of an emitting cube is observed
+        const int oi = vinfo[(size_t)ob * 512 + ((z & 7) * 8 + (y & 7)) * 8 + (x & 7)];
+        tris[(size_t)slot * 3 + k] = vbase[nbk] + (oi >> 3) + __popc((oi & 7) & ((1 << kOwner[e][3]) - 1));
+      }
     }
   }
 }
@@ -391,13 +460,13 @@ static int count_and_scan(yas_mesher* me, yas_map* map, MapView& v) {
   hipStream_t s = me->ctx->stream;
   switch (v.format) {
     case YAS_VOXEL_I16_U8:
-      hipLaunchKernelGGL(k_mc_count<VoxI16>, dim3(n), dim3(512), 0, s, v, n, me->counts, me->cases, me->vinfo);
+      hipLaunchKernelGGL(k_mc_count<VoxI16>, dim3(n), dim3(kMcThreads), 0, s, v, n, me->counts, me->cases, me->vinfo);
       break;
     case YAS_VOXEL_F16_U16:
-      hipLaunchKernelGGL(k_mc_count<VoxF16>, dim3(n), dim3(512), 0, s, v, n, me->counts, me->cases, me->vinfo);
+      hipLaunchKernelGGL(k_mc_count<VoxF16>, dim3(n), dim3(kMcThreads), 0, s, v, n, me->counts, me->cases, me->vinfo);
       break;
     default:
-      hipLaunchKernelGGL(k_mc_count<VoxF32>, dim3(n), dim3(512), 0, s, v, n, me->counts, me->cases, me->vinfo);
+      hipLaunchKernelGGL(k_mc_count<VoxF32>, dim3(n), dim3(kMcThreads), 0, s, v, n, me->counts, me->cases, me->vinfo);
   }
   YAS_LAUNCH_CHECK();
   const int nchunks = (n + 1023) / 1024;
@@ -480,15 +549,15 @@ int yas_mesh_extract(yas_mesher* me, yas_map* map, int* num_triangles) {
   hipStream_t s = me->ctx->stream;
   switch (v.format) {
     case YAS_VOXEL_I16_U8:
-      hipLaunchKernelGGL(k_mc_emit<VoxI16>, dim3(n), dim3(512), 0, s, v, n, (const int2*)me->offsets, me->tris,
+      hipLaunchKernelGGL(k_mc_emit<VoxI16>, dim3(n), dim3(kMcThreads), 0, s, v, n, (const int2*)me->offsets, me->tris,
                          me->normals, me->max_tri);
       break;
     case YAS_VOXEL_F16_U16:
-      hipLaunchKernelGGL(k_mc_emit<VoxF16>, dim3(n), dim3(512), 0, s, v, n, (const int2*)me->offsets, me->tris,
+      hipLaunchKernelGGL(k_mc_emit<VoxF16>, dim3(n), dim3(kMcThreads), 0, s, v, n, (const int2*)me->offsets, me->tris,
                          me->normals, me->max_tri);
       break;
     default:
-      hipLaunchKernelGGL(k_mc_emit<VoxF32>, dim3(n), dim3(512), 0, s, v, n, (const int2*)me->offsets, me->tris,
+      hipLaunchKernelGGL(k_mc_emit<VoxF32>, dim3(n), dim3(kMcThreads), 0, s, v, n, (const int2*)me->offsets, me->tris,
                          me->normals, me->max_tri);
   }
   YAS_LAUNCH_CHECK();
@@ -517,15 +586,15 @@ int yas_mesh_extract_indexed(yas_mesher* me, yas_map* map, int* num_vertices, in
   const int vcap = me->max_tri * 3;
   switch (v.format) {
     case YAS_VOXEL_I16_U8:
-      hipLaunchKernelGGL(k_mc_emit_indexed<VoxI16>, dim3(n), dim3(512), 0, s, v, n, (const int2*)me->offsets,
+      hipLaunchKernelGGL(k_mc_emit_indexed<VoxI16>, dim3(n), dim3(kMcThreads), 0, s, v, n, (const int2*)me->offsets,
                          (const uint16_t*)me->vinfo, me->normals, vcap, tri_idx, me->max_tri);
       break;
     case YAS_VOXEL_F16_U16:
-      hipLaunchKernelGGL(k_mc_emit_indexed<VoxF16>, dim3(n), dim3(512), 0, s, v, n, (const int2*)me->offsets,
+      hipLaunchKernelGGL(k_mc_emit_indexed<VoxF16>, dim3(n), dim3(kMcThreads), 0, s, v, n, (const int2*)me->offsets,
                          (const uint16_t*)me->vinfo, me->normals, vcap, tri_idx, me->max_tri);
       break;
     default:
-      hipLaunchKernelGGL(k_mc_emit_indexed<VoxF32>, dim3(n), dim3(512), 0, s, v, n, (const int2*)me->offsets,
+      hipLaunchKernelGGL(k_mc_emit_indexed<VoxF32>, dim3(n), dim3(kMcThreads), 0, s, v, n, (const int2*)me->offsets,
                          (const uint16_t*)me->vinfo, me->normals, vcap, tri_idx, me->max_tri);
   }
   YAS_LAUNCH_CHECK();
