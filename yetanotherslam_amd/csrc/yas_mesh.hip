@@ -97,14 +97,24 @@ struct McCube {
 
 // Pool indices of the block and its neighbours at offset (k&1, k>>1&1,
 // k>>2&1) in nb[k] (-1 if absent), and the staged cube.  The block's own
-// voxels are loaded while threads 0-7 probe the neighbours.
+// voxels are loaded while threads 0-7 probe the neighbours.  With `vinfo`,
+// the count pass's per-voxel vertex info of the same 9x9x9 voxels is staged
+// in `cvi` too (0 in absent blocks).
 template <class VT>
-__device__ inline void stage_cube(const MapView& m, int blk, int* nb, McCube& c, int& bx, int& by, int& bz) {
+__device__ inline void stage_cube(const MapView& m, int blk, int* nb, McCube& c, int& bx, int& by, int& bz,
+                                  const uint16_t* vinfo = nullptr, uint16_t* cvi = nullptr) {
   const typename VT::T* vox = (const typename VT::T*)m.voxels;
   const int t = threadIdx.x;
   typename VT::T own[kMcRounds];
 #pragma unroll
   for (int r = 0; r < kMcRounds; ++r) own[r] = vox[(size_t)blk * 512 + r * kMcThreads + t];
+  if (vinfo) {
+#pragma unroll
+    for (int r = 0; r < kMcRounds; ++r) {
+      const int l = r * kMcThreads + t;
+      cvi[((l >> 6) * 9 + ((l >> 3) & 7)) * 9 + (l & 7)] = vinfo[(size_t)blk * 512 + l];
+    }
+  }
   unpack_key(m.block_keys[blk], bx, by, bz);
   if (t < 8) nb[t] = t == 0 ? blk : find_block_xyz(m, bx + (t & 1), by + ((t >> 1) & 1), bz + ((t >> 2) & 1));
 #pragma unroll
@@ -127,10 +137,13 @@ __device__ inline void stage_cube(const MapView& m, int blk, int* nb, McCube& c,
       x = (h - 153) & 7; y = (h - 153) >> 3; z = 8;
     }
     const int b = nb[(x >> 3) | ((y >> 3) << 1) | ((z >> 3) << 2)];
+    const size_t at = (size_t)b * 512 + ((z & 7) * 8 + (y & 7)) * 8 + (x & 7);
     float v = 0.0f;
     unsigned char o = 0;
+    uint16_t vi = 0;
     if (b >= 0) {
-      const typename VT::T w = vox[(size_t)b * 512 + ((z & 7) * 8 + (y & 7)) * 8 + (x & 7)];
+      const typename VT::T w = vox[at];
+      if (vinfo) vi = vinfo[at];
       if (VT::observed(w)) {
         v = VT::sdf(w);
         o = 1;
@@ -139,6 +152,7 @@ __device__ inline void stage_cube(const MapView& m, int blk, int* nb, McCube& c,
     const int i = (z * 9 + y) * 9 + x;
     c.v[i] = v;
     c.o[i] = o;
+    if (vinfo) cvi[i] = vi;
   }
   __syncthreads();
 }
@@ -350,10 +364,11 @@ __global__ void __launch_bounds__(kMcThreads) k_mc_emit_indexed(MapView m, int n
   __shared__ int wsum[kMcThreads / 64];
   __shared__ int vbase[8];
   __shared__ McCube c;
+  __shared__ uint16_t cvi[kMcCube];
   const int blk = blockIdx.x;
   if (blk >= nblocks) return;
   int bx, by, bz;
-  stage_cube<VT>(m, blk, nb, c, bx, by, bz);
+  stage_cube<VT>(m, blk, nb, c, bx, by, bz, vinfo, cvi);
   if (threadIdx.x < 8) vbase[threadIdx.x] = nb[threadIdx.x] >= 0 ? offsets[nb[threadIdx.x]].y : 0;
   const float vs = m.voxel_size;
   int base = offsets[blk].x;
@@ -372,7 +387,7 @@ __global__ void __launch_bounds__(kMcThreads) k_mc_emit_indexed(MapView m, int n
     base += total;
     const float px = (float)(bx * 8 + lx), py = (float)(by * 8 + ly), pz = (float)(bz * 8 + lz);
     // owned vertices, interpolated from this voxel towards +x, +y, +z
-    const int own = vinfo[(size_t)blk * 512 + l];
+    const int own = cvi[(lz * 9 + ly) * 9 + lx];
     int vid = vbase[0] + (own >> 3);
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
@@ -392,8 +407,8 @@ __global__ void __launch_bounds__(kMcThreads) k_mc_emit_indexed(MapView m, int n
         const int e = (int)((word >> (4 + 12 * i + 4 * k)) & 0xF);
         const int x = lx + kOwner[e][0], y = ly + kOwner[e][1], z = lz + kOwner[e][2];
         const int nbk = (x >> 3) | ((y >> 3) << 1) | ((z >> 3) << 2);
-        const int ob = nb[nbk];  // present: every corner of an emitting cube is observed
-        const int oi = vinfo[(size_t)ob * 512 + ((z & 7) * 8 + (y & 7)) * 8 + (x & 7)];
+        // the owner's block is present: every corner of an emitting cube is observed
+        const int oi = cvi[(z * 9 + y) * 9 + x];
         tris[(size_t)slot * 3 + k] = vbase[nbk] + (oi >> 3) + __popc((oi & 7) & ((1 << kOwner[e][3]) - 1));
       }
     }
