@@ -100,9 +100,12 @@ struct McCube {
 // voxels are loaded while threads 0-7 probe the neighbours.  With `vinfo`,
 // the count pass's per-voxel vertex info of the same 9x9x9 voxels is staged
 // in `cvi` too (0 in absent blocks).
+// The neighbours come from `nbr_in` (the count pass's record, 8 per block)
+// when given, else from the hash, and are recorded in `nbr_out` when given.
 template <class VT>
 __device__ inline void stage_cube(const MapView& m, int blk, int* nb, McCube& c, int& bx, int& by, int& bz,
-                                  const uint16_t* vinfo = nullptr, uint16_t* cvi = nullptr) {
+                                  const int* nbr_in, int* nbr_out, const uint16_t* vinfo = nullptr,
+                                  uint16_t* cvi = nullptr) {
   const typename VT::T* vox = (const typename VT::T*)m.voxels;
   const int t = threadIdx.x;
   typename VT::T own[kMcRounds];
@@ -116,7 +119,12 @@ __device__ inline void stage_cube(const MapView& m, int blk, int* nb, McCube& c,
     }
   }
   unpack_key(m.block_keys[blk], bx, by, bz);
-  if (t < 8) nb[t] = t == 0 ? blk : find_block_xyz(m, bx + (t & 1), by + ((t >> 1) & 1), bz + ((t >> 2) & 1));
+  if (t < 8) {
+    const int b = nbr_in ? nbr_in[(size_t)blk * 8 + t]
+                         : t == 0 ? blk : find_block_xyz(m, bx + (t & 1), by + ((t >> 1) & 1), bz + ((t >> 2) & 1));
+    nb[t] = b;
+    if (nbr_out) nbr_out[(size_t)blk * 8 + t] = b;
+  }
 #pragma unroll
   for (int r = 0; r < kMcRounds; ++r) {
     const int l = r * kMcThreads + t, x = l & 7, y = (l >> 3) & 7, z = l >> 6;
@@ -222,14 +230,14 @@ __device__ inline int block_excl_scan(int v, int* wsum, int& total) {
 // and 1536).
 template <class VT>
 __global__ void __launch_bounds__(kMcThreads) k_mc_count(MapView m, int nblocks, int2* counts, int16_t* cases,
-                                                         uint16_t* vinfo) {
+                                                         uint16_t* vinfo, int* nbr) {
   __shared__ int nb[8];
   __shared__ int wsum[kMcThreads / 64];
   __shared__ McCube c;
   const int blk = blockIdx.x;
   if (blk >= nblocks) return;
   int bx, by, bz;
-  stage_cube<VT>(m, blk, nb, c, bx, by, bz);
+  stage_cube<VT>(m, blk, nb, c, bx, by, bz, nullptr, nbr);
   int tri = 0, vert = 0;  // totals of the rounds so far
 #pragma unroll 1
   for (int r = 0; r < kMcRounds; ++r) {
@@ -306,14 +314,14 @@ __device__ inline void edge_vertex(int e, const float* s, float px, float py, fl
 // voxel index order) of the per-voxel counts.
 template <class VT>
 __global__ void __launch_bounds__(kMcThreads) k_mc_emit(MapView m, int nblocks, const int2* offsets, float* tris,
-                                                        float* normals, int cap) {
+                                                        float* normals, int cap, const int* nbr) {
   __shared__ int nb[8];
   __shared__ int wsum[kMcThreads / 64];
   __shared__ McCube c;
   const int blk = blockIdx.x;
   if (blk >= nblocks) return;
   int bx, by, bz;
-  stage_cube<VT>(m, blk, nb, c, bx, by, bz);
+  stage_cube<VT>(m, blk, nb, c, bx, by, bz, nbr, nullptr);
   const float vs = m.voxel_size;
   int base = offsets[blk].x;
 #pragma unroll 1
@@ -359,7 +367,7 @@ __global__ void __launch_bounds__(kMcThreads) k_mc_emit(MapView m, int nblocks, 
 template <class VT>
 __global__ void __launch_bounds__(kMcThreads) k_mc_emit_indexed(MapView m, int nblocks, const int2* offsets,
                                                                 const uint16_t* vinfo, float* verts, int vcap,
-                                                                int* tris, int tcap) {
+                                                                int* tris, int tcap, const int* nbr) {
   __shared__ int nb[8];
   __shared__ int wsum[kMcThreads / 64];
   __shared__ int vbase[8];
@@ -368,7 +376,7 @@ __global__ void __launch_bounds__(kMcThreads) k_mc_emit_indexed(MapView m, int n
   const int blk = blockIdx.x;
   if (blk >= nblocks) return;
   int bx, by, bz;
-  stage_cube<VT>(m, blk, nb, c, bx, by, bz, vinfo, cvi);
+  stage_cube<VT>(m, blk, nb, c, bx, by, bz, nbr, nullptr, vinfo, cvi);
   if (threadIdx.x < 8) vbase[threadIdx.x] = nb[threadIdx.x] >= 0 ? offsets[nb[threadIdx.x]].y : 0;
   const float vs = m.voxel_size;
   int base = offsets[blk].x;
@@ -428,6 +436,7 @@ struct yas_mesher {
   int2* total;      // {triangles, vertices}
   int16_t* cases;   // per voxel cube index
   uint16_t* vinfo;  // per voxel (local vertex prefix << 3 | owned edges)
+  int* nbr;         // per block, pool indices of it and its 7 +x/+y/+z neighbours (-1 absent)
   int block_cap;
   int num_blocks;
   int num_tri;
@@ -443,6 +452,8 @@ static int ensure_block_cap(yas_mesher* me, int n) {
   hipFree(me->chunk);
   hipFree(me->cases);
   hipFree(me->vinfo);
+  hipFree(me->nbr);
+  me->nbr = nullptr;
   me->counts = me->offsets = me->chunk = nullptr;
   me->cases = nullptr;
   me->vinfo = nullptr;
@@ -452,6 +463,7 @@ static int ensure_block_cap(yas_mesher* me, int n) {
   YAS_HIP(hipMalloc((void**)&me->chunk, sizeof(int2) * ((n + 1023) / 1024)));
   YAS_HIP(hipMalloc((void**)&me->cases, sizeof(int16_t) * 512 * (size_t)n));
   YAS_HIP(hipMalloc((void**)&me->vinfo, sizeof(uint16_t) * 512 * (size_t)n));
+  YAS_HIP(hipMalloc((void**)&me->nbr, sizeof(int) * 8 * (size_t)n));
   me->block_cap = n;
   return YAS_OK;
 }
@@ -476,13 +488,16 @@ static int count_and_scan(yas_mesher* me, yas_map* map, MapView& v) {
   hipStream_t s = me->ctx->stream;
   switch (v.format) {
     case YAS_VOXEL_I16_U8:
-      hipLaunchKernelGGL(k_mc_count<VoxI16>, dim3(n), dim3(kMcThreads), 0, s, v, n, me->counts, me->cases, me->vinfo);
+      hipLaunchKernelGGL(k_mc_count<VoxI16>, dim3(n), dim3(kMcThreads), 0, s, v, n, me->counts, me->cases, me->vinfo,
+                         me->nbr);
       break;
     case YAS_VOXEL_F16_U16:
-      hipLaunchKernelGGL(k_mc_count<VoxF16>, dim3(n), dim3(kMcThreads), 0, s, v, n, me->counts, me->cases, me->vinfo);
+      hipLaunchKernelGGL(k_mc_count<VoxF16>, dim3(n), dim3(kMcThreads), 0, s, v, n, me->counts, me->cases, me->vinfo,
+                         me->nbr);
       break;
     default:
-      hipLaunchKernelGGL(k_mc_count<VoxF32>, dim3(n), dim3(kMcThreads), 0, s, v, n, me->counts, me->cases, me->vinfo);
+      hipLaunchKernelGGL(k_mc_count<VoxF32>, dim3(n), dim3(kMcThreads), 0, s, v, n, me->counts, me->cases, me->vinfo,
+                         me->nbr);
   }
   YAS_LAUNCH_CHECK();
   const int nchunks = (n + 1023) / 1024;
@@ -549,6 +564,7 @@ int yas_mesher_destroy(yas_mesher* me) {
   hipFree(me->total);
   hipFree(me->cases);
   hipFree(me->vinfo);
+  hipFree(me->nbr);
   yas_ctx_release(me->ctx);
   delete me;
   return YAS_OK;
@@ -566,15 +582,15 @@ int yas_mesh_extract(yas_mesher* me, yas_map* map, int* num_triangles) {
   switch (v.format) {
     case YAS_VOXEL_I16_U8:
       hipLaunchKernelGGL(k_mc_emit<VoxI16>, dim3(n), dim3(kMcThreads), 0, s, v, n, (const int2*)me->offsets, me->tris,
-                         me->normals, me->max_tri);
+                         me->normals, me->max_tri, (const int*)me->nbr);
       break;
     case YAS_VOXEL_F16_U16:
       hipLaunchKernelGGL(k_mc_emit<VoxF16>, dim3(n), dim3(kMcThreads), 0, s, v, n, (const int2*)me->offsets, me->tris,
-                         me->normals, me->max_tri);
+                         me->normals, me->max_tri, (const int*)me->nbr);
       break;
     default:
       hipLaunchKernelGGL(k_mc_emit<VoxF32>, dim3(n), dim3(kMcThreads), 0, s, v, n, (const int2*)me->offsets, me->tris,
-                         me->normals, me->max_tri);
+                         me->normals, me->max_tri, (const int*)me->nbr);
   }
   YAS_LAUNCH_CHECK();
   if ((rc = read_totals(me))) return rc;
@@ -603,15 +619,18 @@ int yas_mesh_extract_indexed(yas_mesher* me, yas_map* map, int* num_vertices, in
   switch (v.format) {
     case YAS_VOXEL_I16_U8:
       hipLaunchKernelGGL(k_mc_emit_indexed<VoxI16>, dim3(n), dim3(kMcThreads), 0, s, v, n, (const int2*)me->offsets,
-                         (const uint16_t*)me->vinfo, me->normals, vcap, tri_idx, me->max_tri);
+                         (const uint16_t*)me->vinfo, me->normals, vcap, tri_idx, me->max_tri,
+                         (const int*)me->nbr);
       break;
     case YAS_VOXEL_F16_U16:
       hipLaunchKernelGGL(k_mc_emit_indexed<VoxF16>, dim3(n), dim3(kMcThreads), 0, s, v, n, (const int2*)me->offsets,
-                         (const uint16_t*)me->vinfo, me->normals, vcap, tri_idx, me->max_tri);
+                         (const uint16_t*)me->vinfo, me->normals, vcap, tri_idx, me->max_tri,
+                         (const int*)me->nbr);
       break;
     default:
       hipLaunchKernelGGL(k_mc_emit_indexed<VoxF32>, dim3(n), dim3(kMcThreads), 0, s, v, n, (const int2*)me->offsets,
-                         (const uint16_t*)me->vinfo, me->normals, vcap, tri_idx, me->max_tri);
+                         (const uint16_t*)me->vinfo, me->normals, vcap, tri_idx, me->max_tri,
+                         (const int*)me->nbr);
   }
   YAS_LAUNCH_CHECK();
   if ((rc = read_totals(me))) return rc;
