@@ -10,11 +10,20 @@ namespace yas {
 // Depth images: float metres, or uint16 raw units converted as
 // (float)raw * scale (FullSystem/src/System.cpp:52, the same conversion as
 // the standalone u16 -> f32 kernel, so both give identical floats).
+// DepthF32 addresses a pixel by a 32-bit byte offset from the image's
+// (wavefront-uniform) base, which the loads take as a scalar base plus one
+// vector register; the host launchers refuse images whose last pixel lies
+// 4 GiB or more past the base (depth_span_fits).
 struct DepthF32 {
   const float* p;
   size_t pitch;
-  __device__ inline float at(int x, int y) const { return ((const float*)((const char*)p + (size_t)y * pitch))[x]; }
+  __device__ inline float at(int x, int y) const {
+    return *(const float*)((const char*)p + ((unsigned)y * (unsigned)pitch + (unsigned)x * 4u));
+  }
 };
+inline bool depth_span_fits(size_t pitch, int W, int H) {
+  return pitch < (1ull << 32) && (unsigned long long)(H - 1) * pitch + (unsigned long long)W * 4u <= 0xFFFFFFFFull;
+}
 struct DepthU16 {
   const uint16_t* p;
   size_t pitch;
@@ -169,7 +178,7 @@ __device__ inline int integrate_wave(const MapView& m, int b, int groups) {
     const int j = b & 7, k = b >> 3;
     g = k < per ? j * per + k : groups;
   }
-  return g * wpg + (threadIdx.x >> 6);
+  return __builtin_amdgcn_readfirstlane(g * wpg + (int)(threadIdx.x >> 6));
 }
 
 // K5 (VoxelMap.cu:719-786): one wavefront per visible block, lane = (x, y),

@@ -187,8 +187,10 @@ __global__ void __launch_bounds__(256) k_visible(MapView m, const PosePair* pose
 // four counters (allocated, visible, flags, walk steps), final once
 // allocation and visibility have run.
 template <class VT>
-// At least six wavefronts per SIMD (the float storage's kernel takes 72
-// registers, seven wavefronts, no spills; unbounded it took 84 and five).
+// At least six wavefronts per SIMD.  With the block index made
+// wavefront-uniform and 32-bit depth offsets every storage's kernel fits
+// in 64 registers (eight wavefronts, no spills); asking for eight with
+// 64-bit depth addresses spilled the float storage's kernel and slowed it.
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6)))
 k_integrate(MapView m, const PosePair* pose, Intr K, const float* depth, size_t pitch, int W, int H, int* stats) {
   if (stats && blockIdx.x == 0 && threadIdx.x < 4) stats[threadIdx.x] = m.ctr[threadIdx.x];
@@ -232,6 +234,7 @@ static int launch_integrate(yas_map* map, const float* depth, size_t pitch, int 
 // device (integration then sees no visible blocks).
 int yas_map_alloc_stage(yas_map* map, const float* depth, size_t pitch, int W, int H, const int* gate,
                         const hipEvent_t* ev = nullptr) {
+  YAS_REQUIRE(depth_span_fits(pitch, W, H), "depth image spans 4 GiB or more");
   dim3 g((W + 15) / 16, (H + 15) / 16);
   launch_timed(k_alloc, g, dim3(256), map->ctx->stream, ev, map->v, depth, pitch, W, H, map->K,
                (const PosePair*)map->pose_dev, gate);
@@ -246,6 +249,7 @@ int yas_map_visible_stage(yas_map* map, int W, int H, const int* gate, const hip
 }
 int yas_map_integrate_stage(yas_map* map, const float* depth, size_t pitch, int W, int H,
                             const hipEvent_t* ev = nullptr, int* stats = nullptr) {
+  YAS_REQUIRE(depth_span_fits(pitch, W, H), "depth image spans 4 GiB or more");
   return launch_integrate(map, depth, pitch, W, H, ev, stats);
 }
 int yas_map_fuse_prepared(yas_map* map, const float* depth, size_t pitch, int W, int H, const int* gate) {
