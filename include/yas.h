@@ -450,6 +450,12 @@ int yas_tracker_is_tracking_good(yas_tracker* trk, int* good);
  * photometric {sum r, count, mean, Huber threshold}. Synchronous. */
 int yas_tracker_step_sums(yas_tracker* trk, int level, const double T[16], int kind, float out[29],
                           float stats[4]);
+/* The Gauss-Newton step's solve of H x = b (the LDLT of RGBDTracking.cu:185,
+ * Eigen's diagonal pivoting) for n systems (1 <= n <= 2^20), run
+ * by the same device code as the tracker's steps (test access): H is n
+ * row-major 6x6 float matrices, b n float 6-vectors, x receives n double
+ * 6-vectors (host arrays). Synchronous. */
+int yas_tracker_solve6(yas_ctx* ctx, const float* H, const float* b, double* x, int n);
 /* Device pointer and pitch of one pyramid image (test access):
  * which = 0 current intensity, 1 reference intensity, 2 gx, 3 gy,
  * 4 current inverse depth, 5 reference inverse depth (float);

@@ -757,6 +757,51 @@ int yas_tracker_step_sums(yas_tracker* t, int level, const double T[16], int kin
   return YAS_OK;
 }
 
+namespace {
+// One wavefront per system: the Gauss-Newton step's 6x6 solve as the
+// tracker's kernels run it (ldlt_solve6_wave), for yas_tracker_solve6.
+__global__ void __launch_bounds__(64) k_solve6(const float* H, const float* b, double* x) {
+  __shared__ float sH[36], sb[6];
+  __shared__ double sx[6];
+  const size_t s = blockIdx.x;
+  const int lane = threadIdx.x;
+  if (lane < 36) sH[lane] = H[s * 36 + lane];
+  if (lane < 6) sb[lane] = b[s * 6 + lane];
+  wave_lds_sync();
+  double xi[6];
+  ldlt_solve6_wave(sH, sb, sx, xi);
+  if (lane == 0) {
+#pragma unroll
+    for (int i = 0; i < 6; ++i) x[s * 6 + i] = xi[i];
+  }
+}
+}  // namespace
+
+int yas_tracker_solve6(yas_ctx* ctx, const float* H, const float* b, double* x, int n) {
+  YAS_REQUIRE(ctx && H && b && x, "yas_tracker_solve6: null argument");
+  YAS_REQUIRE(n >= 1 && n <= (1 << 20), "yas_tracker_solve6: n must be 1..2^20");
+  YAS_HIP(hipSetDevice(ctx->device));
+  const size_t nh = (size_t)n * 36 * sizeof(float), nb = (size_t)n * 6 * sizeof(float), nx = (size_t)n * 6 * sizeof(double);
+  char* buf = nullptr;
+  YAS_HIP(hipMalloc((void**)&buf, nh + nb + nx));
+  float* dH = (float*)buf;
+  float* db = (float*)(buf + nh);
+  double* dx = (double*)(buf + nh + nb);
+  hipStream_t s = ctx->stream;
+  hipError_t e = hipMemcpyAsync(dH, H, nh, hipMemcpyHostToDevice, s);
+  if (e == hipSuccess) e = hipMemcpyAsync(db, b, nb, hipMemcpyHostToDevice, s);
+  if (e == hipSuccess) {
+    hipLaunchKernelGGL(k_solve6, dim3(n), dim3(64), 0, s, dH, db, dx);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipMemcpyAsync(x, dx, nx, hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  else hipStreamSynchronize(s);
+  hipFree(buf);
+  YAS_HIP(e);
+  return YAS_OK;
+}
+
 int yas_tracker_level(yas_tracker* t, int which, int level, void** ptr, size_t* pitch, int* w, int* h) {
   YAS_REQUIRE(t && ptr, "null argument");
   YAS_REQUIRE(level >= 0 && level < t->P.levels, "bad level");

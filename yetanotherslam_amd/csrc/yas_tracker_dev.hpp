@@ -812,6 +812,117 @@ __host__ __device__ inline void ldlt_solve6_presorted(const float* H, const floa
   for (int i = 0; i < n; ++i) x[i] = scratch[i];
 }
 
+// Lanes of one wavefront exchanging values through LDS or registers: the
+// fences keep the compiler from moving LDS accesses across this point (a
+// wavefront's own LDS accesses complete in order).
+__device__ inline void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+// Lane `src`'s value (src a constant or wavefront-uniform).
+__device__ inline double lane_value(double v, int src) {
+  const unsigned long long u = __builtin_bit_cast(unsigned long long, v);
+  const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)u, src);
+  const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(u >> 32), src);
+  return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
+}
+
+// ldlt_solve6_presorted run by a whole wavefront (all 64 lanes active,
+// every lane gets x): lane r < 6 holds row r of the permuted matrix, and at
+// factorisation step k the rows r >= k update their column k at once, each
+// by the same operations in the same order as the serial loops (the
+// products temp[j] = D_j L_kj from lane j's diagonal and lane k's row, the
+// sums from j = 0 up, the division by D_k), so x has the same bits.  The
+// pivots and the back substitution stay serial (the latter's sums run over
+// j ascending, which needs the finished y_j in that order); lanes 6..63
+// mirror row 5.  H and b are read from LDS; `scratch` (6 doubles, LDS)
+// undoes the permutation.
+__device__ inline void ldlt_solve6_wave(const float* H, const float* b, double* scratch, double (&x)[6]) {
+  constexpr int n = 6;
+  const int lane = threadIdx.x & 63;
+  const int r = lane < n ? lane : n - 1;
+  double d[n];
+  int p[n];
+#pragma unroll
+  for (int i = 0; i < n; ++i) {
+    d[i] = (double)H[i * n + i];
+    p[i] = i;
+  }
+  bool stop = false;
+#pragma unroll
+  for (int k = 0; k < n; ++k) {
+    int big = k;
+    double bv = fabs(d[k]);
+#pragma unroll
+    for (int i = k + 1; i < n; ++i)
+      if (fabs(d[i]) > bv) {
+        bv = fabs(d[i]);
+        big = i;
+      }
+#pragma unroll
+    for (int i = k + 1; i < n; ++i) {
+      const bool sw = i == big;
+      swap_if(sw, d[k], d[i]);
+      const int pk = p[k], pi = p[i];
+      p[k] = sw ? pi : pk;
+      p[i] = sw ? pk : pi;
+    }
+    if (k == 0 && !(fabs(d[0]) > 0.0)) stop = true;
+  }
+  if (stop) {
+#pragma unroll
+    for (int i = 0; i < n; ++i) p[i] = i;
+  }
+  int pr = p[0];
+#pragma unroll
+  for (int j = 1; j < n; ++j) pr = r == j ? p[j] : pr;
+  double a[n];  // row r: a[j] = A(r, j) for j <= r
+#pragma unroll
+  for (int j = 0; j < n; ++j) a[j] = (double)H[pr * n + p[j]];
+  double y = (double)b[pr];
+  if (!stop) {
+#pragma unroll
+    for (int k = 0; k < n; ++k) {
+      if (k > 0) {
+        double temp[n];
+#pragma unroll
+        for (int j = 0; j < k; ++j) temp[j] = lane_value(a[j], j) * lane_value(a[j], k);
+        double s = 0.0;
+#pragma unroll
+        for (int j = 0; j < k; ++j) s += a[j] * temp[j];
+        if (r >= k) a[k] -= s;
+      }
+      const double akk = lane_value(a[k], k);
+      if (fabs(akk) > 0.0 && r > k) a[k] /= akk;
+    }
+  }
+  // forward substitution: y_r -= A(r, j) y_j for j = 0 .. r-1, in that order
+#pragma unroll
+  for (int j = 0; j < n - 1; ++j) {
+    const double yj = lane_value(y, j);
+    if (r > j) y -= a[j] * yj;
+  }
+  double dd = a[0];
+#pragma unroll
+  for (int j = 1; j < n; ++j) dd = r == j ? a[j] : dd;
+  y = fabs(dd) > DBL_MIN ? y / dd : 0.0;
+  double yu[n];
+#pragma unroll
+  for (int i = 0; i < n; ++i) yu[i] = lane_value(y, i);
+#pragma unroll
+  for (int i = n - 1; i >= 0; --i)
+#pragma unroll
+    for (int j = i + 1; j < n; ++j) yu[i] -= lane_value(a[i], j) * yu[j];
+  if (lane == 0) {
+#pragma unroll
+    for (int i = 0; i < n; ++i) scratch[p[i]] = yu[i];
+  }
+  wave_lds_sync();
+#pragma unroll
+  for (int i = 0; i < n; ++i) x[i] = scratch[i];
+}
+
 // SE(3) exponential, twist (rho, omega), Rodrigues' formula.
 __device__ inline void se3_exp(const double* xi, double* T) {
   const double* w = xi + 3;
@@ -855,87 +966,90 @@ __device__ inline void unpack29(const float* s, float* H, float* b) {
     }
 }
 
-// Thread 0's part of gn_update: inlier counts and, when do_step and the
-// state is good, the step from the pose s_est.  Writes the state's fields to
-// st and the stepped pose to est_out (each when not null).
-__device__ __attribute__((always_inline)) inline void gn_step_lane0(const float (*sums)[32], const double* s_est,
+// Wavefront 0's part of gn_update (all 64 lanes, which leave together):
+// inlier counts and, when do_step and the state is good, the step from the
+// pose s_est.  Writes the state's fields to st and the stepped pose to
+// est_out (each when not null).  Each value is formed by the operations of
+// the serial form, in its order: H and b one element per lane
+// (RankUpdateHessian's packing, the RGB term weighted first), the solve by
+// ldlt_solve6_wave, the exponential in every lane alike, and E * est one
+// element per lane for lanes 0..15.  lds_H (36), lds_b (6), lds_x (6) and
+// lds_E (16) are LDS.
+__device__ __attribute__((always_inline)) inline void gn_step_wave0(const float (*sums)[32], const double* s_est,
                                                                     int s_good, float s_last_err, TrackState* st,
                                                                     double* est_out, int gi, int gr, int mode,
                                                                     int first_of_level, int do_step, int record,
                                                                     float wrh, float wrb, float wih, float wib,
-                                                                    float* lds_H, float* lds_b, double* lds_x) {
-  if (st && record && gi) st->icp_inliers = (int)sums[0][28];
-  if (st && record && gr) st->rgb_inliers = (int)sums[1][28];
+                                                                    float* lds_H, float* lds_b, double* lds_x,
+                                                                    double* lds_E) {
+  const int lane = threadIdx.x & 63;
+  if (st && lane == 0 && record && gi) st->icp_inliers = (int)sums[0][28];
+  if (st && lane == 0 && record && gr) st->rgb_inliers = (int)sums[1][28];
   if (!do_step) return;
   float last_err = s_last_err;
   if (first_of_level) {
     last_err = FLT_MAX;
-    if (st) st->last_err = FLT_MAX;
+    if (st && lane == 0) st->last_err = FLT_MAX;
   }
   if (!s_good) return;
-  float H[36], b[6];
-#pragma unroll
-  for (int i = 0; i < 36; ++i) H[i] = 0.0f;
-#pragma unroll
-  for (int i = 0; i < 6; ++i) b[i] = 0.0f;
+  if (lane < 42) {
+    // element (i, j), i <= j, of the packed upper rows [H | b] (7 - i entries in row i)
+    const bool isb = lane >= 36;
+    const int e0 = lane / 6, e1 = lane - 6 * e0;
+    const int i = isb ? lane - 36 : (e0 < e1 ? e0 : e1);
+    const int j = isb ? 6 : (e0 < e1 ? e1 : e0);
+    const int k = 7 * i - (i * (i - 1)) / 2 + (j - i);
+    float v = 0.0f;
+    if (mode != YAS_TRACK_DEPTH) {
+      const float w = mode == YAS_TRACK_RGBD ? (isb ? wrb : wrh) : 1.0f;
+      v = v + w * sums[1][k];
+    }
+    if (mode != YAS_TRACK_RGB) {
+      const float w = mode == YAS_TRACK_RGBD ? (isb ? wib : wih) : 1.0f;
+      v = v + w * sums[0][k];
+    }
+    if (isb) lds_b[i] = v;
+    else lds_H[lane] = v;
+  }
   float rsum = 0.0f, nres = 0.0f;
   if (mode != YAS_TRACK_DEPTH) {
-    float Hr[36], br[6];
-    unpack29(sums[1], Hr, br);
-    const float wh = mode == YAS_TRACK_RGBD ? wrh : 1.0f, wb = mode == YAS_TRACK_RGBD ? wrb : 1.0f;
-#pragma unroll
-    for (int i = 0; i < 36; ++i) H[i] = H[i] + wh * Hr[i];
-#pragma unroll
-    for (int i = 0; i < 6; ++i) b[i] = b[i] + wb * br[i];
     rsum = sums[1][27];
     nres = sums[1][28];
   }
-  if (mode != YAS_TRACK_RGB) {
-    float Hi[36], bi[6];
-    unpack29(sums[0], Hi, bi);
-    const float wh = mode == YAS_TRACK_RGBD ? wih : 1.0f, wb = mode == YAS_TRACK_RGBD ? wib : 1.0f;
-#pragma unroll
-    for (int i = 0; i < 36; ++i) H[i] = H[i] + wh * Hi[i];
-#pragma unroll
-    for (int i = 0; i < 6; ++i) b[i] = b[i] + wb * bi[i];
-    rsum = sums[0][27];
-  }
+  if (mode != YAS_TRACK_RGB) rsum = sums[0][27];
   const float err = sqrtf(rsum) / (nres + 6.0f);
-  // the solve reads H and b through its pivot permutation, so from LDS
-#pragma unroll
-  for (int i = 0; i < 36; ++i) lds_H[i] = H[i];
-#pragma unroll
-  for (int i = 0; i < 6; ++i) lds_b[i] = b[i];
+  wave_lds_sync();
   double xi[6];
-  ldlt_solve6_presorted(lds_H, lds_b, lds_x, xi);
+  ldlt_solve6_wave(lds_H, lds_b, lds_x, xi);
   if (isnan(xi[0])) {
-    if (st) st->good = 0;
+    if (st && lane == 0) st->good = 0;
     return;
   }
-  double E[16], est[16];
+  double E[16];
   se3_exp(xi, E);
+  if (lane < 16) {
 #pragma unroll
-  for (int i = 0; i < 16; ++i) est[i] = s_est[i];
-  mul44(E, est, est);
-  if (est_out) {
-#pragma unroll
-    for (int i = 0; i < 16; ++i) est_out[i] = est[i];
+    for (int q = 0; q < 16; ++q) lds_E[q] = E[q];
   }
-  if (!st) return;
-#pragma unroll
-  for (int i = 0; i < 16; ++i) st->est[i] = est[i];
-  if (err < last_err) {
-#pragma unroll
-    for (int i = 0; i < 16; ++i) st->last_ok[i] = est[i];
-    st->last_err = err;
+  wave_lds_sync();
+  if (lane < 16) {
+    const int i = lane >> 2, j = lane & 3;
+    const double t = lds_E[4 * i] * s_est[j] + lds_E[4 * i + 1] * s_est[4 + j] + lds_E[4 * i + 2] * s_est[8 + j] +
+                     lds_E[4 * i + 3] * s_est[12 + j];
+    if (est_out) est_out[lane] = t;
+    if (st) {
+      st->est[lane] = t;
+      if (err < last_err) st->last_ok[lane] = t;
+    }
   }
+  if (st && lane == 0 && err < last_err) st->last_err = err;
 }
 
 // Fold the partial rows of one linearisation and, when `do_step`, take one
 // Gauss-Newton step of RGBDTracking.cu:161-204.  One workgroup.  The state
 // is read from st_in; when `write`, st_out receives it with the fold's and
 // the step's changes (st_out may be st_in).  When est_out (LDS) is given,
-// thread 0 leaves the pose after the step there.
+// wavefront 0 leaves the pose after the step there.
 __device__ __attribute__((always_inline)) inline void gn_update(const TrackState* st_in, TrackState* st_out,
                                                                 bool write, const float* part_icp, int gi,
                                                                 const float* part_rgb, int gr, int mode,
@@ -946,8 +1060,8 @@ __device__ __attribute__((always_inline)) inline void gn_update(const TrackState
   __shared__ double s_est[16];
   __shared__ int s_good;
   __shared__ float s_last_err;
-  __shared__ float s_H[36], s_b[6];  // thread 0's solve
-  __shared__ double s_x[6];
+  __shared__ float s_H[36], s_b[6];  // wavefront 0's step
+  __shared__ double s_x[6], s_E[16];
   // the state the step reads, loaded alongside the partial rows
   if (threadIdx.x < 16) s_est[threadIdx.x] = st_in->est[threadIdx.x];
   if (threadIdx.x == 16) s_good = st_in->good;
@@ -987,13 +1101,10 @@ __device__ __attribute__((always_inline)) inline void gn_update(const TrackState
     if (gi) st_out->sums_icp[threadIdx.x] = sums[0][threadIdx.x];
     if (gr) st_out->sums_rgb[threadIdx.x] = sums[1][threadIdx.x];
   }
-  if (threadIdx.x != 0) return;
-  if (est_out) {  // the pose where no step is taken
-#pragma unroll
-    for (int i = 0; i < 16; ++i) est_out[i] = s_est[i];
-  }
-  gn_step_lane0(sums, s_est, s_good, s_last_err, write ? st_out : nullptr, est_out, gi, gr, mode, first_of_level,
-                do_step, record, wrh, wrb, wih, wib, s_H, s_b, s_x);
+  if (threadIdx.x >= 64) return;
+  if (est_out && threadIdx.x < 16) est_out[threadIdx.x] = s_est[threadIdx.x];  // the pose where no step is taken
+  gn_step_wave0(sums, s_est, s_good, s_last_err, write ? st_out : nullptr, est_out, gi, gr, mode, first_of_level,
+                do_step, record, wrh, wrb, wih, wib, s_H, s_b, s_x, s_E);
 }
 __device__ inline void update_at(TrackState* st, const float* part_icp, int gi, const float* part_rgb, int gr,
                                  int mode, int first_of_level, int do_step, int record, float wrh, float wrb, float wih,

@@ -143,6 +143,7 @@ SIGNATURES = {
     "yas_tracker_swap_frame_buffer": (I, [P]),
     "yas_tracker_is_tracking_good": (I, [P, PI]),
     "yas_tracker_step_sums": (I, [P, I, P, I, P, P]),
+    "yas_tracker_solve6": (I, [P, P, P, P, I]),
     "yas_tracker_level": (I, [P, I, I, PP, ctypes.POINTER(SZ), PI, PI]),
     "yas_depth_u16_to_f32": (I, [P, P, SZ, P, SZ, I, I, F]),
     "yas_rgb_to_intensity": (I, [P, P, SZ, P, SZ, I, I, I]),
