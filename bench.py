@@ -87,8 +87,8 @@ def cpu_baseline(p, K, depth, rgb, budget_s=12.0, all_cores=False):
 # carried here, keyed by the run they were measured on: (warmup, steps),
 # i.e. the timed frames warmup+1 .. warmup+steps of the sequence.  Other runs
 # report no traffic figure.
-TRAFFIC_BYTES = {(10, 290): {"k_march": 54.46e6, "k_integrate": 108.62e6, "k_alloc": 4.45e6},
-                 (5, 20): {"k_march": 38.44e6, "k_integrate": 87.16e6, "k_alloc": 4.42e6}}
+TRAFFIC_BYTES = {(10, 290): {"k_march": 54.42e6, "k_integrate": 108.93e6, "k_alloc": 4.45e6},
+                 (5, 20): {"k_march": 38.42e6, "k_integrate": 87.36e6, "k_alloc": 4.42e6}}
 
 # VALU instructions (wavefront instructions summed over the GPU) per launch
 # of the ray march, from a rocprofv3 SQ_INSTS_VALU pass over this script's
