@@ -90,10 +90,16 @@ constexpr int kMcRounds = 512 / kMcThreads;
 constexpr int kMcCube = 9 * 9 * 9;
 constexpr int kMcHalo = kMcCube - 512;
 
+// One word per voxel: the sdf, or kMcAbsent's bits where the voxel is not
+// observed (absent block or weight 0).  An observed NaN sdf is staged as the
+// canonical quiet NaN, so no observed value has those bits.
+constexpr unsigned kMcAbsent = 0xFFFFFFFFu;
 struct McCube {
-  float v[kMcCube];          // sdf, 0 where not observed (absent block or weight 0)
-  unsigned char o[kMcCube];  // observed
+  float v[kMcCube];
 };
+__device__ inline float mc_staged(bool observed, float sdf) {
+  return !observed ? __uint_as_float(kMcAbsent) : sdf != sdf ? __uint_as_float(0x7FC00000u) : sdf;
+}
 
 // Pool indices of the block and its neighbours at offset (k&1, k>>1&1,
 // k>>2&1) in nb[k] (-1 if absent), and the staged cube.  The block's own
@@ -129,9 +135,7 @@ __device__ inline void stage_cube(const MapView& m, int blk, int* nb, McCube& c,
   for (int r = 0; r < kMcRounds; ++r) {
     const int l = r * kMcThreads + t, x = l & 7, y = (l >> 3) & 7, z = l >> 6;
     const int i = (z * 9 + y) * 9 + x;
-    const bool obs = VT::observed(own[r]);
-    c.v[i] = obs ? VT::sdf(own[r]) : 0.0f;
-    c.o[i] = obs ? 1 : 0;
+    c.v[i] = mc_staged(VT::observed(own[r]), VT::sdf(own[r]));
   }
   __syncthreads();
   for (int h = t; h < kMcHalo; h += kMcThreads) {
@@ -146,20 +150,15 @@ __device__ inline void stage_cube(const MapView& m, int blk, int* nb, McCube& c,
     }
     const int b = nb[(x >> 3) | ((y >> 3) << 1) | ((z >> 3) << 2)];
     const size_t at = (size_t)b * 512 + ((z & 7) * 8 + (y & 7)) * 8 + (x & 7);
-    float v = 0.0f;
-    unsigned char o = 0;
+    float v = __uint_as_float(kMcAbsent);
     uint16_t vi = 0;
     if (b >= 0) {
       const typename VT::T w = vox[at];
       if (vinfo) vi = vinfo[at];
-      if (VT::observed(w)) {
-        v = VT::sdf(w);
-        o = 1;
-      }
+      v = mc_staged(VT::observed(w), VT::sdf(w));
     }
     const int i = (z * 9 + y) * 9 + x;
     c.v[i] = v;
-    c.o[i] = o;
     if (vinfo) cvi[i] = vi;
   }
   __syncthreads();
@@ -173,7 +172,7 @@ __device__ inline int read_corners(const McCube& c, int lx, int ly, int lz, floa
   for (int k = 0; k < 8; ++k) {
     const int i = ((lz + kCorner[k][2]) * 9 + (ly + kCorner[k][1])) * 9 + (lx + kCorner[k][0]);
     s[k] = c.v[i];
-    valid |= (int)c.o[i] << k;
+    valid |= (int)(__float_as_uint(s[k]) != kMcAbsent) << k;
   }
   return valid;
 }
