@@ -202,6 +202,29 @@ __device__ inline int owned_edges(int valid, const float* s) {
 
 // Exclusive prefix of v over the threads of a workgroup (thread order), and
 // the workgroup total.  `wsum` holds one int per wavefront.
+// Exclusive prefix of v over a one-wavefront workgroup (the marching-cubes
+// passes run kMcThreads = 64 threads); `total` receives the sum.  No LDS
+// word and no barrier.
+static_assert(kMcThreads == 64, "the marching-cubes passes run one wavefront per block");
+__device__ inline int wave_excl_scan(int v, int& total) {
+  const int lane = threadIdx.x & 63;
+  int inc = v;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const int t = __shfl_up(inc, o, 64);
+    if (lane >= o) inc += t;
+  }
+  total = __builtin_amdgcn_readlane(inc, 63);
+  return inc - v;
+}
+// Triangle counts of the 256 cases (kTriTable's low nibbles) into LDS, for
+// the count pass to read per voxel instead of the table in memory (the
+// emission passes need the whole table word); published by stage_cube's
+// first barrier.
+__device__ inline void stage_tri_counts(unsigned char* nt) {
+  for (int i = threadIdx.x; i < 256; i += kMcThreads) nt[i] = (unsigned char)(kTriTable[i] & 0xF);
+}
+
 __device__ inline int block_excl_scan(int v, int* wsum, int& total) {
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
   int inc = v;
@@ -231,11 +254,12 @@ template <class VT>
 __global__ void __launch_bounds__(kMcThreads) k_mc_count(MapView m, int nblocks, int2* counts, int16_t* cases,
                                                          uint16_t* vinfo, int* nbr) {
   __shared__ int nb[8];
-  __shared__ int wsum[kMcThreads / 64];
+  __shared__ unsigned char tri_n[256];
   __shared__ McCube c;
   const int blk = blockIdx.x;
   if (blk >= nblocks) return;
   int bx, by, bz;
+  stage_tri_counts(tri_n);
   stage_cube<VT>(m, blk, nb, c, bx, by, bz, nullptr, nbr);
   int tri = 0, vert = 0;  // totals of the rounds so far
 #pragma unroll 1
@@ -246,9 +270,9 @@ __global__ void __launch_bounds__(kMcThreads) k_mc_count(MapView m, int nblocks,
     const int cube = cube_index(valid, s);
     const int om = owned_edges(valid, s);
     cases[(size_t)blk * 512 + l] = (int16_t)cube;
-    const int nt = cube > 0 ? (int)(kTriTable[cube] & 0xF) : 0;
+    const int nt = cube > 0 ? (int)tri_n[cube] : 0;
     int total;
-    const int pre = block_excl_scan((nt << 16) | __popc(om), wsum, total);
+    const int pre = wave_excl_scan((nt << 16) | __popc(om), total);
     vinfo[(size_t)blk * 512 + l] = (uint16_t)(((vert + (pre & 0xFFFF)) << 3) | om);
     tri += total >> 16;
     vert += total & 0xFFFF;
@@ -315,7 +339,6 @@ template <class VT>
 __global__ void __launch_bounds__(kMcThreads) k_mc_emit(MapView m, int nblocks, const int2* offsets, float* tris,
                                                         float* normals, int cap, const int* nbr) {
   __shared__ int nb[8];
-  __shared__ int wsum[kMcThreads / 64];
   __shared__ McCube c;
   const int blk = blockIdx.x;
   if (blk >= nblocks) return;
@@ -331,7 +354,7 @@ __global__ void __launch_bounds__(kMcThreads) k_mc_emit(MapView m, int nblocks, 
     const uint64_t word = cube > 0 ? kTriTable[cube] : 0ull;
     const int nt = (int)(word & 0xF);
     int total;
-    int slot = base + block_excl_scan(nt, wsum, total);
+    int slot = base + wave_excl_scan(nt, total);
     base += total;
     const float px = (float)(bx * 8 + lx), py = (float)(by * 8 + ly), pz = (float)(bz * 8 + lz);
     for (int i = 0; i < nt && slot < cap; ++i, ++slot) {
@@ -368,7 +391,6 @@ __global__ void __launch_bounds__(kMcThreads) k_mc_emit_indexed(MapView m, int n
                                                                 const uint16_t* vinfo, float* verts, int vcap,
                                                                 int* tris, int tcap, const int* nbr) {
   __shared__ int nb[8];
-  __shared__ int wsum[kMcThreads / 64];
   __shared__ int vbase[8];
   __shared__ McCube c;
   __shared__ uint16_t cvi[kMcCube];
@@ -377,6 +399,7 @@ __global__ void __launch_bounds__(kMcThreads) k_mc_emit_indexed(MapView m, int n
   int bx, by, bz;
   stage_cube<VT>(m, blk, nb, c, bx, by, bz, nbr, nullptr, vinfo, cvi);
   if (threadIdx.x < 8) vbase[threadIdx.x] = nb[threadIdx.x] >= 0 ? offsets[nb[threadIdx.x]].y : 0;
+  __syncthreads();
   const float vs = m.voxel_size;
   int base = offsets[blk].x;
   const int far[3] = {1, 3, 4};
@@ -389,8 +412,7 @@ __global__ void __launch_bounds__(kMcThreads) k_mc_emit_indexed(MapView m, int n
     const uint64_t word = cube > 0 ? kTriTable[cube] : 0ull;
     const int nt = (int)(word & 0xF);
     int total;
-    // (the scan's barriers also publish vbase)
-    int slot = base + block_excl_scan(nt, wsum, total);
+    int slot = base + wave_excl_scan(nt, total);
     base += total;
     const float px = (float)(bx * 8 + lx), py = (float)(by * 8 + ly), pz = (float)(bz * 8 + lz);
     // owned vertices, interpolated from this voxel towards +x, +y, +z
