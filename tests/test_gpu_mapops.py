@@ -300,3 +300,19 @@ def test_batch_refuses_a_map_reallocated_after_planning():
         b.run()
     b.plan(depth, poses)
     b.run()
+
+
+def test_fuse_refuses_a_depth_image_spanning_4_gib():
+    """The fusion kernels address a depth pixel by a 32-bit byte offset from
+    the image base, so an image whose last pixel lies 4 GiB or more past it is
+    refused before any kernel reads it, and the map is left as it was."""
+    K = np.array([500.0, 500.0, 32.0, 4.0], np.float32)
+    m = rf.MapStruct(K, 1 << 12, 1 << 10, VOX, MU)
+    z = torch.full((8, 64), 1.0, dtype=torch.float32, device="cuda")
+    T = np.ascontiguousarray(np.eye(4), np.float64)
+    for pitch in (1 << 31, (1 << 32) // 7 + 4):  # rows 0..7 reach past 4 GiB
+        with pytest.raises(native.YasError):
+            native.call("yas_map_fuse", m.h, z.data_ptr(), pitch, 64, 8, T.ctypes.data)
+    assert m.stats().num_allocated == 0
+    m.Fuse(z, T)  # the same image at its own pitch
+    assert m.stats().num_allocated > 0
