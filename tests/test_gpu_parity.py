@@ -5,6 +5,7 @@ maps, image pyramids and marching-cubes cases/triangles are bit-exact; the
 tracker's reduced sums are within FP32 reduction-order tolerance of the
 oracle's exact sums, and poses within 1e-4 m / 1e-4 rad.
 """
+import ctypes
 import os
 import subprocess
 
@@ -13,6 +14,7 @@ import pytest
 import torch
 
 from oracle import pyoracle as po
+from yetanotherslam_amd import native
 from yetanotherslam_amd import rgbdfusion as rf
 from yetanotherslam_amd import synth
 
@@ -311,6 +313,34 @@ def test_indexed_mesh_matches_the_soup_and_shares_vertices(fmt):
     assert (uses == 2).mean() > 0.9
     assert len(np.unique(T)) > 0.9 * len(V)  # few unreferenced vertices
 
+
+
+def test_mesh_over_capacity_reports_the_count_and_keeps_the_first_triangles():
+    """Mesh buffers are sized in triangles (the reference's MeshEngine fills
+    a fixed buffer): an extraction that does not fit returns YAS_E_CAPACITY
+    with the full counts, and the triangles (and, indexed, the vertices)
+    that fit are the first ones of the complete extraction, in order."""
+    _, _, _, _, _, gm = fuse_both(FMTS[1])
+    full = rf.MeshEngine(2_000_000)
+    soup = full.Meshify(gm)
+    ix = full.MeshifyIndexed(gm)
+    n, nv = soup["num_triangles"], ix["num_vertices"]
+    assert n > 3000
+    cap = n // 3
+    small = rf.MeshEngine(cap)
+    with pytest.raises(native.YasError) as e:
+        native.call("yas_mesh_extract", small.h, gm.h, ctypes.byref(ctypes.c_int()))
+    assert e.value.code == 6  # YAS_E_CAPACITY
+    part = small.Meshify(gm)
+    assert part["num_triangles"] == n and len(part["tris"]) == cap
+    assert_bits_equal(part["tris"], soup["tris"][:cap])
+    assert_bits_equal(part["normals"], soup["normals"][:cap])
+    pix = small.MeshifyIndexed(gm)
+    assert pix["num_triangles"] == n and pix["num_vertices"] == nv
+    kv = min(nv, 3 * cap)
+    assert len(pix["triangles"]) == cap and len(pix["vertices"]) == kv
+    np.testing.assert_array_equal(pix["triangles"], ix["triangles"][:cap])
+    assert_bits_equal(pix["vertices"], ix["vertices"][:kv])
 
 def test_pipeline_matches_oracle_pipeline():
     """The dense half of Tracking::TrackRGBD over 6 frames: GPU poses within
