@@ -379,6 +379,40 @@ def test_pool_exhaustion_reports_and_keeps_a_subset():
     assert {tuple(k) for k in kg} <= ko
 
 
+def test_a_full_table_reports_the_probe_limit_and_keeps_a_subset():
+    """A table of 64 slots fills before the frame's blocks are all inserted:
+    the probe-limit flag is set (the pool has room), all 64 slots hold
+    blocks, and each is a block the oracle allocates."""
+    p, K, fr = frames(1)
+    z = po.depth_u16_to_f32(fr[0][0])
+    om = po.OracleMap(0.008, 0.024)
+    om.fuse(z, K, np.eye(4))
+    ko = {tuple(k) for k in om.export(with_voxels=False)}
+    assert len(ko) > 64
+    gm = rf.MapStruct(K.astype(np.float32), 64, 1 << 12, 0.008, 0.024)
+    gm.Fuse(to_dev(z), np.eye(4))
+    st = gm.stats()
+    assert st.flags & 2 and not st.flags & 1, st.flags
+    assert st.num_allocated == 64
+    kg, _ = gm.export()
+    assert {tuple(k) for k in kg} <= ko
+
+
+def test_blocks_beyond_the_key_range_are_flagged_and_not_allocated():
+    """Block coordinates are packed in 21 bits each (+-2^20 blocks); with the
+    camera 100 km along x (1.56 M blocks of 6.4 cm) every block of the frame
+    is out of range: the key-range flag is set and nothing is allocated."""
+    p, K, fr = frames(1)
+    z = po.depth_u16_to_f32(fr[0][0])
+    gm = rf.MapStruct(K.astype(np.float32), 1 << 14, 1 << 12, 0.008, 0.024)
+    T = np.eye(4)
+    T[0, 3] = 1.0e5
+    gm.Fuse(to_dev(z), T)
+    st = gm.stats()
+    assert st.flags & 4, st.flags
+    assert st.num_allocated == 0
+
+
 def test_empty_depth_frame_allocates_nothing_and_tracks_identity():
     p, K, fr = frames(1)
     W, H = p.width, p.height
